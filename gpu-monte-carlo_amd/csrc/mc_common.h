@@ -1,0 +1,130 @@
+// Shared definitions for the gfx950 Metropolis + parallel-tempering library.
+//
+// Numerical policy (identical in every kernel):
+//  * float expressions are evaluated in source order without contraction
+//    (the library is compiled with -ffp-contract=off);
+//  * a float32 transcendental is the float64 function rounded once to
+//    float32 (cos, sin, exp, log); sqrt and division are IEEE correctly
+//    rounded; float64 paths call the float64 functions;
+//  * per-site RNG streams are numba's xoroshiro128+ records {s0, s1}.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/mcgpu.h"
+
+namespace mc {
+
+// ------------------------------------------------------------ errors
+void set_error(int code, const char *fmt, ...);
+int fail_hip(hipError_t err, const char *what);
+
+#define MC_TRY_HIP(expr)                                   \
+    do {                                                   \
+        hipError_t e_ = (expr);                            \
+        if (e_ != hipSuccess) return ::mc::fail_hip(e_, #expr); \
+    } while (0)
+
+#define MC_CHECK_LAUNCH() MC_TRY_HIP(hipGetLastError())
+
+#define MC_REQUIRE(cond, ...)                                   \
+    do {                                                        \
+        if (!(cond)) {                                          \
+            ::mc::set_error(MC_ERR_VALUE, __VA_ARGS__);         \
+            return MC_ERR_VALUE;                                \
+        }                                                       \
+    } while (0)
+
+// Library-owned device scratch for reductions; grows on demand, never shrinks.
+// Returns MC_OK and sets *ptr, or an error code.  The first `counter_bytes`
+// of the returned region are zero and kernels leave them zero.
+int scratch(size_t bytes, void **ptr);
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ------------------------------------------------------------- RNG
+// numba.cuda.random xoroshiro128+ (Blackman & Vigna); record = {s0, s1}.
+__device__ __forceinline__ uint64_t rotl64(uint64_t x, unsigned k) {
+    return (x << k) | (x >> (64u - k));
+}
+
+struct Rng {
+    uint64_t s0, s1;
+    __device__ __forceinline__ uint64_t next() {
+        const uint64_t a = s0;
+        uint64_t b = s1;
+        const uint64_t result = a + b;
+        b ^= a;
+        s0 = rotl64(a, 55) ^ b ^ (b << 14);
+        s1 = rotl64(b, 36);
+        return result;
+    }
+    // (x >> 11) * 2^-53 exactly, as float64
+    __device__ __forceinline__ double uniform_f64() {
+        return (double)(next() >> 11) * 0x1.0p-53;
+    }
+    // float64 uniform rounded to nearest float32 (may be 1.0f)
+    __device__ __forceinline__ float uniform_f32() { return (float)uniform_f64(); }
+    template <class R> __device__ __forceinline__ R uniform();
+};
+template <> __device__ __forceinline__ float Rng::uniform<float>() { return uniform_f32(); }
+template <> __device__ __forceinline__ double Rng::uniform<double>() { return uniform_f64(); }
+
+__device__ __forceinline__ Rng load_rng(const uint64_t *states, size_t k) {
+    const ulonglong2 v = reinterpret_cast<const ulonglong2 *>(states)[k];
+    Rng g;
+    g.s0 = v.x;
+    g.s1 = v.y;
+    return g;
+}
+__device__ __forceinline__ void store_rng(uint64_t *states, size_t k, const Rng &g) {
+    ulonglong2 v;
+    v.x = g.s0;
+    v.y = g.s1;
+    reinterpret_cast<ulonglong2 *>(states)[k] = v;
+}
+
+// ---------------------------------------------------- transcendentals
+__device__ __forceinline__ float rcos(float x) { return (float)::cos((double)x); }
+__device__ __forceinline__ float rsin(float x) { return (float)::sin((double)x); }
+__device__ __forceinline__ float rexp(float x) { return (float)::exp((double)x); }
+__device__ __forceinline__ float rlog(float x) { return (float)::log((double)x); }
+__device__ __forceinline__ float rsqrt_(float x) { return ::sqrtf(x); }
+__device__ __forceinline__ double rcos(double x) { return ::cos(x); }
+__device__ __forceinline__ double rsin(double x) { return ::sin(x); }
+__device__ __forceinline__ double rexp(double x) { return ::exp(x); }
+__device__ __forceinline__ double rlog(double x) { return ::log(x); }
+__device__ __forceinline__ double rsqrt_(double x) { return ::sqrt(x); }
+
+template <class R> struct RealConst;
+template <> struct RealConst<float> {
+    static __device__ __forceinline__ float pi() { return 3.14159274101257324f; }       // float32(pi)
+    static __device__ __forceinline__ float two_pi() { return 6.28318548202514648f; }   // float32(2 pi)
+};
+template <> struct RealConst<double> {
+    static __device__ __forceinline__ double pi() { return 3.141592653589793; }
+    static __device__ __forceinline__ double two_pi() { return 6.283185307179586; }
+};
+
+// ------------------------------------------------------------ complex
+template <class R> struct Cpx {
+    R re, im;
+};
+template <class R> __device__ __forceinline__ Cpx<R> cx(R re, R im) { return Cpx<R>{re, im}; }
+template <class R> __device__ __forceinline__ Cpx<R> operator+(Cpx<R> a, Cpx<R> b) { return {a.re + b.re, a.im + b.im}; }
+template <class R> __device__ __forceinline__ Cpx<R> operator-(Cpx<R> a, Cpx<R> b) { return {a.re - b.re, a.im - b.im}; }
+template <class R> __device__ __forceinline__ Cpx<R> operator*(Cpx<R> a, Cpx<R> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <class R> __device__ __forceinline__ Cpx<R> scale(R s, Cpx<R> a) { return {s * a.re, s * a.im}; }
+template <class R> __device__ __forceinline__ Cpx<R> conj(Cpx<R> a) { return {a.re, -a.im}; }
+template <class R> __device__ __forceinline__ Cpx<R> neg(Cpx<R> a) { return {-a.re, -a.im}; }
+template <class R> __device__ __forceinline__ R norm2(Cpx<R> z) { return z.re * z.re + z.im * z.im; }
+
+// periodic neighbour index without a modulo on the hot path
+__device__ __forceinline__ int wrap_up(int i, int L) { return i == 0 ? L - 1 : i - 1; }
+__device__ __forceinline__ int wrap_dn(int i, int L) { return i == L - 1 ? 0 : i + 1; }
+
+}  // namespace mc

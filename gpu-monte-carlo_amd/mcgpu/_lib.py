@@ -1,0 +1,126 @@
+"""ctypes binding of libmcgpu.so (the gfx950 HIP library, see include/mcgpu.h).
+
+There is no fallback: if the shared library is missing or no HIP device is
+visible, the first GPU call raises.
+"""
+
+import ctypes
+import threading
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libmcgpu.so"
+
+MC_OK, MC_ERR_VALUE, MC_ERR_HIP, MC_ERR_NOTIMPL, MC_ERR_TYPE = 0, 1, 2, 3, 4
+MC_F32, MC_F64 = 0, 1
+
+P = ctypes.c_void_p
+I32 = ctypes.c_int32
+I64 = ctypes.c_int64
+U64 = ctypes.c_uint64
+F32 = ctypes.c_float
+F64 = ctypes.c_double
+SZ = ctypes.c_size_t
+INT = ctypes.c_int
+
+# name -> argument types (every function returns int status)
+SIGNATURES = {
+    "mc_device_count": [ctypes.POINTER(INT)],
+    "mc_init": [INT],
+    "mc_malloc": [ctypes.POINTER(P), SZ],
+    "mc_free": [P],
+    "mc_memcpy_htod": [P, P, SZ],
+    "mc_memcpy_dtoh": [P, P, SZ],
+    "mc_memcpy_dtod": [P, P, SZ],
+    "mc_memset": [P, INT, SZ],
+    "mc_synchronize": [],
+    "mc_stream_create": [ctypes.POINTER(P)],
+    "mc_stream_destroy": [P],
+    "mc_stream_synchronize": [P],
+    "mc_rng_init": [P, P, I64, U64, U64],
+    "mc_ising_update_color": [P, P, P, P, INT, INT, INT, F32],
+    "mc_ising_measure": [P, P, INT, INT, F32, P, P, P],
+    "mc_ising_energy": [P, P, INT, INT, F32, P],
+    "mc_xy_update_color": [P, INT, P, P, P, INT, INT, INT, F64, F64],
+    "mc_xy_measure": [P, INT, P, INT, INT, F64, P, P, P],
+    "mc_xy_energy": [P, INT, P, INT, INT, F64, P],
+    "mc_kawamura_update_color": [P, INT, P, P, P, INT, INT, INT, F64, F64],
+    "mc_kawamura_measure": [P, INT, P, INT, INT, F64, P, P, P],
+    "mc_kawamura_energy": [P, INT, P, INT, INT, F64, P],
+    "mc_he3_update_color": [P, INT, P, P, P, INT, INT, INT, P, F64, F64, F64],
+    "mc_he3_gaussian_update_color": [P, INT, P, P, P, INT, INT, INT, P, F64, F64, F64, P, F64],
+    "mc_he3_measure": [P, INT, P, INT, INT, P, F64, F64, P, P, P],
+    "mc_he3_energy": [P, INT, P, INT, INT, P, F64, F64, P],
+    "mc_pt_exchange": [P, INT, P, P, P, P, P, P, P, INT, INT, I64],
+    "mc_pt_record_walkers": [P, P, P, INT, P],
+}
+
+_lock = threading.Lock()
+_lib = None
+_initialised = False
+
+
+class MCError(RuntimeError):
+    """HIP runtime failure inside libmcgpu."""
+
+
+def load():
+    """Load libmcgpu.so and declare its signatures (no device is touched)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise ImportError(
+                    f"{LIB_PATH} is missing: build it with `python -m mcgpu.build` "
+                    "(or __graft_entry__.build()); there is no CPU fallback")
+            lib = ctypes.CDLL(str(LIB_PATH))
+            for name, args in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.argtypes = args
+                fn.restype = INT
+            lib.mc_last_error.argtypes = []
+            lib.mc_last_error.restype = ctypes.c_char_p
+            lib.mc_version.argtypes = []
+            lib.mc_version.restype = ctypes.c_char_p
+            _lib = lib
+    return _lib
+
+
+def check(status):
+    if status == MC_OK:
+        return
+    message = load().mc_last_error().decode(errors="replace")
+    if status == MC_ERR_VALUE:
+        raise ValueError(message)
+    if status == MC_ERR_TYPE:
+        raise TypeError(message)
+    if status == MC_ERR_NOTIMPL:
+        raise NotImplementedError(message)
+    raise MCError(message)
+
+
+def device_count():
+    n = INT(0)
+    check(load().mc_device_count(ctypes.byref(n)))
+    return int(n.value)
+
+
+def ensure_device(device=0):
+    """Select the HIP device once per process; raises if none is visible."""
+    global _initialised
+    if _initialised:
+        return
+    with _lock:
+        if not _initialised:
+            lib = load()
+            if device_count() < 1:
+                raise MCError("no HIP device is visible; libmcgpu requires an AMD GPU")
+            check(lib.mc_init(int(device)))
+            _initialised = True
+
+
+def call(name, *args):
+    """Call a GPU entry point, initialising the device first."""
+    ensure_device()
+    check(getattr(load(), name)(*args))

@@ -1,0 +1,23 @@
+"""Kawamura energy, chirality, doubled phase, spin tensor and twist sums
+(models/kawamura/measure.py of the reference)."""
+
+from mcgpu import _lib, as_pointer, dtype_code
+from models._device import STREAM, energies_pointer, indices_pointer, output_buffer
+
+MEASURE_THREADS = 256
+MEASUREMENT_NAMES = ("energy", "chirality", "px", "py", "spin2", "twist_cos_x", "twist_sin_x")
+
+
+def measure(state, J, temperature_indices=None, cached_energies=None):
+    R, L = state.shape[0], state.shape[1]
+    out = output_buffer(len(MEASUREMENT_NAMES), R)
+    _lib.call("mc_kawamura_measure", STREAM, dtype_code(state.dtype), as_pointer(state), R, L,
+              float(J), indices_pointer(temperature_indices, R),
+              energies_pointer(cached_energies, R), as_pointer(out))
+    return out.copy_to_host()
+
+
+def energy(state, J, output):
+    R, L = state.shape[0], state.shape[1]
+    _lib.call("mc_kawamura_energy", STREAM, dtype_code(state.dtype), as_pointer(state), R, L,
+              float(J), energies_pointer(output, R))

@@ -1,0 +1,154 @@
+/*
+ * mcgpu — C ABI of the gfx950 (MI355X) checkerboard Metropolis +
+ * parallel-tempering engine.
+ *
+ * Plain pointers and sizes only.  Every device pointer is memory obtained
+ * from mc_malloc (or any hipMalloc'd buffer); `stream` is a hipStream_t
+ * (NULL = the default stream).  Every call returns MC_OK or an error code;
+ * mc_last_error() gives the message.  Launches are asynchronous on `stream`;
+ * the mc_memcpy_* calls are synchronous.  Calls on one stream execute in
+ * order; the library is not thread-safe (callers serialise).
+ *
+ * Layouts follow the reference (JinhaoR/GPU-monte-carlo):
+ *   state      (R, L, L[, comps]) C-order; Ising int8, XY real, Kawamura
+ *              5 reals (theta, tau, dx, dy, dz), He3 3x2 complex (re, im)
+ *   rng        R*L*L records {uint64 s0, s1}, stream k = r*L*L + i*L + j
+ *   out        measurement channels (C, R) float64
+ *   dtype      MC_F32 (reference precision) or MC_F64 (float64 mode)
+ * The reference binds none of this through an FFI; its kernels are Numba
+ * launches.  Each entry point names the reference function it replaces.
+ */
+#ifndef MCGPU_H
+#define MCGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MC_OK = 0,
+    MC_ERR_VALUE = 1,      /* invalid argument  -> ValueError */
+    MC_ERR_HIP = 2,        /* HIP runtime error -> RuntimeError */
+    MC_ERR_NOTIMPL = 3,    /* missing path      -> NotImplementedError */
+    MC_ERR_TYPE = 4        /* wrong model/dtype -> TypeError */
+};
+
+enum { MC_F32 = 0, MC_F64 = 1 };
+
+/* ---------------------------------------------------------------- runtime */
+
+/* Message of the last failed call on this thread. */
+const char *mc_last_error(void);
+/* Library version string. */
+const char *mc_version(void);
+/* Number of visible HIP devices (0 when none). */
+int mc_device_count(int *count);
+/* Make `device` current and prepare per-device tables. */
+int mc_init(int device);
+
+/* Device memory: replaces numba cuda.device_array / cuda.to_device /
+ * DeviceNDArray.copy_to_host / copy_to_device (core/runtime.py:109-119,
+ * core/parallel_tempering.py:151,171-188). */
+int mc_malloc(void **ptr, size_t bytes);
+int mc_free(void *ptr);
+int mc_memcpy_htod(void *dst, const void *src, size_t bytes);
+int mc_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int mc_memcpy_dtod(void *dst, const void *src, size_t bytes);
+int mc_memset(void *dst, int value, size_t bytes);
+int mc_synchronize(void);
+int mc_stream_create(void **stream);
+int mc_stream_destroy(void *stream);
+int mc_stream_synchronize(void *stream);
+
+/* ------------------------------------------------------------------- RNG */
+
+/* numba create_xoroshiro128p_states(n, seed, subsequence_start)
+ * (core/runtime.py:113-115, core/parallel_tempering.py:177-179;
+ * numba/cuda/random.py:225-291): state k = jump^(k + subsequence_start)
+ * of splitmix64(seed).  Computed on the GPU in parallel. */
+int mc_rng_init(void *stream, uint64_t *states, int64_t n, uint64_t seed,
+                uint64_t subsequence_start);
+
+/* --------------------------------------------------------------- Ising */
+
+/* models/ising/update.py:7-29 (metropolis_color), models/ising/model.py:19-22 */
+int mc_ising_update_color(void *stream, int8_t *spins, uint64_t *rng, const float *betas,
+                          int R, int L, int color, float J);
+/* models/ising/measure.py:10-68 (energy, magnetization); temperature_indices
+ * and cached_energies may be NULL (the four kernel variants). */
+int mc_ising_measure(void *stream, const int8_t *spins, int R, int L, float J,
+                     const int32_t *temperature_indices, const double *cached_energies,
+                     double *out);
+/* models/ising/measure.py:71-99 (ising_energy_kernel) */
+int mc_ising_energy(void *stream, const int8_t *spins, int R, int L, float J, double *out);
+
+/* ------------------------------------------------------------------ XY */
+
+/* models/xy/update.py:6-54; betas have the state's dtype */
+int mc_xy_update_color(void *stream, int dtype, void *theta, uint64_t *rng, const void *betas,
+                       int R, int L, int color, double J, double delta);
+/* models/xy/measure.py:17-68 (energy, mx, my, twist_cos_x, twist_sin_x) */
+int mc_xy_measure(void *stream, int dtype, const void *theta, int R, int L, double J,
+                  const int32_t *temperature_indices, const double *cached_energies,
+                  double *out);
+/* models/xy/measure.py:71-96 */
+int mc_xy_energy(void *stream, int dtype, const void *theta, int R, int L, double J,
+                 double *out);
+
+/* ------------------------------------------------------------ Kawamura */
+
+/* models/kawamura/update.py:21-65 */
+int mc_kawamura_update_color(void *stream, int dtype, void *state, uint64_t *rng,
+                             const void *betas, int R, int L, int color, double J,
+                             double delta);
+/* models/kawamura/measure.py:14-70 (energy, chirality, px, py, spin2,
+ * twist_cos_x, twist_sin_x) */
+int mc_kawamura_measure(void *stream, int dtype, const void *state, int R, int L, double J,
+                        const int32_t *temperature_indices, const double *cached_energies,
+                        double *out);
+/* models/kawamura/measure.py:71-99 */
+int mc_kawamura_energy(void *stream, int dtype, const void *state, int R, int L, double J,
+                       double *out);
+
+/* ----------------------------------------------------------------- He3
+ * coef = the five GL betas (host array, state precision as double). */
+
+/* models/he3/update.py:176-227 (Cartesian Metropolis) */
+int mc_he3_update_color(void *stream, int dtype, void *state, uint64_t *rng, const void *betas,
+                        int R, int L, int color, const double *coef, double K, double delta,
+                        double g_d);
+/* models/he3/gaussian_fast.py:91-119 (conditional-Gaussian MH); counts is
+ * int64 (3, R, L, L) */
+int mc_he3_gaussian_update_color(void *stream, int dtype, void *state, uint64_t *rng,
+                                 const void *betas, int R, int L, int color,
+                                 const double *coef, double K, double rho, double mass,
+                                 int64_t *counts, double g_d);
+/* models/he3/measure.py:22-89 (10 channels) */
+int mc_he3_measure(void *stream, int dtype, const void *state, int R, int L,
+                   const double *coef, double K, double g_d,
+                   const int32_t *temperature_indices, const double *cached_energies,
+                   double *out);
+/* models/he3/measure.py:115-143 */
+int mc_he3_energy(void *stream, int dtype, const void *state, int R, int L,
+                  const double *coef, double K, double g_d, double *out);
+
+/* ------------------------------------------------ parallel tempering */
+
+/* core/parallel_tempering.py:35-91 (_exchange_kernel): one matching of
+ * neighbour pairs of `parity`, then round-trip bookkeeping.  betas has
+ * `beta_dtype`; counts int64 (3, R-1); trips int64 (5, R). */
+int mc_pt_exchange(void *stream, int beta_dtype, const double *energies, void *betas,
+                   int32_t *walker_to_temperature, int32_t *temperature_to_walker,
+                   uint64_t *pt_rng, int64_t *counts, int64_t *trips, int R, int parity,
+                   int64_t sweep);
+/* core/parallel_tempering.py:28-32 (_record_walkers_kernel) */
+int mc_pt_record_walkers(void *stream, const int32_t *walker_to_temperature,
+                         const int32_t *walker_ids, int n, int32_t *history_row);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
