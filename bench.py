@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""Headline benchmark: site-updates/ns on one MI355X, batched FP32 vs plain FP64.
+
+Workload (one lattice size, the XY benchmark configuration): XY model, J = 1,
+L = 128 periodic, 32 replicas on a geometric temperature ladder 0.7-1.1,
+angles uniform from seed 170, uniform-window proposals (delta = 1.5),
+replica exchange every 10 sweeps.  One step = 10 sweeps + 1 exchange.
+
+* value: the batched FP32 engine (core.batched.BatchedRun, one persistent
+  launch for all timed steps), site-updates per ns of wall-clock time with the
+  state already resident on the GPU.  A site-update is one visit of one site
+  in one colour pass (L*L*R per sweep).
+* plain_fp64: the drop-in path (Runtime + ParallelTempering, float64 XY,
+  four colour launches per sweep from Python) on the same workload.
+* roofline: algorithmic bytes of the batched kernel, 40 B per site-update
+  (float32 angle read + write, 16 B xoroshiro128+ state read + write), over
+  its HIP-event time, against 8 TB/s.
+* cpu_baseline: the CPU oracle's restatement of the reference XY update on
+  the same lattice, a bounded number of sweeps on the host cores.
+
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build measures one GPU.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parent
+for p in (str(REPO / "gpu-monte-carlo_amd"), str(REPO)):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+METRIC = "site-updates/ns on 1 GPU (batched FP32 vs plain FP64) + % HBM roofline"
+L, R, SEED, EXCHANGE = 128, 32, 170, 10
+BYTES_PER_SITE_UPDATE = 40          # XY float32: 2 * 4 B angle + 2 * 16 B RNG state
+HBM_PEAK_GBS = 8000.0
+
+
+def temperatures():
+    return np.geomspace(0.7, 1.1, R)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def bench_batched(steps, warmup, strip_rows, autotune):
+    from core.batched import BatchedRun, autotune_strip_rows
+    from mcgpu import synchronize
+    from mcgpu.timing import EventTimer
+    from models.xy.model import XYModel
+
+    model = XYModel(J=1.0, delta=1.5, start="random")
+    if strip_rows is None and autotune:
+        log("autotuning strip height")
+        strip_rows = autotune_strip_rows(model, L, temperatures(), seed=SEED, exchange_interval=EXCHANGE,
+                                         sweeps=400, verbose=True)
+    run = BatchedRun(model, L, temperatures(), seed=SEED, exchange_interval=EXCHANGE, strip_rows=strip_rows)
+    log(f"batched FP32: strip rows {run.H}, {run.R * run.strips} workgroups; warmup {warmup} steps")
+    run.run(warmup * EXCHANGE)
+    synchronize()
+    timer = EventTimer(run._stream)
+    sweeps = steps * EXCHANGE
+    log(f"timing {steps} steps ({sweeps} sweeps) in one launch")
+    t0 = time.perf_counter()
+    timer.start()
+    handle = run.launch(sweeps)
+    timer.stop()
+    run.collect(handle)
+    synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = timer.elapsed_ms()
+    energy = run.measure()["energy"]
+    if not np.all(np.isfinite(energy)):
+        raise RuntimeError("batched run produced nonfinite energies")
+    site_updates = sweeps * L * L * R
+    return {
+        "wall_s": wall,
+        "kernel_ms": kernel_ms,
+        "site_updates": site_updates,
+        "strip_rows": run.H,
+        "workgroups": run.R * run.strips,
+        "acceptance": run.diagnostics()["acceptance_rates"].mean(),
+    }
+
+
+def bench_plain(dtype, sweeps, warmup=20):
+    from core.parallel_tempering import ParallelTempering
+    from core.runtime import Runtime
+    from mcgpu import synchronize
+    from models.xy.model import XYModel
+
+    model = XYModel(J=1.0, delta=1.5, dtype=dtype, start="random")
+    rt = Runtime(model, L, temperatures(), seed=SEED)
+    pt = ParallelTempering(rt, exchange_interval=EXCHANGE, seed=SEED + 1)
+    for _ in range(warmup):
+        pt.sweep()
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(sweeps):
+        pt.sweep()
+    synchronize()
+    wall = time.perf_counter() - t0
+    return {"wall_s": wall, "site_updates": sweeps * L * L * R, "sweeps": sweeps}
+
+
+def cpu_baseline(target_s):
+    import oracle
+    threads = min(16, os.cpu_count() or 1)
+    oracle.set_threads(threads)
+    from models.xy.model import XYModel
+    theta = XYModel(start="random").initialize_state(L, R, SEED)
+    rng = oracle.rng_init(R * L * L, SEED)
+    betas = (1.0 / temperatures()).astype(np.float32)
+    oracle.xy_sweeps(theta, rng, betas, 1.0, 1.5, 2)  # warm caches and threads
+    sweeps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < target_s:
+        oracle.xy_sweeps(theta, rng, betas, 1.0, 1.5, EXCHANGE)
+        oracle.xy_energy(theta, 1.0)
+        sweeps += EXCHANGE
+    wall = time.perf_counter() - t0
+    return {
+        "value": sweeps * L * L * R / (wall * 1e9),
+        "unit": "site-updates/ns",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{sweeps} sweeps of the reference XY float32 update (C restatement, OpenMP over "
+                   f"replicas) on L={L}, R={R}, with a float64 energy pass every {EXCHANGE} sweeps; "
+                   f"{wall:.1f} s"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3000, help="timed steps (10 sweeps + 1 exchange each)")
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--plain-sweeps", type=int, default=300)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--strip-rows", type=int, default=None)
+    ap.add_argument("--no-autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the engine shards by "
+              "replica/lattice size across GPUs by design but this build runs and reports N=1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from mcgpu.timing import device_info
+    name, cus = device_info()
+    log(f"device {name}, {cus} CUs")
+
+    b = bench_batched(args.steps, args.warmup, args.strip_rows, not args.no_autotune)
+    value = b["site_updates"] / (b["wall_s"] * 1e9)
+    kernel_rate = b["site_updates"] / (b["kernel_ms"] * 1e6)
+    achieved = BYTES_PER_SITE_UPDATE * b["site_updates"] / (b["kernel_ms"] * 1e-3) / 1e9
+    log(f"batched FP32: {value:.2f} site-updates/ns wall, {kernel_rate:.2f} in-kernel")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "site-updates/ns",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(b["wall_s"] * 1e3 / args.steps, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"XY L={L} R={R}, T geometric 0.7-1.1, random start seed {SEED}, delta 1.5, "
+                         f"exchange every {EXCHANGE} sweeps; batched FP32 persistent launch"),
+            "L": L, "replicas": R, "sweeps_per_step": EXCHANGE,
+            "strip_rows": b["strip_rows"], "workgroups": b["workgroups"],
+            "device": name, "compute_units": cus,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_site_update": BYTES_PER_SITE_UPDATE,
+            "kernel_ms": round(b["kernel_ms"], 3),
+            "kernel_site_updates_per_ns": round(kernel_rate, 3),
+        },
+        "acceptance_rate_pt": round(float(b["acceptance"]), 4),
+    }
+    if not args.no_plain:
+        p64 = bench_plain(np.float64, args.plain_sweeps)
+        v64 = p64["site_updates"] / (p64["wall_s"] * 1e9)
+        p32 = bench_plain(np.float32, args.plain_sweeps)
+        v32 = p32["site_updates"] / (p32["wall_s"] * 1e9)
+        log(f"plain FP64 {v64:.3f}, plain FP32 {v32:.3f} site-updates/ns")
+        out["plain_fp64"] = {"value": round(v64, 4), "unit": "site-updates/ns",
+                             "ms_per_sweep": round(p64["wall_s"] * 1e3 / p64["sweeps"], 4),
+                             "path": "Runtime + ParallelTempering, 4 colour launches per sweep"}
+        out["plain_fp32"] = {"value": round(v32, 4), "unit": "site-updates/ns",
+                             "ms_per_sweep": round(p32["wall_s"] * 1e3 / p32["sweeps"], 4)}
+        out["speedup_vs_plain_fp64"] = round(value / v64, 2)
+    if not args.no_cpu:
+        log("CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

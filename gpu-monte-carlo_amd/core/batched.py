@@ -1,0 +1,349 @@
+"""Batched persistent engine: whole runs in one cooperative launch per lattice size.
+
+The drop-in path (Runtime + ParallelTempering + Simulation) launches four
+colour kernels per sweep from Python, like the reference.  ``BatchedRun``
+executes the same schedule -- sweeps (core/runtime.py:87-99), an exchange
+every ``exchange_interval`` sweeps (core/parallel_tempering.py:245-269) and a
+sample every ``sample_interval`` sweeps (core/simulation.py:36-72) -- inside
+one launch of ``mc_batch_run``: each workgroup keeps one strip of H rows of
+one replica on chip for the whole launch.
+
+Models:
+* ``IsingModel``: the reference update; trajectories, exchanges and samples
+  are bit-identical to the drop-in path with the same seeds.
+* ``XYModel(dtype=float32)``: the float32 fast update (dE from cached cos/sin
+  of every site, float32 sincos/exp); energies are compensated float32 sums
+  per thread, float64 across threads and strips.
+
+``BatchedSimulation`` runs several lattice sizes (a finite-size-scaling set)
+concurrently, one stream each, and returns the reference's run dictionaries
+(``{L: {"samples", "metadata"}}``, loadable by utils.data_io).
+"""
+
+import ctypes
+import operator
+import time
+
+import numpy as np
+
+from mcgpu import (_lib, as_pointer, create_xoroshiro128p_states, device_array, device_zeros,
+                   synchronize, to_device)
+
+MODEL_ISING, MODEL_XY_FAST = 0, 1
+CHANNELS = 5
+NAMES = {
+    MODEL_ISING: ("energy", "magnetization"),
+    MODEL_XY_FAST: ("energy", "mx", "my", "twist_cos_x", "twist_sin_x"),
+}
+
+_tuned = {}
+
+
+def model_code(model):
+    from models.ising.model import IsingModel
+    from models.xy.model import XYModel
+    if isinstance(model, IsingModel):
+        return MODEL_ISING
+    if isinstance(model, XYModel):
+        if model.dtype != np.float32:
+            raise TypeError("the batched XY path runs float32 sweeps; use XYModel(dtype=np.float32)")
+        return MODEL_XY_FAST
+    raise TypeError(f"no batched kernel for {type(model).__name__}")
+
+
+def max_workgroups(code, L, R, H, exchange):
+    n = ctypes.c_int(0)
+    _lib.call("mc_batch_max_workgroups", code, L, R, H, int(bool(exchange)), ctypes.byref(n))
+    return int(n.value)
+
+
+def candidate_strip_rows(code, L, R, exchange):
+    """Even divisors H of L whose grid (R * L / H workgroups) can be co-resident."""
+    out = []
+    for H in range(4, L + 1, 2):
+        if L % H or (H // 2) * (L // 2) > 8 * 256:
+            continue
+        if R * (L // H) > max_workgroups(code, L, R, H, exchange):
+            continue
+        out.append(H)
+    return out
+
+
+def default_strip_rows(code, L, R, exchange, target_workgroups=512):
+    """Smallest strip height >= 4 whose grid does not exceed `target_workgroups`."""
+    cands = candidate_strip_rows(code, L, R, exchange)
+    if not cands:
+        raise ValueError(f"no strip height fits L={L}, R={R} on this device")
+    good = [H for H in cands if R * (L // H) <= target_workgroups]
+    return min(good) if good else max(cands)
+
+
+class BatchedRun:
+    """One model, one lattice size, all its replicas: state, streams and PT on the device."""
+
+    def __init__(self, model, L, temperatures, seed=42, exchange_interval=10, pt_seed=None,
+                 strip_rows=None, stream=True):
+        if L < 4 or L % 2 != 0:
+            raise ValueError("L must be even and >= 4")
+        temperatures = np.asarray(temperatures, dtype=np.float64)
+        if temperatures.ndim != 1 or len(temperatures) == 0:
+            raise ValueError("Temperatures must be a nonempty 1D array")
+        if not np.all(np.isfinite(temperatures) & (temperatures > 0)):
+            raise ValueError("Temperatures must be positive and finite")
+        self.code = model_code(model)
+        if isinstance(exchange_interval, (bool, np.bool_)):
+            raise ValueError("exchange_interval must be a nonnegative integer")
+        exchange_interval = operator.index(exchange_interval)
+        if exchange_interval < 0:
+            raise ValueError("exchange_interval must be a nonnegative integer")
+        self.model = model
+        self.L = L
+        self.R = len(temperatures)
+        self.temperatures = temperatures.copy()
+        self.seed = seed
+        self.exchange_interval = exchange_interval
+        betas = (1.0 / temperatures).astype(np.float32)
+        if exchange_interval:
+            if self.R < 2:
+                raise ValueError("Parallel tempering requires at least two temperatures")
+            if not np.all(np.diff(betas) < 0):
+                raise ValueError("PT temperatures must have distinct, finite positive float32 betas")
+
+        state = np.ascontiguousarray(model.initialize_state(L, self.R, seed))
+        if state.shape != (self.R, L, L):
+            raise ValueError("Invalid model state shape")
+        self.state = to_device(state)
+        self.rng_states = create_xoroshiro128p_states(self.R * L * L, seed=seed)
+        self.betas = to_device(betas)
+        self._temperature_betas = betas.copy()
+
+        R = self.R
+        self.exchange_seed = operator.index(seed + 1 if pt_seed is None else pt_seed)
+        ident = np.arange(R, dtype=np.int32)
+        self.walker_to_temperature = to_device(ident)
+        self.temperature_to_walker = to_device(ident)
+        self.energies = device_zeros(R, np.float64)
+        self._pt_rng = create_xoroshiro128p_states(max(R - 1, 1), seed=self.exchange_seed)
+        self._swap_counts = device_zeros((3, max(R - 1, 1)), np.int64)
+        trips = np.zeros((5, R), dtype=np.int64)
+        trips[0] = -1
+        trips[2] = -1
+        trips[0, 0], trips[0, -1] = 0, R - 1
+        trips[2, [0, -1]] = 0
+        self._trips = to_device(trips)
+        self._parity = device_zeros(1, np.int32)
+        self.sweeps = 0
+        self.exchange_rounds = 0
+
+        self.H = int(strip_rows) if strip_rows else default_strip_rows(
+            self.code, L, R, exchange_interval)
+        self._alloc_workspace()
+        self._stream = None
+        if stream:
+            p = ctypes.c_void_p()
+            _lib.check(_lib.load().mc_stream_create(ctypes.byref(p)))
+            self._stream = p
+
+    def _alloc_workspace(self):
+        hf, fw, ed = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
+        _lib.call("mc_batch_workspace_size", self.code, self.L, self.R, self.H, ctypes.byref(hf),
+                  ctypes.byref(fw), ctypes.byref(ed))
+        self._halo = device_array(max(hf.value, 1), np.float32)
+        self._flags = device_zeros(fw.value, np.uint32)
+        self._epart = device_zeros(max(ed.value, 1), np.float64)
+        self._error = device_zeros(1, np.int32)
+        limit = max_workgroups(self.code, self.L, self.R, self.H, self.exchange_interval)
+        if self.R * (self.L // self.H) > limit:
+            raise ValueError(f"grid of {self.R * (self.L // self.H)} workgroups exceeds the "
+                             f"{limit} that can be co-resident; choose a taller strip")
+
+    @property
+    def strips(self):
+        return self.L // self.H
+
+    def __del__(self):
+        s = getattr(self, "_stream", None)
+        if s is not None:
+            try:
+                _lib.load().mc_stream_destroy(s)
+            except Exception:
+                pass
+
+    # ------------------------------------------------------------- running
+    def launch(self, nsweeps, sample_interval=0, sample_first=None):
+        """Queue `nsweeps` sweeps (asynchronous).  Returns a handle for collect()."""
+        nsweeps = int(nsweeps)
+        if nsweeps < 0:
+            raise ValueError("nsweeps must be nonnegative")
+        if sample_interval and sample_interval < 1:
+            raise ValueError("sample_interval must be positive")
+        sweep0 = self.sweeps
+        if sample_interval:
+            if sample_first is None:
+                sample_first = sweep0 + sample_interval
+            last = sweep0 + nsweeps
+            n_samples = 0 if last < sample_first else (last - sample_first) // sample_interval + 1
+        else:
+            sample_first, n_samples = 0, 0
+        S = self.strips
+        partials = device_array((max(n_samples, 1), self.R, S, CHANNELS), np.float64)
+        sw2t = device_array((max(n_samples, 1), self.R), np.int32)
+        _lib.call(
+            "mc_batch_run", self._stream, self.code, self.L, self.R, self.H, as_pointer(self.state),
+            as_pointer(self.rng_states), as_pointer(self.betas), float(self.model.J),
+            float(getattr(self.model, "delta", 0.0)), nsweeps, sweep0, self.exchange_interval,
+            as_pointer(self._parity), as_pointer(self.energies), as_pointer(self.walker_to_temperature),
+            as_pointer(self.temperature_to_walker), as_pointer(self._pt_rng), as_pointer(self._swap_counts),
+            as_pointer(self._trips), int(sample_interval or 0), int(sample_first), n_samples,
+            as_pointer(partials), as_pointer(sw2t), as_pointer(self._halo), as_pointer(self._flags),
+            as_pointer(self._epart), as_pointer(self._error))
+        self.sweeps += nsweeps
+        if self.exchange_interval:
+            self.exchange_rounds += (sweep0 + nsweeps) // self.exchange_interval - sweep0 // self.exchange_interval
+        return (partials, sw2t, n_samples, sample_first, sample_interval)
+
+    def synchronize(self):
+        if self._stream is not None:
+            _lib.check(_lib.load().mc_stream_synchronize(self._stream))
+        else:
+            synchronize()
+        err = self._error.copy_to_host()[0]
+        if err:
+            raise RuntimeError(f"batched launch stopped by its watchdog (code {err}): "
+                               "a workgroup waited too long for a neighbour or the barrier")
+
+    def collect(self, handle):
+        """Wait for a launch and return its samples in temperature order."""
+        partials, sw2t, n, first, interval = handle
+        self.synchronize()
+        names = NAMES[self.code]
+        if n == 0:
+            return {name: np.zeros((0, self.R)) for name in names}, np.zeros(0, dtype=np.int64)
+        per_strip = partials.copy_to_host()[:n]
+        w2t = sw2t.copy_to_host()[:n]
+        walker = per_strip.sum(axis=2)  # (n, R, C), strips added in order
+        out = {}
+        rows = np.arange(n)[:, None]
+        for c, name in enumerate(names):
+            values = np.empty((n, self.R))
+            values[rows, w2t] = walker[:, :, c]
+            out[name] = values
+        return out, first + interval * np.arange(n, dtype=np.int64)
+
+    def run(self, nsweeps, sample_interval=0, sample_first=None):
+        return self.collect(self.launch(nsweeps, sample_interval, sample_first))
+
+    # ---------------------------------------------------------- inspection
+    def snapshot(self):
+        """Host state in walker order."""
+        return self.state.copy_to_host()
+
+    def measure(self):
+        """Observables of the current state through the drop-in measurement kernels,
+        in temperature order."""
+        return self.model.measure(self.state, temperature_indices=self.walker_to_temperature)
+
+    def diagnostics(self):
+        counts = self._swap_counts.copy_to_host()
+        if self.R > 1 and np.any(counts[2]):
+            raise FloatingPointError("Nonfinite energies encountered during replica exchange")
+        trips = self._trips.copy_to_host()
+        attempts, accepted = counts[0, :self.R - 1], counts[1, :self.R - 1]
+        rates = np.divide(accepted, attempts, out=np.zeros(self.R - 1), where=attempts > 0)
+        completed = int(trips[3].sum())
+        return {
+            "exchange_interval": self.exchange_interval,
+            "exchange_seed": self.exchange_seed,
+            "sweeps": self.sweeps,
+            "exchange_rounds": self.exchange_rounds,
+            "next_parity": int(self._parity.copy_to_host()[0]),
+            "diagnostics_scope": "since_wrapper_creation_including_thermalization",
+            "temperature_betas": self._temperature_betas.copy(),
+            "attempts": attempts.copy(),
+            "accepted": accepted.copy(),
+            "acceptance_rates": rates,
+            "walker_to_temperature": self.walker_to_temperature.copy_to_host(),
+            "temperature_to_walker": self.temperature_to_walker.copy_to_host(),
+            "walker_round_trips": trips[3].copy(),
+            "walker_round_trip_total_sweeps": trips[4].copy(),
+            "mean_round_trip_sweeps": float(trips[4].sum()) / completed if completed else None,
+            "strip_rows": self.H,
+        }
+
+
+def autotune_strip_rows(model, L, temperatures, seed=42, exchange_interval=10, sweeps=300,
+                        candidates=None, verbose=False):
+    """Time short runs for each candidate strip height and return the fastest.
+
+    Results are cached per (model type, L, R, exchange) for the process.
+    """
+    code = model_code(model)
+    R = len(temperatures)
+    key = (code, L, R, bool(exchange_interval))
+    if key in _tuned and candidates is None:
+        return _tuned[key]
+    cands = candidates or candidate_strip_rows(code, L, R, exchange_interval)
+    best, best_t = None, None
+    timings = {}
+    for H in cands:
+        run = BatchedRun(model, L, temperatures, seed=seed, exchange_interval=exchange_interval,
+                         strip_rows=H)
+        run.run(20)
+        t0 = time.perf_counter()
+        run.run(sweeps)
+        dt = time.perf_counter() - t0
+        timings[H] = dt
+        if verbose:
+            print(f"  H={H:4d} strips={L // H:4d} {dt / sweeps * 1e6:9.2f} us/sweep", flush=True)
+        if best_t is None or dt < best_t:
+            best, best_t = H, dt
+    _tuned[key] = best
+    return best
+
+
+class BatchedSimulation:
+    """Thermalise and sample several BatchedRuns concurrently (one stream each).
+
+    Mirrors core/simulation.py: ``sample_sweeps = thermalization + step`` for
+    every ``step % sample_interval == 0``; returns ``{L: {"samples", "metadata"}}``.
+    """
+
+    def __init__(self, runs, thermalization_sweeps=1000, measurement_sweeps=4000, sample_interval=10):
+        if thermalization_sweeps < 0:
+            raise ValueError("Thermalization sweeps cannot be negative")
+        if measurement_sweeps < 1:
+            raise ValueError("Measurement sweeps must be positive")
+        if sample_interval < 1 or sample_interval > measurement_sweeps:
+            raise ValueError("Invalid sampling interval")
+        self.runs = list(runs)
+        self.thermalization_sweeps = thermalization_sweeps
+        self.measurement_sweeps = measurement_sweeps
+        self.sample_interval = sample_interval
+
+    def run(self):
+        total = self.thermalization_sweeps + self.measurement_sweeps
+        handles = []
+        for br in self.runs:
+            first = br.sweeps + self.thermalization_sweeps + self.sample_interval
+            handles.append(br.launch(total, self.sample_interval, first))
+        out = {}
+        for br, h in zip(self.runs, handles):
+            samples, sweeps = br.collect(h)
+            metadata = {
+                "L": br.L,
+                "temperatures": br.temperatures.copy(),
+                "seed": br.seed,
+                "model_parameters": br.model.parameters(),
+                "thermalization_sweeps": self.thermalization_sweeps,
+                "measurement_sweeps": self.measurement_sweeps,
+                "sample_interval": self.sample_interval,
+                "sample_sweeps": sweeps - (br.sweeps - total),
+                "engine": "batched",
+            }
+            if br.exchange_interval:
+                metadata["parallel_tempering"] = br.diagnostics()
+            key = br.L
+            while key in out:  # several models at one L: keep all
+                key = (key, len(out))
+            out[key] = {"samples": samples, "metadata": metadata}
+        return out

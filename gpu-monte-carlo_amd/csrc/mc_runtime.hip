@@ -255,6 +255,41 @@ int mc_stream_synchronize(void *stream) {
     return MC_OK;
 }
 
+int mc_event_create(void **event) {
+    hipEvent_t e = nullptr;
+    MC_TRY_HIP(hipEventCreate(&e));
+    *event = e;
+    return MC_OK;
+}
+
+int mc_event_destroy(void *event) {
+    if (event) MC_TRY_HIP(hipEventDestroy(reinterpret_cast<hipEvent_t>(event)));
+    return MC_OK;
+}
+
+int mc_event_record(void *event, void *stream) {
+    MC_TRY_HIP(hipEventRecord(reinterpret_cast<hipEvent_t>(event), as_stream(stream)));
+    return MC_OK;
+}
+
+int mc_event_elapsed_ms(void *start, void *stop, float *ms) {
+    MC_TRY_HIP(hipEventSynchronize(reinterpret_cast<hipEvent_t>(stop)));
+    MC_TRY_HIP(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(start), reinterpret_cast<hipEvent_t>(stop)));
+    return MC_OK;
+}
+
+int mc_device_info(char *name, int name_bytes, int *compute_units) {
+    int dev = 0;
+    MC_TRY_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    MC_TRY_HIP(hipGetDeviceProperties(&prop, dev));
+    if (name && name_bytes > 0) {
+        snprintf(name, (size_t)name_bytes, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (compute_units) *compute_units = prop.multiProcessorCount;
+    return MC_OK;
+}
+
 int mc_rng_init(void *stream, uint64_t *states, int64_t n, uint64_t seed,
                 uint64_t subsequence_start) {
     MC_REQUIRE(n >= 0, "number of RNG states must be nonnegative, got %lld", (long long)n);

@@ -36,6 +36,11 @@ SIGNATURES = {
     "mc_stream_create": [ctypes.POINTER(P)],
     "mc_stream_destroy": [P],
     "mc_stream_synchronize": [P],
+    "mc_event_create": [ctypes.POINTER(P)],
+    "mc_event_destroy": [P],
+    "mc_event_record": [P, P],
+    "mc_event_elapsed_ms": [P, P, ctypes.POINTER(F32)],
+    "mc_device_info": [ctypes.c_char_p, INT, ctypes.POINTER(INT)],
     "mc_rng_init": [P, P, I64, U64, U64],
     "mc_ising_update_color": [P, P, P, P, INT, INT, INT, F32],
     "mc_ising_measure": [P, P, INT, INT, F32, P, P, P],
@@ -52,9 +57,14 @@ SIGNATURES = {
     "mc_he3_energy": [P, INT, P, INT, INT, P, F64, F64, P],
     "mc_pt_exchange": [P, INT, P, P, P, P, P, P, P, INT, INT, I64],
     "mc_pt_record_walkers": [P, P, P, INT, P],
+    "mc_batch_workspace_size": [INT, INT, INT, INT, ctypes.POINTER(SZ), ctypes.POINTER(SZ),
+                                ctypes.POINTER(SZ)],
+    "mc_batch_max_workgroups": [INT, INT, INT, INT, INT, ctypes.POINTER(INT)],
+    "mc_batch_run": [P, INT, INT, INT, INT, P, P, P, F32, F32, INT, I64, INT, P, P, P, P, P, P, P,
+                     INT, I64, INT, P, P, P, P, P, P],
 }
 
-_lock = threading.Lock()
+_lock = threading.RLock()
 _lib = None
 _initialised = False
 
@@ -111,9 +121,9 @@ def ensure_device(device=0):
     global _initialised
     if _initialised:
         return
+    lib = load()
     with _lock:
         if not _initialised:
-            lib = load()
             if device_count() < 1:
                 raise MCError("no HIP device is visible; libmcgpu requires an AMD GPU")
             check(lib.mc_init(int(device)))

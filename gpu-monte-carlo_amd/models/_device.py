@@ -44,8 +44,11 @@ def energies_pointer(energies, replicas):
 
 
 def real_dtype(dtype):
+    """Precision of the betas a state of `dtype` is updated with (int8 Ising uses float32)."""
     dtype = np.dtype(dtype)
-    return np.dtype(np.float32) if dtype in (np.float32, np.complex64) else np.dtype(np.float64)
+    if dtype in (np.float32, np.complex64, np.int8):
+        return np.dtype(np.float32)
+    return np.dtype(np.float64)
 
 
 def check_update_args(state, rng_states, betas, color, per_site=1):
@@ -63,4 +66,5 @@ def check_update_args(state, rng_states, betas, color, per_site=1):
         raise TypeError(f"betas dtype {betas.dtype} does not match state precision {state.dtype}")
     if not 0 <= int(color) < 4:
         raise ValueError("color must be in 0..3")
-    return R, L, dtype_code(state.dtype)
+    code = dtype_code(real_dtype(state.dtype))
+    return R, L, code

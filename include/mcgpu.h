@@ -62,6 +62,15 @@ int mc_synchronize(void);
 int mc_stream_create(void **stream);
 int mc_stream_destroy(void *stream);
 int mc_stream_synchronize(void *stream);
+/* HIP events for timing work on a given stream (no reference counterpart:
+ * the reference times with cuda.synchronize + perf_counter,
+ * profiling/profile_he3.py:112-133). */
+int mc_event_create(void **event);
+int mc_event_destroy(void *event);
+int mc_event_record(void *event, void *stream);
+int mc_event_elapsed_ms(void *start, void *stop, float *ms);
+/* Name and compute-unit count of the current device. */
+int mc_device_info(char *name, int name_bytes, int *compute_units);
 
 /* ------------------------------------------------------------------- RNG */
 
@@ -147,6 +156,28 @@ int mc_pt_exchange(void *stream, int beta_dtype, const double *energies, void *b
 /* core/parallel_tempering.py:28-32 (_record_walkers_kernel) */
 int mc_pt_record_walkers(void *stream, const int32_t *walker_to_temperature,
                          const int32_t *walker_ids, int n, int32_t *history_row);
+
+/* ---------------------------------------------- batched persistent path
+ * No single reference function: this runs core/simulation.py:36-72's loop
+ * (sweeps from core/runtime.py:87-99, exchanges from
+ * core/parallel_tempering.py:245-269, samples every sample_interval) for
+ * one lattice size inside one cooperative launch.  model 0 = Ising (the
+ * reference update, bit-identical to mc_ising_update_color), model 1 = XY
+ * float32 fast update (cached cos/sin, float32 sincos/exp).  The state is
+ * split into L/H strips of H rows per replica; each workgroup keeps its
+ * strip on chip.  Workspace sizes come from mc_batch_workspace_size. */
+int mc_batch_workspace_size(int model, int L, int R, int H, size_t *halo_floats,
+                            size_t *flag_words, size_t *epart_doubles);
+/* Largest grid (R * L / H workgroups) that can be co-resident for this
+ * model and strip height. */
+int mc_batch_max_workgroups(int model, int L, int R, int H, int exchange, int *max_workgroups);
+int mc_batch_run(void *stream, int model, int L, int R, int H, void *state, uint64_t *rng,
+                 float *betas, float J, float delta, int nsweeps, long long sweep0,
+                 int exchange_interval, int *parity, double *energies, int32_t *walker_to_temperature,
+                 int32_t *temperature_to_walker, uint64_t *pt_rng, long long *counts,
+                 long long *trips, int sample_interval, long long sample_first, int max_samples,
+                 double *sample_partials, int32_t *sample_w2t, float *halo, unsigned *flags,
+                 double *epart, int *error);
 
 #ifdef __cplusplus
 }

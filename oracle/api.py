@@ -25,7 +25,8 @@ __all__ = [
     "kawamura_energy",
     "he3_potential", "he3_gradient_difference", "he3_local_energy", "he3_color",
     "he3_sweeps", "he3_gaussian_color", "he3_measure", "he3_energy",
-    "pt_exchange",
+    "pt_exchange", "fast_sincosf", "fast_expf", "xyfast_cache", "xyfast_color",
+    "xyfast_sweeps", "xyfast_observables",
 ]
 
 
@@ -100,6 +101,12 @@ def _declare(lib):
         "orc_he3_energy_f32": (None, [P, I32, I32, P, F32, F32, P]),
         "orc_he3_energy_f64": (None, [P, I32, I32, P, F64, F64, P]),
         "orc_pt_exchange_f32": (None, [P, P, P, P, P, P, P, I32, I32, I64]),
+        "orc_fast_sincosf": (None, [F32, P, P]),
+        "orc_fast_expf": (F32, [F32]),
+        "orc_xyfast_init_cache": (None, [P, I64, P, P]),
+        "orc_xyfast_color": (None, [P, P, P, P, P, I32, I32, I32, F32, F32]),
+        "orc_xyfast_sweeps": (None, [P, P, P, P, P, I32, I32, F32, F32, I64]),
+        "orc_xyfast_observables": (None, [P, P, I32, I32, F32, P]),
         "orc_pt_exchange_f64": (None, [P, P, P, P, P, P, P, I32, I32, I64]),
     }
     for name, (res, args) in sig.items():
@@ -374,3 +381,55 @@ def pt_exchange(energies, betas, w2t, t2w, pt_rng, counts, trips, parity, sweep)
         _ptr(energies, np.float64), _ptr(betas, betas.dtype, True), _ptr(w2t, np.int32, True),
         _ptr(t2w, np.int32, True), _rng(pt_rng), _ptr(counts, np.int64, True),
         _ptr(trips, np.int64, True), R, int(parity), int(sweep))
+
+
+# ------------------------------------------------ batched float32 fast path
+
+def fast_sincosf(x):
+    """Vectorised wrapper: (sin, cos) arrays from the oracle's float32 routine."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    sn = np.empty_like(x)
+    cs = np.empty_like(x)
+    f = load().orc_fast_sincosf
+    fs, fc = sn.reshape(-1), cs.reshape(-1)
+    for k, v in enumerate(x.reshape(-1)):
+        a, b = F32(), F32()
+        f(float(v), ctypes.byref(a), ctypes.byref(b))
+        fs[k], fc[k] = a.value, b.value
+    return sn, cs
+
+
+def fast_expf(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    f = load().orc_fast_expf
+    return np.array([f(float(v)) for v in x.reshape(-1)], dtype=np.float32).reshape(x.shape)
+
+
+def xyfast_cache(theta):
+    """(cos, sin) cache arrays of the fast XY update."""
+    c = np.empty_like(theta, dtype=np.float32)
+    s = np.empty_like(theta, dtype=np.float32)
+    load().orc_xyfast_init_cache(_ptr(theta, np.float32), theta.size, c.ctypes.data, s.ctypes.data)
+    return c, s
+
+
+def xyfast_color(theta, c, s, rng, betas, color, J, delta):
+    R, L = theta.shape[0], theta.shape[1]
+    load().orc_xyfast_color(_ptr(theta, np.float32, True), _ptr(c, np.float32, True),
+                            _ptr(s, np.float32, True), _rng(rng), _ptr(betas, np.float32), R, L,
+                            int(color), float(J), float(delta))
+
+
+def xyfast_sweeps(theta, c, s, rng, betas, J, delta, nsweeps):
+    R, L = theta.shape[0], theta.shape[1]
+    load().orc_xyfast_sweeps(_ptr(theta, np.float32, True), _ptr(c, np.float32, True),
+                             _ptr(s, np.float32, True), _rng(rng), _ptr(betas, np.float32), R, L,
+                             float(J), float(delta), int(nsweeps))
+
+
+def xyfast_observables(c, s, J):
+    R, L = c.shape[0], c.shape[1]
+    out = np.zeros((5, R), dtype=np.float64)
+    load().orc_xyfast_observables(_ptr(c, np.float32), _ptr(s, np.float32), R, L, float(J),
+                                  out.ctypes.data)
+    return out

@@ -126,6 +126,16 @@ void orc_pt_exchange_f64(const double *energies, double *betas, int32_t *w2t, in
                          uint64_t *pt_rng, int64_t *counts, int64_t *trips, int R,
                          int parity, int64_t sweep);
 
+/* ---- batched float32 fast path (mc_oracle_fast.c) ---- */
+void orc_fast_sincosf(float x, float *sn, float *cs);
+float orc_fast_expf(float x);
+void orc_xyfast_init_cache(const float *th, int64_t n, float *c, float *s);
+void orc_xyfast_color(float *th, float *c, float *s, uint64_t *rng, const float *betas, int R, int L,
+                      int color, float J, float delta);
+void orc_xyfast_sweeps(float *th, float *c, float *s, uint64_t *rng, const float *betas, int R, int L,
+                       float J, float delta, int64_t nsweeps);
+void orc_xyfast_observables(const float *c, const float *s, int R, int L, float J, double *out);
+
 #ifdef __cplusplus
 }
 #endif

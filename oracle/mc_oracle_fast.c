@@ -1,0 +1,157 @@
+/*
+ * CPU oracle for the batched float32 fast path (test infrastructure only).
+ *
+ * The fast XY update differs from the reference formula only in how it
+ * evaluates dE:
+ *   E_loc(x) = -J sum_k cos(x - theta_k) = -J (cos x * C + sin x * S),
+ *   C = sum_k cos theta_k, S = sum_k sin theta_k,
+ * with every site's (cos theta, sin theta) cached and refreshed on
+ * acceptance, and with the float32 functions below in place of libm.  The
+ * proposal, draws, wrap and acceptance rule are those of
+ * models/xy/update.py:6-54.
+ *
+ * fast_sincosf / fast_expf are written from the mathematics: Cody-Waite
+ * reduction by pi/2 (three-part split) or ln 2 (two-part split) with fused
+ * multiply-adds, then the Taylor series of sin, cos (to x^9, x^10) and exp (to
+ * x^7) in Horner form.  The HIP kernels evaluate the same operations in the
+ * same order, so both produce identical bits.
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "mc_oracle.h"
+
+static const float PIO2_A = 0x1.921fb6p+0f;
+static const float PIO2_B = -0x1.777a5cp-25f;
+static const float PIO2_C = -0x1p-49f;
+static const float TWO_OVER_PI = 0x1.45f306p-1f;
+static const float S1 = -0x1.555556p-3f, S2 = 0x1.111112p-7f, S3 = -0x1.a01a02p-13f, S4 = 0x1.71de3ap-19f;
+static const float C1 = -0x1p-1f, C2 = 0x1.555556p-5f, C3 = -0x1.6c16c2p-10f, C4 = 0x1.a01a02p-16f,
+                   C5 = -0x1.27e4fcp-22f;
+static const float LOG2E = 0x1.715476p+0f;
+static const float LN2_HI = 0x1.62e430p-1f, LN2_LO = -0x1.05c610p-29f;
+static const float E2 = 0x1p-1f, E3 = 0x1.555556p-3f, E4 = 0x1.555556p-5f, E5 = 0x1.111112p-7f,
+                   E6 = 0x1.6c16c2p-10f, E7 = 0x1.a01a02p-13f;
+
+/* valid for |x| <= 2^20; the engine only calls it on angles in [-2 pi, 2 pi] */
+void orc_fast_sincosf(float x, float *sn, float *cs) {
+    const float k = rintf(x * TWO_OVER_PI);
+    float r = fmaf(-k, PIO2_A, x);
+    r = fmaf(-k, PIO2_B, r);
+    r = fmaf(-k, PIO2_C, r);
+    const float r2 = r * r;
+    float ps = fmaf(r2, S4, S3);
+    ps = fmaf(r2, ps, S2);
+    ps = fmaf(r2, ps, S1);
+    const float s = fmaf(r * r2, ps, r);
+    float pc = fmaf(r2, C5, C4);
+    pc = fmaf(r2, pc, C3);
+    pc = fmaf(r2, pc, C2);
+    pc = fmaf(r2, pc, C1);
+    const float c = fmaf(r2, pc, 1.0f);
+    const int q = ((int)k) & 3;
+    if (q == 0) { *sn = s; *cs = c; }
+    else if (q == 1) { *sn = c; *cs = -s; }
+    else if (q == 2) { *sn = -s; *cs = -c; }
+    else { *sn = -c; *cs = s; }
+}
+
+/* exp for x <= 88; NaN propagates, x below -104 gives 0 */
+float orc_fast_expf(float x) {
+    if (!(x > -104.0f)) return x != x ? x : 0.0f;
+    const float k = rintf(x * LOG2E);
+    float r = fmaf(-k, LN2_HI, x);
+    r = fmaf(-k, LN2_LO, r);
+    float p = fmaf(r, E7, E6);
+    p = fmaf(p, r, E5);
+    p = fmaf(p, r, E4);
+    p = fmaf(p, r, E3);
+    p = fmaf(p, r, E2);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
+    int e = (int)k;
+    if (e < -125) { /* two exact power-of-two scalings keep 2^e normal */
+        p *= 0x1p-64f;
+        e += 64;
+    }
+    union { uint32_t u; float f; } two_e;
+    two_e.u = (uint32_t)(e + 127) << 23;
+    return p * two_e.f;
+}
+
+void orc_xyfast_init_cache(const float *th, int64_t n, float *c, float *s) {
+    for (int64_t k = 0; k < n; k++) orc_fast_sincosf(th[k], s + k, c + k);
+}
+
+#define FSITE(L, r, i, j) ((size_t)(r) * (size_t)(L) * (size_t)(L) + (size_t)(i) * (size_t)(L) + (size_t)(j))
+
+void orc_xyfast_color(float *th, float *c, float *s, uint64_t *rng, const float *betas, int R, int L,
+                      int color, float J, float delta) {
+    const float one = 1.0f, two = 2.0f;
+    const float pi = (float)M_PI, two_pi = (float)(2.0 * M_PI);
+    const int h = L / 2;
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < R; r++) {
+        for (int row = 0; row < h; row++) {
+            for (int col = 0; col < h; col++) {
+                const int i = 2 * row + color / 2, j = 2 * col + color % 2;
+                const size_t k = FSITE(L, r, i, j);
+                uint64_t *st = rng + 2 * k;
+                const float theta = th[k];
+                const float u = orc_uniform_f32(st);
+                float p = theta + delta * (two * u - one);
+                if (p >= pi) p -= two_pi;
+                else if (p < -pi) p += two_pi;
+                float sp, cp;
+                orc_fast_sincosf(p, &sp, &cp);
+                const size_t ku = FSITE(L, r, (i + L - 1) % L, j), kd = FSITE(L, r, (i + 1) % L, j);
+                const size_t kl = FSITE(L, r, i, (j + L - 1) % L), kr = FSITE(L, r, i, (j + 1) % L);
+                const float C = ((c[ku] + c[kd]) + c[kl]) + c[kr];
+                const float S = ((s[ku] + s[kd]) + s[kl]) + s[kr];
+                float t = (cp - c[k]) * C;
+                t = fmaf(sp - s[k], S, t);
+                const float dE = -J * t;
+                const float v = orc_uniform_f32(st);
+                if (dE <= 0.0f || v < orc_fast_expf(-betas[r] * dE)) {
+                    th[k] = p;
+                    c[k] = cp;
+                    s[k] = sp;
+                }
+            }
+        }
+    }
+}
+
+void orc_xyfast_sweeps(float *th, float *c, float *s, uint64_t *rng, const float *betas, int R, int L,
+                       float J, float delta, int64_t nsweeps) {
+    for (int64_t n = 0; n < nsweeps; n++)
+        for (int col = 0; col < 4; col++) orc_xyfast_color(th, c, s, rng, betas, R, L, col, J, delta);
+}
+
+/* energy, mx, my, twist_cos_x, twist_sin_x from the cache; per-site terms in
+ * float32 exactly as the kernel forms them, sums in float64 */
+void orc_xyfast_observables(const float *c, const float *s, int R, int L, float J, double *out) {
+    for (int r = 0; r < R; r++) {
+        double e = 0, mx = 0, my = 0, cx = 0, qx = 0;
+        for (int i = 0; i < L; i++)
+            for (int j = 0; j < L; j++) {
+                const size_t k = FSITE(L, r, i, j);
+                const size_t kd = FSITE(L, r, (i + 1) % L, j), kr = FSITE(L, r, i, (j + 1) % L);
+                const float bxc = fmaf(s[k], s[kd], c[k] * c[kd]);
+                const float byc = fmaf(s[k], s[kr], c[k] * c[kr]);
+                const float bxs = fmaf(s[k], c[kd], -(c[k] * s[kd]));
+                e += (double)(-J * (bxc + byc));
+                mx += c[k];
+                my += s[k];
+                cx += bxc;
+                qx += bxs;
+            }
+        out[0 * R + r] = e;
+        out[1 * R + r] = mx;
+        out[2 * R + r] = my;
+        out[3 * R + r] = cx;
+        out[4 * R + r] = qx;
+    }
+}

@@ -1,0 +1,153 @@
+"""GPU parity of the persistent batched path.
+
+* Ising: trajectories, RNG streams, betas, maps, swap counts, round trips and
+  samples are bit-identical to the drop-in path (Runtime + ParallelTempering
+  + Simulation) and to the CPU oracle, for one and several strips per replica.
+* XY float32 fast update: trajectories and RNG streams bit-identical to the
+  oracle's fast restatement; observables match it to float32 accuracy.
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from gpu_helpers import device_rng_as_pairs
+
+pytestmark = pytest.mark.gpu
+
+
+def _plain_ising(L, temps, seed, interval, pt_seed, nsweeps, start="random"):
+    from core.parallel_tempering import ParallelTempering
+    from core.runtime import Runtime
+    from models.ising.model import IsingModel
+    rt = Runtime(IsingModel(start=start), L, temps, seed=seed)
+    pt = ParallelTempering(rt, exchange_interval=interval, seed=pt_seed)
+    for _ in range(nsweeps):
+        pt.sweep()
+    return rt, pt
+
+
+@pytest.mark.parametrize("L,R,H", [(8, 2, 8), (16, 5, 4), (32, 8, 32), (32, 8, 8), (64, 3, 16)])
+def test_batched_ising_matches_drop_in_path_bit_for_bit(gpu, L, R, H):
+    from core.batched import BatchedRun
+    from models.ising.model import IsingModel
+    temps = np.linspace(2.0, 2.6, R)
+    seed, pt_seed, interval, n = 40 + L, 43, 3, 37
+    br = BatchedRun(IsingModel(start="random"), L, temps, seed=seed, exchange_interval=interval,
+                    pt_seed=pt_seed, strip_rows=H)
+    br.run(n)
+    rt, pt = _plain_ising(L, temps, seed, interval, pt_seed, n)
+    np.testing.assert_array_equal(br.snapshot(), rt.snapshot())
+    np.testing.assert_array_equal(device_rng_as_pairs(br.rng_states), device_rng_as_pairs(rt.rng_states))
+    np.testing.assert_array_equal(br.betas.copy_to_host(), rt.betas.copy_to_host())
+    d_b, d_p = br.diagnostics(), pt.diagnostics()
+    for key in ("attempts", "accepted", "walker_to_temperature", "temperature_to_walker",
+                "walker_round_trips", "walker_round_trip_total_sweeps"):
+        np.testing.assert_array_equal(d_b[key], d_p[key], err_msg=key)
+    assert d_b["exchange_rounds"] == d_p["exchange_rounds"]
+    assert d_b["next_parity"] == d_p["next_parity"]
+    np.testing.assert_array_equal(br.energies.copy_to_host(), pt.energies.copy_to_host())
+
+
+def test_batched_ising_split_launches_continue_the_same_chain(gpu):
+    from core.batched import BatchedRun
+    from models.ising.model import IsingModel
+    temps = np.linspace(2.1, 2.45, 6)
+    one = BatchedRun(IsingModel(start="random"), 16, temps, seed=3, exchange_interval=2, strip_rows=4)
+    one.run(30)
+    two = BatchedRun(IsingModel(start="random"), 16, temps, seed=3, exchange_interval=2, strip_rows=4)
+    two.run(7)
+    two.run(23)
+    np.testing.assert_array_equal(one.snapshot(), two.snapshot())
+    np.testing.assert_array_equal(one.betas.copy_to_host(), two.betas.copy_to_host())
+
+
+def test_batched_ising_simulation_samples_equal_drop_in_simulation(gpu):
+    from core.batched import BatchedRun, BatchedSimulation
+    from core.simulation import Simulation
+    from models.ising.model import IsingModel
+    L, temps = 16, np.linspace(2.0, 2.6, 4)
+    br = BatchedRun(IsingModel(start="random"), L, temps, seed=58, exchange_interval=1, pt_seed=43,
+                    strip_rows=8)
+    batched = BatchedSimulation([br], thermalization_sweeps=20, measurement_sweeps=60,
+                                sample_interval=5).run()[L]
+    rt, pt = _plain_ising(L, temps, 58, 1, 43, 0)
+    plain = Simulation(pt, thermalization_sweeps=20, measurement_sweeps=60, sample_interval=5).run()
+    np.testing.assert_array_equal(batched["metadata"]["sample_sweeps"], plain["metadata"]["sample_sweeps"])
+    for name in ("energy", "magnetization"):
+        np.testing.assert_array_equal(batched["samples"][name], plain["samples"][name], err_msg=name)
+
+
+@pytest.mark.parametrize("L,R,H", [(16, 3, 16), (32, 4, 8), (128, 2, 16)])
+def test_batched_xy_fast_trajectory_matches_oracle(gpu, L, R, H):
+    from core.batched import BatchedRun
+    from models.xy.model import XYModel
+    temps = np.geomspace(0.7, 1.1, R)
+    model = XYModel(start="random")
+    br = BatchedRun(model, L, temps, seed=170, exchange_interval=0, strip_rows=H)
+    theta = model.initialize_state(L, R, 170).copy()
+    c, s = oracle.xyfast_cache(theta)
+    rng = oracle.rng_init(R * L * L, 170)
+    betas = (1.0 / temps).astype(np.float32)
+    samples, _ = br.run(24, sample_interval=8)
+    oracle.xyfast_sweeps(theta, c, s, rng, betas, model.J, model.delta, 24)
+    np.testing.assert_array_equal(br.snapshot(), theta)
+    np.testing.assert_array_equal(device_rng_as_pairs(br.rng_states), rng)
+    ref = oracle.xyfast_observables(c, s, model.J)
+    for k, name in enumerate(("energy", "mx", "my", "twist_cos_x", "twist_sin_x")):
+        np.testing.assert_allclose(samples[name][-1], ref[k], rtol=2e-6, atol=2e-3 * L, err_msg=name)
+    # the cached-sincos energy agrees with the float64 reference measurement
+    plain = model.measure(br.state)
+    np.testing.assert_allclose(samples["energy"][-1], plain["energy"], rtol=1e-5, atol=1e-3 * L)
+
+
+def test_batched_xy_fast_with_exchanges_matches_oracle_loop(gpu):
+    from core.batched import BatchedRun
+    from models.xy.model import XYModel
+    L, R, interval = 32, 6, 5
+    temps = np.geomspace(0.7, 1.1, R)
+    model = XYModel(start="random")
+    br = BatchedRun(model, L, temps, seed=11, exchange_interval=interval, pt_seed=12, strip_rows=8)
+    br.run(40)
+    theta = model.initialize_state(L, R, 11).copy()
+    c, s = oracle.xyfast_cache(theta)
+    rng = oracle.rng_init(R * L * L, 11)
+    betas = (1.0 / temps).astype(np.float32)
+    w2t = np.arange(R, dtype=np.int32)
+    t2w = np.arange(R, dtype=np.int32)
+    counts = np.zeros((3, R - 1), np.int64)
+    trips = np.zeros((5, R), np.int64)
+    trips[0] = trips[2] = -1
+    trips[0, 0], trips[0, -1] = 0, R - 1
+    trips[2, [0, -1]] = 0
+    pt_rng = oracle.rng_init(R - 1, 12)
+    parity = 0
+    for sweep in range(1, 41):
+        oracle.xyfast_sweeps(theta, c, s, rng, betas, model.J, model.delta, 1)
+        if sweep % interval == 0:
+            e = oracle.xyfast_observables(c, s, model.J)[0]
+            oracle.pt_exchange(e, betas, w2t, t2w, pt_rng, counts, trips, parity, sweep)
+            parity = 1 - parity
+    np.testing.assert_array_equal(br.snapshot(), theta)
+    np.testing.assert_array_equal(br.betas.copy_to_host(), betas)
+    d = br.diagnostics()
+    np.testing.assert_array_equal(d["attempts"], counts[0])
+    np.testing.assert_array_equal(d["accepted"], counts[1])
+    np.testing.assert_array_equal(d["walker_round_trips"], trips[3])
+
+
+def test_batched_simulation_runs_several_sizes_concurrently(gpu):
+    from core.batched import BatchedRun, BatchedSimulation
+    from models.ising.model import IsingModel
+    from models.xy.model import XYModel
+    runs = [BatchedRun(IsingModel(), L, np.linspace(2.1, 2.45, 6), seed=42 + L, exchange_interval=10)
+            for L in (16, 24, 32)]
+    runs.append(BatchedRun(XYModel(), 32, np.linspace(0.8, 1.0, 6), seed=74, exchange_interval=10))
+    out = BatchedSimulation(runs, thermalization_sweeps=50, measurement_sweeps=100,
+                            sample_interval=10).run()
+    assert len(out) == 4
+    for run in out.values():
+        for values in run["samples"].values():
+            assert values.shape == (10, 6)
+            assert np.all(np.isfinite(values))
+        np.testing.assert_array_equal(run["metadata"]["sample_sweeps"], np.arange(60, 151, 10))
