@@ -14,7 +14,10 @@ replica exchange every 10 sweeps.  One step = 10 sweeps + 1 exchange.
   four colour launches per sweep from Python) on the same workload.
 * roofline: algorithmic bytes of the batched kernel, 40 B per site-update
   (float32 angle read + write, 16 B xoroshiro128+ state read + write), over
-  its HIP-event time, against 8 TB/s.
+  its HIP-event time, against 8 TB/s.  `traffic` is the HBM bytes of the timed
+  launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE measurement
+  (per sweep, scaled by a copy kernel of known size; profiles/) times the
+  launch's sweeps, when that measurement used the same strip height.
 * cpu_baseline: the CPU oracle's restatement of the reference XY update on
   the same lattice, a bounded number of sweeps on the host cores.
 
@@ -39,6 +42,7 @@ METRIC = "site-updates/ns on 1 GPU (batched FP32 vs plain FP64) + % HBM roofline
 L, R, SEED, EXCHANGE = 128, 32, 170, 10
 BYTES_PER_SITE_UPDATE = 40          # XY float32: 2 * 4 B angle + 2 * 16 B RNG state
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = REPO / "profiles" / "batch_kernel_xy_L128_R32.traffic.json"
 
 
 def temperatures():
@@ -109,6 +113,29 @@ def bench_plain(dtype, sweeps, warmup=20):
     return {"wall_s": wall, "site_updates": sweeps * L * L * R, "sweeps": sweeps}
 
 
+def copy_probe(mib):
+    """One device-to-device copy of `mib` MiB: reads and writes exactly mib MiB."""
+    from mcgpu import _lib, as_pointer, device_zeros, synchronize
+    n = mib * (1 << 20)
+    src = device_zeros(n, np.uint8)
+    dst = device_zeros(n, np.uint8)
+    synchronize()
+    _lib.call("mc_memcpy_dtod", as_pointer(dst), as_pointer(src), n)
+    synchronize()
+    log(f"copy probe: {mib} MiB device-to-device")
+
+
+def measured_traffic(strip_rows, sweeps):
+    """HBM bytes of a launch of `sweeps` sweeps from the committed counter runs, or None."""
+    try:
+        t = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None, None
+    if int(t.get("strip_rows", -1)) != strip_rows:
+        return None, None
+    return float(t["traffic_bytes_per_sweep"]) * sweeps, t.get("summary")
+
+
 def cpu_baseline(target_s):
     import oracle
     threads = min(16, os.cpu_count() or 1)
@@ -147,6 +174,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--strip-rows", type=int, default=None)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device once, so a counter run holds a "
+                         "plain copy kernel of known size to scale its byte counters by")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the engine shards by "
@@ -157,12 +187,15 @@ def main():
     from mcgpu.timing import device_info
     name, cus = device_info()
     log(f"device {name}, {cus} CUs")
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
 
     b = bench_batched(args.steps, args.warmup, args.strip_rows, not args.no_autotune)
     value = b["site_updates"] / (b["wall_s"] * 1e9)
     kernel_rate = b["site_updates"] / (b["kernel_ms"] * 1e6)
     achieved = BYTES_PER_SITE_UPDATE * b["site_updates"] / (b["kernel_ms"] * 1e-3) / 1e9
     log(f"batched FP32: {value:.2f} site-updates/ns wall, {kernel_rate:.2f} in-kernel")
+    traffic, traffic_src = measured_traffic(b["strip_rows"], args.steps * EXCHANGE)
 
     out = {
         "metric": METRIC,
@@ -190,7 +223,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": None if traffic is None else round(traffic),
+            "traffic_source": traffic_src,
+            "algorithmic_bytes": BYTES_PER_SITE_UPDATE * b["site_updates"],
             "bytes_per_site_update": BYTES_PER_SITE_UPDATE,
             "kernel_ms": round(b["kernel_ms"], 3),
             "kernel_site_updates_per_ns": round(kernel_rate, 3),

@@ -3,17 +3,29 @@
 //
 // Decomposition: each replica lattice (L x L) is cut into S = L / H strips of
 // H full rows; one 256-thread workgroup owns one strip for the whole launch.
-// Everything the strip needs lives in LDS: the RNG streams and state of its
-// own rows, and the data neighbours read (cos/sin of the angle, or the int8
-// spin) for its rows plus one halo row above and below.  Rows are stored
-// colour-split (even columns, then odd columns) so every access of a colour
-// step is unit-stride.  Global memory is touched only at launch start/end,
-// for the halo rows (two per strip per sweep) and for exchanges and samples.
+// Everything the strip needs lives in LDS, rows stored colour-split (even
+// columns, then odd columns) so every access of a colour step is unit-stride:
+//   local row 0      bottom row of the strip above (received)
+//   rows 1..H        own rows: state, RNG streams, cos/sin (or spins)
+//   row H+1          ghost: the top row of the strip below, with a copy of its
+//                    RNG streams, updated here in lock-step with its owner
+//   row H+2          second row of the strip below (received)
+// Global memory is touched only at launch start/end, for the two received
+// rows per sweep and for exchanges and samples.
 //
-// Per sweep a strip hands two halo rows to its ring neighbours through global
-// memory and per-strip flags: its top row after colours 0/1, its bottom row
-// after colours 2/3.  Interior rows are updated before each wait (see the
-// schedule in the sweep loop), so the hand-off latency overlaps with work.
+// The ghost row removes every dependence on a neighbour's current sweep: the
+// top row of strip s+1 depends on the bottom row of strip s (here) and on row
+// 2 of strip s+1 from the previous sweep, and it is all that the bottom row
+// of strip s needs from below.  Its update repeats the owner's operations on
+// the same inputs and streams, so both copies agree bit for bit.  So a strip
+// only waits for data of the previous sweep -- the bottom row of s-1 and row 2
+// of s+1 -- published at the end of that sweep and received after two colour
+// passes of interior rows, which hide the hand-off latency.  A strip can run
+// at most one sweep ahead of its neighbours, so halo slots are double-buffered
+// by sweep parity.  Each halo word carries its payload (angle or spin) and the
+// sweep it belongs to, so the receiver polls the data itself: one L2 round
+// trip per hand-off, no flags.
+//
 // Strips of one replica are placed on one XCD.  Every `exchange_interval`
 // sweeps the strips publish energy partials and meet at a grid barrier; every
 // workgroup then evaluates the same exchange matching on identical inputs
@@ -28,6 +40,8 @@
 //    per-colour kernels (thresholds exp(-beta dE) recomputed per beta change).
 //  * MODEL_XY_FAST: the XY update with dE from cached (cos, sin) of every site
 //    and the bit-reproducible float32 sincos/exp of mc_fastmath.h.
+#include <algorithm>
+
 #include "mc_common.h"
 #include "mc_fastmath.h"
 
@@ -63,8 +77,8 @@ struct BatchArgs {
     double *sample_partials;    // [max_samples][R][S][kChannels]
     int32_t *sample_w2t;        // [max_samples][R]
     // synchronisation
-    float *halo;                // [R][S][2 (top, bottom)][2 channels][L]
-    unsigned *flags;            // [R][S][2] (launch-local counters, zeroed by host)
+    uint64_t *halo;             // [R][S][2 (row 2, row H)][2 (sweep parity)][L] words
+                                //   (sweep << 32 | payload)
     unsigned *barrier;          // [2] count, generation
     double *epart;              // [2][R][S]
     int *error;
@@ -75,26 +89,23 @@ __device__ __forceinline__ int lds_index(int row, int j, int L) {
     return row * L + (j & 1) * (L >> 1) + (j >> 1);
 }
 
-// Hand-off protocol.  Every word that crosses workgroups (halo rows, energy
-// partials, flags, barrier counters) is written and read with agent-scope
+// Cross-workgroup protocol.  Every word that crosses workgroups (halo words,
+// energy partials, barrier counters) is written and read with agent-scope
 // relaxed atomics, which are device-coherent on their own (gfx950 sets the
 // device-scope cache bits on them), so no L2 write-back or invalidate is
-// needed:
-//  * publisher: every thread stores its part, waits until its stores are
-//    complete (s_waitcnt), workgroup barrier, then thread 0 stores the flag;
-//  * consumer: thread 0 polls the flag, workgroup barrier, then the data
-//    loads are issued -- after the flag was seen, so they read the new data.
+// needed.  Halo words are single-copy atomic 64-bit values whose high half
+// names the sweep, so a reader that sees the expected sweep also sees the
+// payload written with it; energy partials are published before the grid
+// barrier's arrival, whose counter update is ordered after them by a full
+// s_waitcnt and a workgroup barrier.
 struct Sync {
     const BatchArgs &a;
-    __device__ unsigned *flag(int rr, int ss, int which) const {
-        return a.flags + ((size_t)rr * a.S + ss) * 2 + which;
-    }
-    // thread 0: poll until *f >= target (or != target), bounded by the watchdog
-    __device__ bool poll(unsigned *f, unsigned target, bool until_changed) const {
+    // thread 0: poll until *f != target, bounded by the watchdog
+    __device__ bool poll_changed(unsigned *f, unsigned target) const {
         unsigned n = 0;
         for (;;) {
             const unsigned v = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (until_changed ? (v != target) : (v >= target)) return true;
+            if (v != target) return true;
             __builtin_amdgcn_s_sleep(1);
             if ((++n & 1023u) == 0 &&
                 (n > kWatchdog || __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
@@ -103,16 +114,21 @@ struct Sync {
             }
         }
     }
-    __device__ bool wait_flag(unsigned *f, unsigned target) const {
-        __shared__ int ok;
-        if (threadIdx.x == 0) ok = poll(f, target, false);
-        __syncthreads();
-        return ok;
-    }
-    __device__ void release_flag(unsigned *f, unsigned value) const {
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        if (threadIdx.x == 0) __hip_atomic_store(f, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // one thread: poll a halo word until it carries sweep `seq`
+    __device__ bool poll_word(const uint64_t *p, unsigned seq, uint32_t &payload) const {
+        unsigned n = 0;
+        for (;;) {
+            const uint64_t w =
+                __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((unsigned)(w >> 32) == seq) {
+                payload = (uint32_t)w;
+                return true;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            if ((++n & 1023u) == 0 &&
+                (n > kWatchdog || __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+                return false;
+        }
     }
     // sense-reversing grid barrier over all workgroups
     __device__ bool grid_barrier(unsigned nwg) const {
@@ -129,7 +145,7 @@ struct Sync {
                 __builtin_amdgcn_s_waitcnt(0);
                 __hip_atomic_store(gen, g + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else {
-                good = poll(gen, g, true);
+                good = poll_changed(gen, g);
             }
             ok = good;
         }
@@ -137,13 +153,6 @@ struct Sync {
         return ok;
     }
 };
-
-__device__ __forceinline__ void store_relaxed(float *p, float v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ float load_relaxed(const float *p) {
-    return __hip_atomic_load(const_cast<float *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // Ising acceptance threshold for 2 s sum_nn = v: always flip when dE <= 0,
 // else u < exp(-beta dE) in float32 through the float64 exponential.
@@ -172,12 +181,21 @@ __device__ void block_sum(double (&v)[C], double (*sh)[kBatchThreads / 64]) {
     __syncthreads();
 }
 
-// LDS bytes of one strip's site data (everything but the PT scratch)
+// LDS bytes of one strip's site data (everything but the PT scratch): RNG
+// streams and angles of rows 1..H+1, cos/sin (or spins) of rows 0..H+2
 __host__ __device__ inline size_t strip_bytes(int model, int L, int H) {
-    const size_t rng = (size_t)H * L * 16;
-    return model == MODEL_ISING ? rng + (size_t)(H + 2) * L
-                                : rng + (size_t)(H + 2) * L * 8 + (size_t)H * L * 4;
+    const size_t rng = (size_t)(H + 1) * L * 16;
+    return model == MODEL_ISING ? rng + (size_t)(H + 3) * L
+                                : rng + (size_t)(H + 3) * L * 8 + (size_t)(H + 1) * L * 4;
 }
+
+// One group of colour rows of a pass: colour c, colour rows lo..hi (empty if
+// hi < lo).  Colour rows of colours 0/1 are local rows 1, 3, ..., H+1
+// (ri = 0..H/2, ri = H/2 being the ghost row); of colours 2/3 local rows
+// 2, 4, ..., H (ri = 0..H/2-1).
+struct Seg {
+    int c, lo, hi;
+};
 
 template <int MODEL>
 __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
@@ -195,11 +213,11 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
     const int s_up = (s + S - 1) % S, s_dn = (s + 1) % S;
 
     // ---- LDS carve-up
-    ulonglong2 *lrng = reinterpret_cast<ulonglong2 *>(smem);            // own rows, H * L
-    float *lc = reinterpret_cast<float *>(lrng + (size_t)H * L);         // rows 0..H+1
-    float *ls = lc + (H + 2) * L;                                        // rows 0..H+1
-    float *lth = ls + (H + 2) * L;                                       // own rows, H * L
-    int8_t *lsp = reinterpret_cast<int8_t *>(lrng + (size_t)H * L);      // rows 0..H+1 (Ising)
+    ulonglong2 *lrng = reinterpret_cast<ulonglong2 *>(smem);              // rows 1..H+1
+    float *lc = reinterpret_cast<float *>(lrng + (size_t)(H + 1) * L);     // rows 0..H+2
+    float *ls = lc + (H + 3) * L;                                          // rows 0..H+2
+    float *lth = ls + (H + 3) * L;                                         // rows 1..H+1
+    int8_t *lsp = reinterpret_cast<int8_t *>(lrng + (size_t)(H + 1) * L);  // rows 0..H+2 (Ising)
     unsigned char *pt_base = smem + ((strip_bytes(MODEL, L, H) + 15) & ~(size_t)15);
     float *pt_beta = reinterpret_cast<float *>(pt_base);                 // R
     int32_t *pt_t2w = reinterpret_cast<int32_t *>(pt_beta + R);          // R
@@ -214,13 +232,13 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
     const int row0 = s * H;  // first global row of the strip
     const bool use_pt = a.exchange_interval > 0;
 
-    // ---- load the strip: own rows (state, RNG) and neighbour data incl. halos
-    for (int idx = tid; idx < (H + 2) * L; idx += kBatchThreads) {
+    // ---- load the strip: rows 0..H+2 (global rows row0-1 .. row0+H+1)
+    for (int idx = tid; idx < (H + 3) * L; idx += kBatchThreads) {
         const int lr = idx / L, j = idx - lr * L;
         const int gi = (row0 + lr - 1 + L) % L;
         const size_t gk = rep + (size_t)gi * L + j;
-        const bool own_row = lr >= 1 && lr <= H;
-        if (own_row) lrng[lds_index(lr - 1, j, L)] = reinterpret_cast<const ulonglong2 *>(a.rng)[gk];
+        const bool upd_row = lr >= 1 && lr <= H + 1;  // own rows and the ghost
+        if (upd_row) lrng[lds_index(lr - 1, j, L)] = reinterpret_cast<const ulonglong2 *>(a.rng)[gk];
         if (MODEL == MODEL_ISING) {
             lsp[lds_index(lr, j, L)] = reinterpret_cast<const int8_t *>(a.state)[gk];
         } else {
@@ -229,7 +247,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
             fast_sincosf(th, sv, cv);
             lc[lds_index(lr, j, L)] = cv;
             ls[lds_index(lr, j, L)] = sv;
-            if (own_row) lth[lds_index(lr - 1, j, L)] = th;
+            if (upd_row) lth[lds_index(lr - 1, j, L)] = th;
         }
     }
     if (use_pt) {
@@ -249,88 +267,157 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
     const float J = a.J, delta = a.delta;
     const float one = 1.0f, two = 2.0f;
     const float pi = 3.14159274101257324f, two_pi = 6.28318548202514648f;
-    // fixed thread -> (column, row offset) map of a colour step
+    // fixed thread -> (column, colour-row offset) map of every pass
     const int tcol = tid % half, trow = tid / half, rows_per_pass = kBatchThreads / half;
 
-    // Update the sites of colour c in colour rows [ri_lo, ri_hi]; colour rows of
-    // colours 0/1 are local rows 1, 3, ...; of colours 2/3 local rows 2, 4, ...
-    auto update = [&](const int c, const int ri_lo, const int ri_hi, const float beta) {
-        if (trow >= rows_per_pass) return;
+    // Update site (colour row ri, column 2 tcol + (c & 1)) of colour c.
+    auto site = [&](const int c, const int ri, const float beta) {
         const int p = c & 1;
-        for (int ri = ri_lo + trow; ri <= ri_hi; ri += rows_per_pass) {
-            const int lr = 1 + 2 * ri + (c >> 1);
-            const int col = tcol;
-            const int base = lr * L;
-            const int self = base + p * half + col;
-            const int up = self - L, dn = self + L;
-            const int lf = p ? base + col : base + half + (col == 0 ? half - 1 : col - 1);
-            const int rt = p ? base + (col == half - 1 ? 0 : col + 1) : base + half + col;
-            const int own = self - L;  // own-row arrays start at local row 1
-            const ulonglong2 st = lrng[own];
-            Rng g;
-            g.s0 = st.x;
-            g.s1 = st.y;
-            if (MODEL == MODEL_ISING) {
-                const int sv = lsp[self];
-                const int nb = (int)lsp[up] + (int)lsp[dn] + (int)lsp[lf] + (int)lsp[rt];
-                const float u = g.uniform_f32();
-                if (u < thr[(2 * sv * nb + 8) >> 2]) lsp[self] = (int8_t)(-sv);
-            } else {
-                const float th = lth[own];
-                const float u = g.uniform_f32();
-                float pr = th + delta * (two * u - one);
-                if (pr >= pi)
-                    pr -= two_pi;
-                else if (pr < -pi)
-                    pr += two_pi;
-                float sp, cp;
-                fast_sincosf(pr, sp, cp);
-                const float Cs = ((lc[up] + lc[dn]) + lc[lf]) + lc[rt];
-                const float Ss = ((ls[up] + ls[dn]) + ls[lf]) + ls[rt];
-                float t = (cp - lc[self]) * Cs;
-                t = fmaf(sp - ls[self], Ss, t);
-                const float dE = -J * t;
-                const float v = g.uniform_f32();
-                if (dE <= 0.0f || v < fast_expf(-beta * dE)) {
-                    lth[own] = pr;
-                    lc[self] = cp;
-                    ls[self] = sp;
+        const int lr = 1 + 2 * ri + (c >> 1);
+        const int base = lr * L;
+        const int self = base + p * half + tcol;
+        const int up = self - L, dn = self + L;
+        const int lf = p ? base + tcol : base + half + (tcol == 0 ? half - 1 : tcol - 1);
+        const int rt = p ? base + (tcol == half - 1 ? 0 : tcol + 1) : base + half + tcol;
+        const int own = self - L;  // RNG and angle arrays start at local row 1
+        const ulonglong2 st = lrng[own];
+        Rng g;
+        g.s0 = st.x;
+        g.s1 = st.y;
+        if (MODEL == MODEL_ISING) {
+            const int sv = lsp[self];
+            const int nb = (int)lsp[up] + (int)lsp[dn] + (int)lsp[lf] + (int)lsp[rt];
+            const float u = g.uniform_f32();
+            if (u < thr[(2 * sv * nb + 8) >> 2]) lsp[self] = (int8_t)(-sv);
+        } else {
+            const float th = lth[own];
+            const float u = g.uniform_f32();
+            float pr = th + delta * (two * u - one);
+            if (pr >= pi)
+                pr -= two_pi;
+            else if (pr < -pi)
+                pr += two_pi;
+            float sp, cp;
+            fast_sincosf(pr, sp, cp);
+            const float Cs = ((lc[up] + lc[dn]) + lc[lf]) + lc[rt];
+            const float Ss = ((ls[up] + ls[dn]) + ls[lf]) + ls[rt];
+            float t = (cp - lc[self]) * Cs;
+            t = fmaf(sp - ls[self], Ss, t);
+            const float dE = -J * t;
+            const float v = g.uniform_f32();
+            if (dE <= 0.0f || v < fast_expf(-beta * dE)) {
+                lth[own] = pr;
+                lc[self] = cp;
+                ls[self] = sp;
+            }
+        }
+        ulonglong2 out;
+        out.x = g.s0;
+        out.y = g.s1;
+        lrng[own] = out;
+    };
+    // One pass: up to four groups of colour rows, spread over the threads in
+    // the fixed map (a thread takes colour rows trow, trow + rows_per_pass, ...
+    // of the concatenated groups).
+    auto pass = [&](const Seg g0, const Seg g1, const Seg g2, const Seg g3, const float beta) {
+        if (trow >= rows_per_pass) return;
+        const int n0 = max(g0.hi - g0.lo + 1, 0), n1 = max(g1.hi - g1.lo + 1, 0);
+        const int n2 = max(g2.hi - g2.lo + 1, 0), n3 = max(g3.hi - g3.lo + 1, 0);
+        const int total = n0 + n1 + n2 + n3;
+        for (int q = trow; q < total; q += rows_per_pass) {
+            int k = q;
+            Seg g = g0;
+            if (k >= n0) {
+                k -= n0;
+                g = g1;
+                if (k >= n1) {
+                    k -= n1;
+                    g = g2;
+                    if (k >= n2) {
+                        k -= n2;
+                        g = g3;
+                    }
                 }
             }
-            ulonglong2 out;
-            out.x = g.s0;
-            out.y = g.s1;
-            lrng[own] = out;
+            site(g.c, g.lo + k, beta);
         }
     };
-    auto halo_ptr = [&](int ss, int which) { return a.halo + (((size_t)r * S + ss) * 2 + which) * 2 * L; };
-    auto load_row = [&](int lds_row, const float *src) {
-        for (int j = tid; j < L; j += kBatchThreads) {
-            if (MODEL == MODEL_ISING) {
-                lsp[lds_index(lds_row, j, L)] = (int8_t)load_relaxed(src + j);
-            } else {
-                lc[lds_index(lds_row, j, L)] = load_relaxed(src + j);
-                ls[lds_index(lds_row, j, L)] = load_relaxed(src + L + j);
+    const Seg none{0, 0, -1};
+    auto halo_ptr = [&](int ss, int which, int par) {
+        return a.halo + ((((size_t)r * S + ss) * 2 + which) * 2 + par) * L;
+    };
+    auto put_row = [&](int lds_row, int j, uint32_t payload) {
+        if (MODEL == MODEL_ISING) {
+            lsp[lds_index(lds_row, j, L)] = (int8_t)(uint8_t)payload;
+        } else {
+            float sv, cv;
+            fast_sincosf(__uint_as_float(payload), sv, cv);
+            lc[lds_index(lds_row, j, L)] = cv;
+            ls[lds_index(lds_row, j, L)] = sv;
+        }
+    };
+    // Rows 0 and H+2 for sweep n: the bottom row of strip s-1 and row 2 of
+    // strip s+1 as of the end of sweep n-1 (tag n).  With one strip they are
+    // this strip's rows H and 2, which still hold sweep n-1 when this runs.
+    // Ends with a workgroup barrier; false if the watchdog expired.
+    auto receive = [&](int n) -> bool {
+        if (S == 1) {
+            for (int k = tid; k < 2 * L; k += kBatchThreads) {
+                const bool top = k < L;
+                const int j = top ? k : k - L;
+                const int dst = lds_index(top ? 0 : H + 2, j, L), src = lds_index(top ? H : 2, j, L);
+                if (MODEL == MODEL_ISING) {
+                    lsp[dst] = lsp[src];
+                } else {
+                    lc[dst] = lc[src];
+                    ls[dst] = ls[src];
+                }
             }
+            __syncthreads();
+            return true;
         }
-    };
-    auto store_row = [&](float *dst, int lds_row) {
-        for (int j = tid; j < L; j += kBatchThreads) {
-            if (MODEL == MODEL_ISING) {
-                store_relaxed(dst + j, (float)lsp[lds_index(lds_row, j, L)]);
-            } else {
-                store_relaxed(dst + j, lc[lds_index(lds_row, j, L)]);
-                store_relaxed(dst + L + j, ls[lds_index(lds_row, j, L)]);
+        if (n == 0) {  // the launch loaded them
+            __syncthreads();
+            return true;
+        }
+        const int par = (n - 1) & 1;
+        int bad = 0;
+        for (int k = tid; k < 2 * L; k += kBatchThreads) {
+            const bool top = k < L;
+            const int j = top ? k : k - L;
+            const uint64_t *src = top ? halo_ptr(s_up, 1, par) : halo_ptr(s_dn, 0, par);
+            uint32_t payload;
+            if (!sync.poll_word(src + j, (unsigned)n, payload)) {
+                bad = 1;
+                break;
             }
+            put_row(top ? 0 : H + 2, j, payload);
         }
+        if (__syncthreads_or(bad)) {
+            if (tid == 0) atomicOr(a.error, BATCH_ERR_WATCHDOG);
+            return false;
+        }
+        return true;
     };
-    auto copy_row = [&](int dst_row, int src_row) {
-        for (int j = tid; j < L; j += kBatchThreads) {
-            if (MODEL == MODEL_ISING) {
-                lsp[lds_index(dst_row, j, L)] = lsp[lds_index(src_row, j, L)];
-            } else {
-                lc[lds_index(dst_row, j, L)] = lc[lds_index(src_row, j, L)];
-                ls[lds_index(dst_row, j, L)] = ls[lds_index(src_row, j, L)];
+    // After the colour-3 pass of sweep n: publish rows 2 and H (tag n + 1).
+    // Mirrors that pass's thread map, so each thread publishes the colour-3
+    // site it just wrote and its colour-2 partner (written before the last
+    // barrier): no extra barrier.
+    auto publish = [&](int n) {
+        if (S == 1 || trow >= rows_per_pass) return;
+        const int par = n & 1;
+        for (int q = trow; q < HH; q += rows_per_pass) {
+            if (q != 0 && q != HH - 1) continue;
+            const int lr = 2 + 2 * q;
+            uint64_t *dst = halo_ptr(s, q == 0 ? 0 : 1, par);
+#pragma unroll
+            for (int pp = 0; pp < 2; pp++) {
+                const int j = 2 * tcol + pp;
+                const uint32_t payload = MODEL == MODEL_ISING
+                                             ? (uint32_t)(uint8_t)lsp[lds_index(lr, j, L)]
+                                             : __float_as_uint(lth[lds_index(lr - 1, j, L)]);
+                __hip_atomic_store(dst + j, ((uint64_t)(unsigned)(n + 1) << 32) | payload, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
             }
         }
     };
@@ -340,61 +427,36 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
     //   c1(row) follows c0(row), c2(row) follows c0(rows +-1),
     //   c3(row) follows c2(row) and c1(rows +-1),
     //   and each site's update precedes its neighbours' next update
-    // gives identical results.  Interior rows therefore go first: only the top
-    // row T (local 1) and T + 1 depend on the strip above (its bottom row after
-    // colours 2/3 of the previous sweep), and only the bottom row B (local H)
-    // on the strip below (its top row after colours 0/1 of this sweep).
+    // gives identical results.  Per sweep n:
+    //   P1  c0 of interior 0/1 rows (local 3..H-1)
+    //   P2  c1 of the same rows; receive rows 0 and H+2 (sweep n-1)
+    //   P3  c0 of rows 1 and H+1 (ghost); c2 of local rows 4..H-2
+    //   P4  c1 of rows 1 and H+1; c2 of rows 2 and H
+    //   P5  c3 of all 2/3 rows; publish rows 2 and H
+    // P5 of sweep n and P1 of sweep n+1 touch disjoint sites and P2 starts
+    // after a barrier, so P5 needs no barrier of its own.
     for (int n = 0; n < a.nsweeps; n++) {
         const long long sweep_global = a.sweep0 + n + 1;
         const float beta = my_beta;
-        update(0, 1, HH - 1, beta);                  // c0, interior rows of colours 0/1
+        pass(Seg{0, 1, HH - 1}, none, none, none, beta);
         __syncthreads();
-        update(1, 1, HH - 1, beta);                  // c1, same rows
-        update(2, 1, HH - 2, beta);                  // c2, interior rows of colours 2/3
+        pass(Seg{1, 1, HH - 1}, none, none, none, beta);
+        if (!receive(n)) goto done;
+        pass(Seg{0, 0, 0}, Seg{0, HH, HH}, Seg{2, 1, HH - 2}, none, beta);
         __syncthreads();
-        if (n > 0 && S > 1) {                        // halo above: row B of strip s-1
-            if (!sync.wait_flag(sync.flag(r, s_up, 1), (unsigned)n)) goto done;
-            load_row(0, halo_ptr(s_up, 1));
-            __syncthreads();
-        }
-        update(3, 1, HH - 2, beta);                  // c3, interior rows of colours 2/3
-        update(0, 0, 0, beta);                       // c0, row T
+        pass(Seg{1, 0, 0}, Seg{1, HH, HH}, Seg{2, 0, 0}, Seg{2, HH - 1, HH - 1}, beta);
         __syncthreads();
-        update(1, 0, 0, beta);                       // c1, row T
-        update(2, 0, 0, beta);                       // c2, row T + 1
-        __syncthreads();
-        if (S > 1) {                                 // publish row T for strip s-1
-            store_row(halo_ptr(s, 0), 1);
-            sync.release_flag(sync.flag(r, s, 0), (unsigned)(n + 1));
-        } else {
-            copy_row(H + 1, 1);                      // one strip: the row below B is T
-        }
-        update(3, 0, 0, beta);                       // c3, row T + 1
-        __syncthreads();
-        if (S > 1) {                                 // halo below: row T of strip s+1
-            if (!sync.wait_flag(sync.flag(r, s_dn, 0), (unsigned)(n + 1))) goto done;
-            load_row(H + 1, halo_ptr(s_dn, 0));
-            __syncthreads();
-        }
-        update(2, HH - 1, HH - 1, beta);             // c2, row B
-        __syncthreads();
-        update(3, HH - 1, HH - 1, beta);             // c3, row B
-        __syncthreads();
-        if (S > 1) {                                 // publish row B for strip s+1
-            store_row(halo_ptr(s, 1), H);
-            sync.release_flag(sync.flag(r, s, 1), (unsigned)(n + 1));
-        } else {
-            copy_row(0, H);                          // one strip: the row above T is B
-            __syncthreads();
-        }
+        pass(Seg{3, 0, HH - 1}, none, none, none, beta);
+        publish(n);
 
         const bool do_exchange = use_pt && (sweep_global % a.exchange_interval == 0);
         const bool do_sample = a.sample_interval > 0 && sweep_global >= a.sample_first &&
                                ((sweep_global - a.sample_first) % a.sample_interval == 0);
         if (!do_exchange && !do_sample) continue;
+        __syncthreads();  // the observables read every row
 
-        // ---- strip observables: right and down bonds (row below = LDS row H+1,
-        // current: it only changes in colours 0/1)
+        // ---- strip observables: right and down bonds (the row below row H is
+        // the ghost row H+1, current for this sweep)
         double v[kChannels];
         {
             double e = 0, m1 = 0, m2 = 0, cx = 0, qx = 0;
@@ -543,6 +605,21 @@ done:
     if (blockIdx.x == 0 && tid == 0 && use_pt) *a.parity = parity;
 }
 
+// Zeroes the launch-local synchronisation words: halo words (their sweep tags
+// restart at 1 every launch), the grid barrier and the error word.  A kernel
+// rather than memset nodes, so a launch can be captured into a graph.
+__global__ void batch_reset_kernel(uint64_t *halo, size_t nhalo, unsigned *barrier, int *error) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < nhalo; k += stride) halo[k] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        barrier[0] = 0;
+        barrier[1] = 0;
+        *error = 0;
+    }
+}
+
+static size_t halo_words(int L, int R, int H) { return (size_t)R * (L / H) * 2 * 2 * L; }
+
 static size_t batch_lds(int model, int L, int R, int H, int exchange) {
     const size_t pt_bytes = exchange ? (size_t)3 * R * 4 + 8 + (size_t)R * 8 + (size_t)2 * R * 8 : 0;
     return ((strip_bytes(model, L, H) + 15) & ~(size_t)15) + pt_bytes + 64;
@@ -569,13 +646,13 @@ using namespace mc;
 
 extern "C" {
 
-int mc_batch_workspace_size(int model, int L, int R, int H, size_t *halo_floats, size_t *flag_words,
+int mc_batch_workspace_size(int model, int L, int R, int H, size_t *halo_words_out, size_t *barrier_words,
                             size_t *epart_doubles) {
     int rc = check_shape(model, L, R, H);
     if (rc) return rc;
     const int S = L / H;
-    *halo_floats = (size_t)R * S * 2 * 2 * L;
-    *flag_words = (size_t)R * S * 2 + 2;
+    *halo_words_out = halo_words(L, R, H);
+    *barrier_words = 2;
     *epart_doubles = (size_t)2 * R * S;
     return MC_OK;
 }
@@ -602,7 +679,7 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, void *state, uint
                  float J, float delta, int nsweeps, long long sweep0, int exchange_interval, int *parity,
                  double *energies, int32_t *w2t, int32_t *t2w, uint64_t *pt_rng, long long *counts,
                  long long *trips, int sample_interval, long long sample_first, int max_samples,
-                 double *sample_partials, int32_t *sample_w2t, float *halo, unsigned *flags,
+                 double *sample_partials, int32_t *sample_w2t, uint64_t *halo, unsigned *barrier,
                  double *epart, int *error) {
     int rc = check_shape(model, L, R, H);
     if (rc) return rc;
@@ -623,13 +700,15 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, void *state, uint
     a.w2t = w2t; a.t2w = t2w; a.pt_rng = pt_rng; a.counts = counts; a.trips = trips;
     a.sample_interval = sample_interval; a.sample_first = sample_first; a.max_samples = max_samples;
     a.sample_partials = sample_partials; a.sample_w2t = sample_w2t;
-    a.halo = halo; a.flags = flags; a.barrier = flags + (size_t)R * S * 2; a.epart = epart; a.error = error;
+    a.halo = halo; a.barrier = barrier; a.epart = epart; a.error = error;
 
     if (lds > 64 * 1024)
         MC_TRY_HIP(hipFuncSetAttribute(kernel_ptr(model), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    // launch-local flag counters and barrier start at zero
-    MC_TRY_HIP(hipMemsetAsync(flags, 0, ((size_t)R * S * 2 + 2) * sizeof(unsigned), as_stream(stream)));
-    MC_TRY_HIP(hipMemsetAsync(error, 0, sizeof(int), as_stream(stream)));
+    const size_t nh = halo_words(L, R, H);
+    const unsigned reset_blocks = (unsigned)std::min<size_t>((nh + 255) / 256, 1024);
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(reset_blocks), dim3(256), 0, as_stream(stream), halo, nh, barrier,
+                       error);
+    MC_TRY_HIP(hipGetLastError());
     void *kargs[] = {&a};
     MC_TRY_HIP(hipLaunchCooperativeKernel(kernel_ptr(model), dim3((unsigned)(R * S)), dim3(kBatchThreads),
                                           kargs, lds, as_stream(stream)));

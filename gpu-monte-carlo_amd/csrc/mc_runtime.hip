@@ -284,7 +284,10 @@ int mc_device_info(char *name, int name_bytes, int *compute_units) {
     hipDeviceProp_t prop;
     MC_TRY_HIP(hipGetDeviceProperties(&prop, dev));
     if (name && name_bytes > 0) {
-        snprintf(name, (size_t)name_bytes, "%s (%s)", prop.name, prop.gcnArchName);
+        if (prop.name[0])
+            snprintf(name, (size_t)name_bytes, "%s (%s)", prop.name, prop.gcnArchName);
+        else  // some runtimes leave the marketing name empty
+            snprintf(name, (size_t)name_bytes, "%s", prop.gcnArchName);
     }
     if (compute_units) *compute_units = prop.multiProcessorCount;
     return MC_OK;
