@@ -103,6 +103,8 @@ def _declare(lib):
         "orc_pt_exchange_f32": (None, [P, P, P, P, P, P, P, I32, I32, I64]),
         "orc_fast_sincosf": (None, [F32, P, P]),
         "orc_fast_expf": (F32, [F32]),
+        "orc_fast_sincosf_n": (None, [P, I64, P, P]),
+        "orc_fast_expf_n": (None, [P, I64, P]),
         "orc_xyfast_init_cache": (None, [P, I64, P, P]),
         "orc_xyfast_color": (None, [P, P, P, P, P, I32, I32, I32, F32, F32]),
         "orc_xyfast_sweeps": (None, [P, P, P, P, P, I32, I32, F32, F32, I64]),
@@ -386,23 +388,19 @@ def pt_exchange(energies, betas, w2t, t2w, pt_rng, counts, trips, parity, sweep)
 # ------------------------------------------------ batched float32 fast path
 
 def fast_sincosf(x):
-    """Vectorised wrapper: (sin, cos) arrays from the oracle's float32 routine."""
+    """(sin, cos) arrays from the oracle's float32 routine."""
     x = np.ascontiguousarray(x, dtype=np.float32)
     sn = np.empty_like(x)
     cs = np.empty_like(x)
-    f = load().orc_fast_sincosf
-    fs, fc = sn.reshape(-1), cs.reshape(-1)
-    for k, v in enumerate(x.reshape(-1)):
-        a, b = F32(), F32()
-        f(float(v), ctypes.byref(a), ctypes.byref(b))
-        fs[k], fc[k] = a.value, b.value
+    load().orc_fast_sincosf_n(x.ctypes.data, x.size, sn.ctypes.data, cs.ctypes.data)
     return sn, cs
 
 
 def fast_expf(x):
     x = np.ascontiguousarray(x, dtype=np.float32)
-    f = load().orc_fast_expf
-    return np.array([f(float(v)) for v in x.reshape(-1)], dtype=np.float32).reshape(x.shape)
+    y = np.empty_like(x)
+    load().orc_fast_expf_n(x.ctypes.data, x.size, y.ctypes.data)
+    return y
 
 
 def xyfast_cache(theta):

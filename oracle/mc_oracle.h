@@ -129,6 +129,8 @@ void orc_pt_exchange_f64(const double *energies, double *betas, int32_t *w2t, in
 /* ---- batched float32 fast path (mc_oracle_fast.c) ---- */
 void orc_fast_sincosf(float x, float *sn, float *cs);
 float orc_fast_expf(float x);
+void orc_fast_sincosf_n(const float *x, int64_t n, float *sn, float *cs);
+void orc_fast_expf_n(const float *x, int64_t n, float *y);
 void orc_xyfast_init_cache(const float *th, int64_t n, float *c, float *s);
 void orc_xyfast_color(float *th, float *c, float *s, uint64_t *rng, const float *betas, int R, int L,
                       int color, float J, float delta);

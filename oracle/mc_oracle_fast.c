@@ -81,6 +81,14 @@ float orc_fast_expf(float x) {
     return p * two_e.f;
 }
 
+void orc_fast_sincosf_n(const float *x, int64_t n, float *sn, float *cs) {
+    for (int64_t k = 0; k < n; k++) orc_fast_sincosf(x[k], sn + k, cs + k);
+}
+
+void orc_fast_expf_n(const float *x, int64_t n, float *y) {
+    for (int64_t k = 0; k < n; k++) y[k] = orc_fast_expf(x[k]);
+}
+
 void orc_xyfast_init_cache(const float *th, int64_t n, float *c, float *s) {
     for (int64_t k = 0; k < n; k++) orc_fast_sincosf(th[k], s + k, c + k);
 }

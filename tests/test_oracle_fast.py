@@ -1,0 +1,134 @@
+"""Pins of the oracle's float32 fast path (the batched XY engine's arithmetic).
+
+* fast_sincosf / fast_expf against float64 libm: error bounds in float32 ulps
+  over dense grids, exact odd/even symmetry, exact special values, monotone exp.
+  A wrong polynomial coefficient, reduction constant or quadrant fails these.
+* The fast XY update evaluates dE = -J[(cos p - cos t) C + (sin p - sin t) S]
+  from cached (cos, sin); mathematically it is the reference update
+  (models/xy/update.py:6-54), so a site with frozen neighbours must sample the
+  von Mises conditional, and the cache must stay equal to fast_sincosf(theta).
+* The fast observables equal the reference XY measurement
+  (models/xy/measure.py:17-134) up to float32 rounding of the bond terms.
+"""
+
+import numpy as np
+import pytest
+from scipy import special
+
+import oracle
+
+
+def _ulp_error(got, ref):
+    ref32 = ref.astype(np.float32)
+    ulp = np.maximum(np.spacing(np.abs(ref32)).astype(np.float64), float(np.spacing(np.float32(2.0**-24))))
+    return np.abs(got.astype(np.float64) - ref) / ulp
+
+
+def test_fast_sincosf_error_bound_and_symmetry():
+    grid = np.linspace(-2 * np.pi, 2 * np.pi, 1_000_001).astype(np.float32)
+    rand = np.random.default_rng(3).uniform(-7.0, 7.0, 1_000_000).astype(np.float32)
+    x = np.concatenate([grid, rand])
+    sn, cs = oracle.fast_sincosf(x)
+    xd = x.astype(np.float64)
+    assert _ulp_error(sn, np.sin(xd)).max() <= 1.5
+    assert _ulp_error(cs, np.cos(xd)).max() <= 1.5
+    sn_neg, cs_neg = oracle.fast_sincosf(-x)
+    assert np.array_equal(sn_neg, -sn)
+    assert np.array_equal(cs_neg, cs)
+    s0, c0 = oracle.fast_sincosf(np.array([0.0], np.float32))
+    assert s0[0] == 0.0 and c0[0] == 1.0
+    norm = sn.astype(np.float64)**2 + cs.astype(np.float64)**2
+    assert np.abs(norm - 1).max() < 4e-7
+
+
+def test_fast_expf_error_bound_monotone_and_special_values():
+    x = np.linspace(-103.9, 0.0, 2_000_001).astype(np.float32)
+    y = oracle.fast_expf(x)
+    ref = np.exp(x.astype(np.float64))
+    normal = ref > np.finfo(np.float32).tiny
+    assert _ulp_error(y[normal], ref[normal]).max() <= 1.0
+    assert np.abs(y[~normal] - ref[~normal]).max() < 2 * np.finfo(np.float32).smallest_subnormal
+    assert np.all(np.diff(y) >= 0)
+    special_in = np.array([0.0, -104.0, -1000.0, -np.inf, np.nan], np.float32)
+    out = oracle.fast_expf(special_in)
+    assert out[0] == 1.0
+    assert np.all(out[1:4] == 0.0)
+    assert np.isnan(out[4])
+
+
+def test_fast_cache_follows_angles_through_sweeps():
+    L, R = 8, 3
+    theta = np.random.default_rng(5).uniform(-np.pi, np.pi, (R, L, L)).astype(np.float32)
+    c, s = oracle.xyfast_cache(theta)
+    rng = oracle.rng_init(R * L * L, 9)
+    betas = np.array([0.3, 1.0, 3.0], np.float32)
+    oracle.xyfast_sweeps(theta, c, s, rng, betas, 1.0, 1.5, 20)
+    s_ref, c_ref = oracle.fast_sincosf(theta)
+    assert np.array_equal(c, c_ref) and np.array_equal(s, s_ref)
+    assert np.all((theta >= -np.pi) & (theta < np.float32(np.pi)))
+
+
+def test_fast_observables_equal_reference_measurement():
+    L, R, J = 16, 4, 1.3
+    theta = np.random.default_rng(7).uniform(-np.pi, np.pi, (R, L, L)).astype(np.float32)
+    c, s = oracle.xyfast_cache(theta)
+    fast = oracle.xyfast_observables(c, s, J)
+    ref = oracle.xy_measure(theta, J)
+    assert np.allclose(fast, ref, rtol=0, atol=L * L * 2e-6), np.abs(fast - ref).max()
+    # ordered state: every bond at -J, twist sums L^2 and 0
+    zero = np.zeros((1, L, L), np.float32)
+    c0, s0 = oracle.xyfast_cache(zero)
+    e, mx, my, cx, qx = oracle.xyfast_observables(c0, s0, J)[:, 0]
+    assert (e, mx, my, cx, qx) == (-2 * float(np.float32(J)) * L * L, L * L, 0.0, L * L, 0.0)
+
+
+def _binned(x, nbins):
+    b = np.asarray(x)[: len(x) // nbins * nbins].reshape(nbins, -1).mean(axis=1)
+    return b.mean(), b.std(ddof=1) / np.sqrt(nbins)
+
+
+def test_fast_update_single_site_conditional_is_von_mises():
+    L, R = 4, 2
+    theta0 = np.random.default_rng(11).uniform(-np.pi, np.pi, (R, L, L)).astype(np.float32)
+    betas = np.array([0.4, 1.5], np.float32)
+    states = oracle.rng_init(R * L * L, 8)
+    theta = theta0.copy()
+    c, s = oracle.xyfast_cache(theta)
+    sites = [(i, j) for i in range(0, L, 2) for j in range(0, L, 2)]
+    samples = []
+    for _ in range(30000):
+        oracle.xyfast_color(theta, c, s, states, betas, 0, 1.0, 1.5)
+        samples.append([[theta[r, i, j] for (i, j) in sites] for r in range(R)])
+    samples = np.asarray(samples, dtype=float)
+    for r in range(R):
+        for k, (i, j) in enumerate(sites):
+            nb = [theta0[r, (i - 1) % L, j], theta0[r, (i + 1) % L, j], theta0[r, i, (j - 1) % L],
+                  theta0[r, i, (j + 1) % L]]
+            S = np.sum(np.exp(1j * np.asarray(nb, float)))
+            kappa, phi = float(betas[r]) * abs(S), np.angle(S)
+            x = samples[:, r, k] - phi
+            for stat, expect in ((np.cos(x), special.iv(1, kappa) / special.iv(0, kappa)),
+                                 (np.sin(x), 0.0),
+                                 (np.cos(2 * x), special.iv(2, kappa) / special.iv(0, kappa))):
+                mean, err = _binned(stat, 30)
+                assert abs(mean - expect) < 5 * err + 1e-3, (r, k, mean, expect, err)
+
+
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_fast_update_hot_and_cold_limits(sign):
+    """beta = 0 accepts every proposal; a huge beta rejects every uphill one."""
+    L, R = 8, 2
+    theta = np.zeros((R, L, L), np.float32) if sign > 0 else \
+        np.random.default_rng(1).uniform(-np.pi, np.pi, (R, L, L)).astype(np.float32)
+    c, s = oracle.xyfast_cache(theta)
+    rng = oracle.rng_init(R * L * L, 4)
+    probe = rng.copy()
+    before = theta.copy()
+    betas = np.array([0.0, 1e30], np.float32)
+    oracle.xyfast_color(theta, c, s, rng, betas, 0, 1.0, 1.5)
+    u = oracle.rng_draw_f32(probe, 0, 1)  # first draw of site (0, 0) of replica 0
+    p = np.float32(before[0, 0, 0] + np.float32(1.5) * (np.float32(2) * u[0] - np.float32(1)))
+    p = p - np.float32(2 * np.pi) if p >= np.float32(np.pi) else (p + np.float32(2 * np.pi) if p < -np.float32(np.pi) else p)
+    assert theta[0, 0, 0] == p
+    if sign > 0:  # ordered start: every proposal is uphill at beta = 1e30
+        assert np.array_equal(theta[1], before[1])
