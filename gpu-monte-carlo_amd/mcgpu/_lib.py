@@ -2,9 +2,18 @@
 
 There is no fallback: if the shared library is missing or no HIP device is
 visible, the first GPU call raises.
+
+One HIP runtime per process: torch's ROCm libraries load their bundled
+libamdhip64 by its unversioned name from their own directory, which the
+dynamic linker does not match against an already loaded libamdhip64.so.7 of
+another path.  So when torch is installed, the loader first loads torch's
+runtime (without importing torch); libmcgpu then binds to it by soname, and a
+later `import torch` reuses the same file.  Device pointers and streams can
+then move freely between torch tensors and this library.
 """
 
 import ctypes
+import importlib.util
 import threading
 from pathlib import Path
 
@@ -57,6 +66,11 @@ SIGNATURES = {
     "mc_he3_energy": [P, INT, P, INT, INT, P, F64, F64, P],
     "mc_pt_exchange": [P, INT, P, P, P, P, P, P, P, INT, INT, I64],
     "mc_pt_record_walkers": [P, P, P, INT, P],
+    "mc_xy_twist_terms": [P, INT, P, INT, INT, P, P],
+    "mc_xy_twist_grad": [P, INT, P, INT, INT, P, P, P, P, P],
+    "mc_xy_twist_hvp": [P, INT, P, INT, INT, P, P, P, P, P, P],
+    "mc_kawamura_twist_terms": [P, INT, P, INT, INT, P, P],
+    "mc_kawamura_twist_grad": [P, INT, P, INT, INT, P, P, P, P, P],
     "mc_batch_workspace_size": [INT, INT, INT, INT, ctypes.POINTER(SZ), ctypes.POINTER(SZ),
                                 ctypes.POINTER(SZ)],
     "mc_batch_max_workgroups": [INT, INT, INT, INT, INT, ctypes.POINTER(INT)],
@@ -73,6 +87,18 @@ class MCError(RuntimeError):
     """HIP runtime failure inside libmcgpu."""
 
 
+def _torch_hip_runtime():
+    """Path of the HIP runtime bundled with torch, or None (torch is not imported)."""
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return None
+    if spec is None or not spec.submodule_search_locations:
+        return None
+    path = Path(list(spec.submodule_search_locations)[0]) / "lib" / "libamdhip64.so"
+    return path if path.exists() else None
+
+
 def load():
     """Load libmcgpu.so and declare its signatures (no device is touched)."""
     global _lib
@@ -84,6 +110,9 @@ def load():
                 raise ImportError(
                     f"{LIB_PATH} is missing: build it with `python -m mcgpu.build` "
                     "(or __graft_entry__.build()); there is no CPU fallback")
+            runtime = _torch_hip_runtime()
+            if runtime is not None:
+                ctypes.CDLL(str(runtime), mode=ctypes.RTLD_GLOBAL)
             lib = ctypes.CDLL(str(LIB_PATH))
             for name, args in SIGNATURES.items():
                 fn = getattr(lib, name)

@@ -157,6 +157,32 @@ int mc_pt_exchange(void *stream, int beta_dtype, const double *energies, void *b
 int mc_pt_record_walkers(void *stream, const int32_t *walker_to_temperature,
                          const int32_t *walker_ids, int n, int32_t *history_row);
 
+/* -------------------------------------------- differentiable energies
+ * No reference counterpart: the reference measures the twist sums only as
+ * observables (models/xy/measure.py:17-68, models/kawamura/measure.py:14-70,
+ * used by models/xy/analysis.py:47-54 and models/kawamura/analysis.py:36-53).
+ * With a twist Delta_r on the x bonds (axis 1) and bond weights w (1 for XY,
+ * (1 + tau tau')(d . d') for Kawamura), a = theta - theta_neighbour:
+ *   Cx = sum_x w cos(a - Delta), Cy = sum_y w cos(a), Sx = sum_x w sin(a - Delta),
+ *   H = -J (Cx + Cy).
+ * twist may be NULL (all zero); every coefficient array may be NULL (zero).
+ * terms: out is (3, R) float64 = (Cx, Cy, Sx).
+ * grad: per-site state gradient of alpha_r Cx + beta_r Cy + gamma_r Sx, same
+ * layout and dtype as the state.
+ * hvp (XY): per-site Hessian of that combination times u. */
+int mc_xy_twist_terms(void *stream, int dtype, const void *theta, int R, int L, const double *twist,
+                      double *out);
+int mc_xy_twist_grad(void *stream, int dtype, const void *theta, int R, int L, const double *twist,
+                     const double *alpha, const double *beta, const double *gamma, void *grad);
+int mc_xy_twist_hvp(void *stream, int dtype, const void *theta, int R, int L, const double *twist,
+                    const double *alpha, const double *beta, const double *gamma, const void *u,
+                    void *out);
+int mc_kawamura_twist_terms(void *stream, int dtype, const void *state, int R, int L,
+                            const double *twist, double *out);
+int mc_kawamura_twist_grad(void *stream, int dtype, const void *state, int R, int L,
+                           const double *twist, const double *alpha, const double *beta,
+                           const double *gamma, void *grad);
+
 /* ---------------------------------------------- batched persistent path
  * No single reference function: this runs core/simulation.py:36-72's loop
  * (sweeps from core/runtime.py:87-99, exchanges from
