@@ -14,6 +14,11 @@ Models:
 * ``XYModel(dtype=float32)``: the float32 fast update (dE from cached cos/sin
   of every site, float32 sincos/exp); energies are compensated float32 sums
   per thread, float64 across threads and strips.
+* ``XYModel(dtype=float32, proposal="gaussian")``: the same with Gaussian
+  proposals of per-temperature width; ``adapt_sigmas`` tunes the widths
+  toward a target acceptance during thermalisation.
+Accepted moves are counted per ladder temperature for every model
+(``acceptance_by_temperature``).
 
 ``BatchedSimulation`` runs several lattice sizes (a finite-size-scaling set)
 concurrently, one stream each, and returns the reference's run dictionaries
@@ -29,11 +34,12 @@ import numpy as np
 from mcgpu import (_lib, as_pointer, create_xoroshiro128p_states, device_array, device_zeros,
                    synchronize, to_device)
 
-MODEL_ISING, MODEL_XY_FAST = 0, 1
+MODEL_ISING, MODEL_XY_FAST, MODEL_XY_GAUSS = 0, 1, 2
 CHANNELS = 5
 NAMES = {
     MODEL_ISING: ("energy", "magnetization"),
     MODEL_XY_FAST: ("energy", "mx", "my", "twist_cos_x", "twist_sin_x"),
+    MODEL_XY_GAUSS: ("energy", "mx", "my", "twist_cos_x", "twist_sin_x"),
 }
 
 _tuned = {}
@@ -47,7 +53,7 @@ def model_code(model):
     if isinstance(model, XYModel):
         if model.dtype != np.float32:
             raise TypeError("the batched XY path runs float32 sweeps; use XYModel(dtype=np.float32)")
-        return MODEL_XY_FAST
+        return MODEL_XY_GAUSS if getattr(model, "proposal", "uniform") == "gaussian" else MODEL_XY_FAST
     raise TypeError(f"no batched kernel for {type(model).__name__}")
 
 
@@ -134,6 +140,12 @@ class BatchedRun:
         self._parity = device_zeros(1, np.int32)
         self.sweeps = 0
         self.exchange_rounds = 0
+        self._accepted = device_zeros(R, np.uint64)
+        self._counted_sweeps = 0
+        self.sigmas = None
+        if self.code == MODEL_XY_GAUSS:
+            self._sigmas_host = np.broadcast_to(np.asarray(model.sigma, dtype=np.float64), (R,)).copy()
+            self.sigmas = to_device(self._sigmas_host.astype(np.float32))
 
         self.H = int(strip_rows) if strip_rows else default_strip_rows(
             self.code, L, R, exchange_interval)
@@ -191,13 +203,15 @@ class BatchedRun:
         _lib.call(
             "mc_batch_run", self._stream, self.code, self.L, self.R, self.H, as_pointer(self.state),
             as_pointer(self.rng_states), as_pointer(self.betas), float(self.model.J),
-            float(getattr(self.model, "delta", 0.0)), nsweeps, sweep0, self.exchange_interval,
+            float(getattr(self.model, "delta", 0.0)), as_pointer(self.sigmas), as_pointer(self._accepted),
+            nsweeps, sweep0, self.exchange_interval,
             as_pointer(self._parity), as_pointer(self.energies), as_pointer(self.walker_to_temperature),
             as_pointer(self.temperature_to_walker), as_pointer(self._pt_rng), as_pointer(self._swap_counts),
             as_pointer(self._trips), int(sample_interval or 0), int(sample_first), n_samples,
             as_pointer(partials), as_pointer(sw2t), as_pointer(self._halo), as_pointer(self._barrier),
             as_pointer(self._epart), as_pointer(self._error))
         self.sweeps += nsweeps
+        self._counted_sweeps += nsweeps
         if self.exchange_interval:
             self.exchange_rounds += (sweep0 + nsweeps) // self.exchange_interval - sweep0 // self.exchange_interval
         return (partials, sw2t, n_samples, sample_first, sample_interval)
@@ -232,6 +246,47 @@ class BatchedRun:
 
     def run(self, nsweeps, sample_interval=0, sample_first=None):
         return self.collect(self.launch(nsweeps, sample_interval, sample_first))
+
+    # -------------------------------------------------- proposals, acceptance
+    def acceptance_by_temperature(self):
+        """Accepted fraction of site updates per ladder temperature since the last reset."""
+        self.synchronize()
+        acc = self._accepted.copy_to_host().astype(np.float64)
+        attempts = float(self.L * self.L * self._counted_sweeps)
+        return acc / attempts if attempts else np.full(self.R, np.nan)
+
+    def reset_acceptance(self):
+        self._accepted.fill_zero()
+        self._counted_sweeps = 0
+
+    def set_sigmas(self, sigmas):
+        from models.xy.proposals import SIGMA_MAX, SIGMA_MIN
+        if self.code != MODEL_XY_GAUSS:
+            raise ValueError("proposal widths exist only for Gaussian XY proposals")
+        sig = np.clip(np.broadcast_to(np.asarray(sigmas, dtype=np.float64), (self.R,)), SIGMA_MIN, SIGMA_MAX)
+        self._sigmas_host = sig.copy()
+        self.sigmas.copy_to_device(sig.astype(np.float32))
+
+    def adapt_sigmas(self, sweeps, interval=50, target=0.5, rate=1.0):
+        """Thermalise `sweeps` sweeps, moving each temperature's width toward `target`
+        acceptance every `interval` sweeps (log sigma += rate (acceptance - target)).
+        Returns [(sigmas, acceptance)] per step; widths stay fixed afterwards."""
+        if self.code != MODEL_XY_GAUSS:
+            raise ValueError("proposal widths exist only for Gaussian XY proposals")
+        if interval < 1:
+            raise ValueError("interval must be positive")
+        history = []
+        self.reset_acceptance()
+        done = 0
+        while done < sweeps:
+            n = min(interval, sweeps - done)
+            self.run(n)
+            done += n
+            rates = self.acceptance_by_temperature()
+            self.set_sigmas(self._sigmas_host * np.exp(rate * (rates - target)))
+            history.append((self._sigmas_host.copy(), rates))
+            self.reset_acceptance()
+        return history
 
     # ---------------------------------------------------------- inspection
     def snapshot(self):
@@ -268,6 +323,8 @@ class BatchedRun:
             "walker_round_trip_total_sweeps": trips[4].copy(),
             "mean_round_trip_sweeps": float(trips[4].sum()) / completed if completed else None,
             "strip_rows": self.H,
+            "acceptance_by_temperature": self.acceptance_by_temperature(),
+            "sigmas": None if self.sigmas is None else self._sigmas_host.copy(),
         }
 
 

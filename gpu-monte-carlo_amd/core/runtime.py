@@ -43,6 +43,11 @@ class Runtime:
         beta_dtype = np.dtype(getattr(model, "beta_dtype", np.float32))
         self.betas = to_device((1.0 / temperatures).astype(beta_dtype))
 
+        # Models with per-ladder device tables (Gaussian XY proposals) bind them here.
+        attach = getattr(model, "attach", None)
+        if attach is not None:
+            attach(self)
+
         # Reference launch shape; kept for API compatibility.
         self.threads_per_block = (threads, threads, 1)
         half_L = L // 2

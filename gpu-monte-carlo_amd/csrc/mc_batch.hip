@@ -40,6 +40,10 @@
 //    per-colour kernels (thresholds exp(-beta dE) recomputed per beta change).
 //  * MODEL_XY_FAST: the XY update with dE from cached (cos, sin) of every site
 //    and the bit-reproducible float32 sincos/exp of mc_fastmath.h.
+//  * MODEL_XY_GAUSS: the same with a Gaussian proposal theta + sigma_t z,
+//    z = sqrt(-2 log u1) cos(2 pi u2), sigma_t per ladder temperature (it
+//    follows the walker's temperature through exchanges).
+// Accepted moves are counted per ladder temperature when `accepted` is given.
 #include <algorithm>
 
 #include "mc_common.h"
@@ -47,7 +51,7 @@
 
 namespace mc {
 
-enum { MODEL_ISING = 0, MODEL_XY_FAST = 1 };
+enum { MODEL_ISING = 0, MODEL_XY_FAST = 1, MODEL_XY_GAUSS = 2 };
 enum { BATCH_ERR_WATCHDOG = 1 };
 
 constexpr int kBatchThreads = 256;
@@ -60,6 +64,8 @@ struct BatchArgs {
     uint64_t *rng;              // R*L*L records
     float *betas;               // (R) walker betas, updated at exchanges
     float J, delta;
+    const float *sigmas;        // (R) per temperature, MODEL_XY_GAUSS
+    unsigned long long *accepted;  // (R) per temperature, may be NULL
     int nsweeps;
     long long sweep0;           // global sweep count before this launch
     // parallel tempering (exchange_interval == 0: none)
@@ -226,7 +232,9 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
     uint64_t *pt_g = reinterpret_cast<uint64_t *>(pt_e + R);             // 2 (R-1)
     __shared__ double red[kChannels][kBatchThreads / 64];
     __shared__ float thr[5];
-    __shared__ float my_beta;
+    __shared__ float my_beta, my_sigma;
+    __shared__ int my_t;
+    __shared__ int red_i[kBatchThreads / 64];
 
     const size_t rep = (size_t)r * L * L;
     const int row0 = s * H;  // first global row of the strip
@@ -258,13 +266,19 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
         }
         for (int k = tid; k < 2 * (R - 1); k += kBatchThreads) pt_g[k] = a.pt_rng[k];
     }
-    if (tid == 0) my_beta = a.betas[r];
+    if (tid == 0) {
+        my_beta = a.betas[r];
+        my_t = use_pt ? a.w2t[r] : r;
+        my_sigma = MODEL == MODEL_XY_GAUSS ? a.sigmas[my_t] : 0.0f;
+    }
     __syncthreads();
     if (MODEL == MODEL_ISING && tid < 5) thr[tid] = ising_threshold(4 * tid - 8, my_beta, a.J);
     int parity = use_pt ? *a.parity : 0;
     __syncthreads();
 
     const float J = a.J, delta = a.delta;
+    float sigma = 0.0f;  // my_sigma for the current sweep
+    int n_acc = 0;       // accepted moves of this thread since the last flush
     const float one = 1.0f, two = 2.0f;
     const float pi = 3.14159274101257324f, two_pi = 6.28318548202514648f;
     // fixed thread -> (column, colour-row offset) map of every pass
@@ -280,6 +294,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
         const int lf = p ? base + tcol : base + half + (tcol == 0 ? half - 1 : tcol - 1);
         const int rt = p ? base + (tcol == half - 1 ? 0 : tcol + 1) : base + half + tcol;
         const int own = self - L;  // RNG and angle arrays start at local row 1
+        const int counted = !((c < 2) & (ri == HH));  // the ghost row's moves are its owner's
         const ulonglong2 st = lrng[own];
         Rng g;
         g.s0 = st.x;
@@ -288,15 +303,36 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
             const int sv = lsp[self];
             const int nb = (int)lsp[up] + (int)lsp[dn] + (int)lsp[lf] + (int)lsp[rt];
             const float u = g.uniform_f32();
-            if (u < thr[(2 * sv * nb + 8) >> 2]) lsp[self] = (int8_t)(-sv);
+            if (u < thr[(2 * sv * nb + 8) >> 2]) {
+                lsp[self] = (int8_t)(-sv);
+                n_acc += counted;
+            }
         } else {
             const float th = lth[own];
-            const float u = g.uniform_f32();
-            float pr = th + delta * (two * u - one);
-            if (pr >= pi)
-                pr -= two_pi;
-            else if (pr < -pi)
-                pr += two_pi;
+            float pr;
+            if (MODEL == MODEL_XY_GAUSS) {
+                float u1 = g.uniform_f32();
+                const float u2 = g.uniform_f32();
+                if (u1 <= 0.0f) u1 = 0x1p-53f;
+                float s2, c2;
+                fast_sincosf(two_pi * u2, s2, c2);
+                const float z = sqrtf(-2.0f * fast_logf(u1)) * c2;
+                pr = th + sigma * z;
+                if (pr >= pi || pr < -pi) {  // whole turns, then the last one's rounding
+                    pr -= two_pi * floorf((pr + pi) / two_pi);
+                    if (pr >= pi)
+                        pr -= two_pi;
+                    else if (pr < -pi)
+                        pr += two_pi;
+                }
+            } else {
+                const float u = g.uniform_f32();
+                pr = th + delta * (two * u - one);
+                if (pr >= pi)
+                    pr -= two_pi;
+                else if (pr < -pi)
+                    pr += two_pi;
+            }
             float sp, cp;
             fast_sincosf(pr, sp, cp);
             const float Cs = ((lc[up] + lc[dn]) + lc[lf]) + lc[rt];
@@ -309,6 +345,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
                 lth[own] = pr;
                 lc[self] = cp;
                 ls[self] = sp;
+                n_acc += counted;
             }
         }
         ulonglong2 out;
@@ -422,6 +459,24 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
         }
     };
 
+    // Add this workgroup's accepted moves to the counter of its temperature.
+    // The ghost row's moves belong to the strip below and are not counted.
+    // Called by every thread (contains barriers).
+    auto flush_accepted = [&]() {
+        if (!a.accepted) return;
+        int x = n_acc;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if ((tid & 63) == 0) red_i[tid >> 6] = x;
+        __syncthreads();
+        if (tid == 0) {
+            const int total = ((red_i[0] + red_i[1]) + red_i[2]) + red_i[3];
+            if (total) atomicAdd(&a.accepted[my_t], (unsigned long long)total);
+        }
+        __syncthreads();
+        n_acc = 0;
+    };
+
     // One sweep = colours 0, 1, 2, 3 in the reference order.  A site's update
     // reads only neighbouring colours, so any order in which
     //   c1(row) follows c0(row), c2(row) follows c0(rows +-1),
@@ -438,6 +493,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
     for (int n = 0; n < a.nsweeps; n++) {
         const long long sweep_global = a.sweep0 + n + 1;
         const float beta = my_beta;
+        sigma = my_sigma;
         pass(Seg{0, 1, HH - 1}, none, none, none, beta);
         __syncthreads();
         pass(Seg{1, 1, HH - 1}, none, none, none, beta);
@@ -484,7 +540,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
                     qx += bxs;
                 }
             }
-            if (MODEL == MODEL_XY_FAST) e = (double)ek - (double)ec;
+            if (MODEL != MODEL_ISING) e = (double)ek - (double)ec;
             v[0] = e;
             v[1] = m1;
             v[2] = m2;
@@ -494,6 +550,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
         block_sum<kChannels>(v, red);
 
         if (do_exchange) {
+            flush_accepted();  // moves so far belong to the current temperature
             const int buf = (int)((sweep_global / a.exchange_interval) & 1);
             double *ep = a.epart + ((size_t)buf * R + r) * S + s;
             if (tid == 0) __hip_atomic_store(ep, v[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -569,7 +626,11 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
                 for (int k = tid; k < 2 * (R - 1); k += kBatchThreads) a.pt_rng[k] = pt_g[k];
             }
             if (R > 2) parity = 1 - parity;
-            if (tid == 0) my_beta = pt_beta[r];
+            if (tid == 0) {
+                my_beta = pt_beta[r];
+                my_t = pt_w2t[r];
+                if (MODEL == MODEL_XY_GAUSS) my_sigma = a.sigmas[my_t];
+            }
             __syncthreads();
             if (MODEL == MODEL_ISING && tid < 5) thr[tid] = ising_threshold(4 * tid - 8, my_beta, a.J);
             __syncthreads();
@@ -591,6 +652,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
     }
 
 done:
+    flush_accepted();
     // ---- write back own rows: state and RNG streams
     __syncthreads();
     for (int idx = tid; idx < H * L; idx += kBatchThreads) {
@@ -626,12 +688,14 @@ static size_t batch_lds(int model, int L, int R, int H, int exchange) {
 }
 
 static const void *kernel_ptr(int model) {
-    return model == MODEL_ISING ? reinterpret_cast<const void *>(&batch_kernel<MODEL_ISING>)
-                                : reinterpret_cast<const void *>(&batch_kernel<MODEL_XY_FAST>);
+    if (model == MODEL_ISING) return reinterpret_cast<const void *>(&batch_kernel<MODEL_ISING>);
+    if (model == MODEL_XY_FAST) return reinterpret_cast<const void *>(&batch_kernel<MODEL_XY_FAST>);
+    return reinterpret_cast<const void *>(&batch_kernel<MODEL_XY_GAUSS>);
 }
 
 static int check_shape(int model, int L, int R, int H) {
-    MC_REQUIRE(model == MODEL_ISING || model == MODEL_XY_FAST, "unknown batch model %d", model);
+    MC_REQUIRE(model == MODEL_ISING || model == MODEL_XY_FAST || model == MODEL_XY_GAUSS,
+               "unknown batch model %d", model);
     MC_REQUIRE(L >= 4 && L % 2 == 0, "L must be even and >= 4, got %d", L);
     MC_REQUIRE(H >= 4 && H % 2 == 0 && L % H == 0, "strip height %d must be even, >= 4 and divide L=%d",
                H, L);
@@ -676,7 +740,7 @@ int mc_batch_max_workgroups(int model, int L, int R, int H, int exchange, int *m
 }
 
 int mc_batch_run(void *stream, int model, int L, int R, int H, void *state, uint64_t *rng, float *betas,
-                 float J, float delta, int nsweeps, long long sweep0, int exchange_interval, int *parity,
+                 float J, float delta, const float *sigmas, unsigned long long *accepted, int nsweeps, long long sweep0, int exchange_interval, int *parity,
                  double *energies, int32_t *w2t, int32_t *t2w, uint64_t *pt_rng, long long *counts,
                  long long *trips, int sample_interval, long long sample_first, int max_samples,
                  double *sample_partials, int32_t *sample_w2t, uint64_t *halo, unsigned *barrier,
@@ -687,6 +751,7 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, void *state, uint
     MC_REQUIRE(exchange_interval >= 0, "exchange_interval must be nonnegative");
     MC_REQUIRE(exchange_interval == 0 || R >= 2, "exchanges need at least two replicas");
     MC_REQUIRE(sample_interval >= 0 && max_samples >= 0, "invalid sampling arguments");
+    MC_REQUIRE(model != MODEL_XY_GAUSS || sigmas, "the Gaussian XY model needs per-temperature sigmas");
     const int S = L / H;
     const size_t lds = batch_lds(model, L, R, H, exchange_interval > 0);
     MC_REQUIRE(lds <= 160 * 1024, "batch workgroup needs %zu bytes of LDS", lds);
@@ -695,6 +760,7 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, void *state, uint
     BatchArgs a;
     a.model = model; a.L = L; a.R = R; a.H = H; a.S = S;
     a.state = state; a.rng = rng; a.betas = betas; a.J = J; a.delta = delta;
+    a.sigmas = sigmas; a.accepted = accepted;
     a.nsweeps = nsweeps; a.sweep0 = sweep0;
     a.exchange_interval = exchange_interval; a.parity = parity; a.energies = energies;
     a.w2t = w2t; a.t2w = t2w; a.pt_rng = pt_rng; a.counts = counts; a.trips = trips;

@@ -1,8 +1,10 @@
-// float32 sin/cos and exp for the batched fast path, bit-reproducible on the
-// host compiler: Cody-Waite reduction (pi/2 in three parts, ln 2 in two)
+// float32 sin/cos, exp and log for the batched fast path, bit-reproducible on
+// the host compiler: Cody-Waite reduction (pi/2 in three parts, ln 2 in two)
 // with fused multiply-adds, then Taylor polynomials of sin, cos (to x^9,
-// x^10) and exp (to x^7) in Horner form.  Errors: sin/cos <= 1.5 ulp on
-// [-2 pi, 2 pi], exp <= 1 ulp for normal results.
+// x^10) and exp (to x^7) in Horner form; log splits x = 2^e m with
+// m in [sqrt(1/2), sqrt(2)) and sums the atanh series of s = (m-1)/(m+1) to
+// s^11.  Errors: sin/cos <= 1.5 ulp on [-2 pi, 2 pi], exp <= 1 ulp for
+// normal results, log <= 1.5 ulp for normal x.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -49,6 +51,28 @@ __device__ __forceinline__ float fast_expf(float x) {
         e += 64;
     }
     return p * __int_as_float((e + 127) << 23);
+}
+
+// log for positive normal x (the engine calls it on uniforms in (0, 1])
+__device__ __forceinline__ float fast_logf(float x) {
+    const unsigned b = __float_as_uint(x);
+    int e = (int)(b >> 23) - 127;
+    float m = __uint_as_float((b & 0x7fffffu) | 0x3f800000u);  // [1, 2)
+    if (m > 0x1.6a09e6p+0f) {                                  // sqrt(2)
+        m *= 0.5f;
+        e += 1;
+    }
+    const float f = m - 1.0f;
+    const float s = f / (2.0f + f);
+    const float z = s * s;
+    float p = fmaf(z, 0x1.745d18p-3f, 0x1.c71c72p-3f);  // 2/11, 2/9
+    p = fmaf(z, p, 0x1.24924ap-2f);                     // 2/7
+    p = fmaf(z, p, 0x1.99999ap-2f);                     // 2/5
+    p = fmaf(z, p, 0x1.555556p-1f);                     // 2/3
+    const float r = z * p;
+    const float lm = fmaf(-s, f - r, f);  // 2 atanh(s) = f - s (f - r)
+    const float ef = (float)e;
+    return fmaf(ef, 0x1.62e430p-1f, fmaf(ef, -0x1.05c610p-29f, lm));
 }
 
 }  // namespace mc

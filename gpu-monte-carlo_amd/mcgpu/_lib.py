@@ -55,6 +55,7 @@ SIGNATURES = {
     "mc_ising_measure": [P, P, INT, INT, F32, P, P, P],
     "mc_ising_energy": [P, P, INT, INT, F32, P],
     "mc_xy_update_color": [P, INT, P, P, P, INT, INT, INT, F64, F64],
+    "mc_xy_gauss_update_color": [P, INT, P, P, P, INT, INT, INT, F64, P, P, INT, P],
     "mc_xy_measure": [P, INT, P, INT, INT, F64, P, P, P],
     "mc_xy_energy": [P, INT, P, INT, INT, F64, P],
     "mc_kawamura_update_color": [P, INT, P, P, P, INT, INT, INT, F64, F64],
@@ -74,7 +75,7 @@ SIGNATURES = {
     "mc_batch_workspace_size": [INT, INT, INT, INT, ctypes.POINTER(SZ), ctypes.POINTER(SZ),
                                 ctypes.POINTER(SZ)],
     "mc_batch_max_workgroups": [INT, INT, INT, INT, INT, ctypes.POINTER(INT)],
-    "mc_batch_run": [P, INT, INT, INT, INT, P, P, P, F32, F32, INT, I64, INT, P, P, P, P, P, P, P,
+    "mc_batch_run": [P, INT, INT, INT, INT, P, P, P, F32, F32, P, P, INT, I64, INT, P, P, P, P, P, P, P,
                      INT, I64, INT, P, P, P, P, P, P],
 }
 

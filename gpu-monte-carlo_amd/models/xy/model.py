@@ -3,6 +3,10 @@
 ``dtype=np.float32`` is the reference precision (float32 angles, betas and
 sweep arithmetic).  ``dtype=np.float64`` runs the same algorithm in float64:
 float64 angles and betas and float64 uniforms (53-bit draws).
+
+``proposal="uniform"`` (default) is the reference's window theta + delta
+(2u - 1).  ``proposal="gaussian"`` draws theta + sigma_t z with a width per
+ladder temperature (models/xy/proposals.py); a Runtime attaches the tables.
 """
 
 import math
@@ -14,7 +18,8 @@ from models._device import require_device
 
 from . import measure as _measure
 from .measure import MEASUREMENT_NAMES
-from .update import metropolis_color
+from .proposals import GaussianTables, ProposalRegistry
+from .update import gaussian_color, metropolis_color
 
 
 def random_angles(shape, seed, dtype):
@@ -28,7 +33,7 @@ def random_angles(shape, seed, dtype):
 
 class XYModel(Model):
 
-    def __init__(self, J=1.0, delta=1.5, dtype=np.float32, start="ordered"):
+    def __init__(self, J=1.0, delta=1.5, dtype=np.float32, start="ordered", proposal="uniform", sigma=1.0):
         if J <= 0:
             raise ValueError("J must be positive")
         if not (0 < delta <= math.pi):
@@ -43,6 +48,24 @@ class XYModel(Model):
         self.J = dtype.type(J)
         self.delta = dtype.type(delta)
         self.start = start
+        if proposal not in ("uniform", "gaussian"):
+            raise ValueError("proposal must be 'uniform' or 'gaussian'")
+        sig = np.asarray(sigma, dtype=np.float64)
+        if not np.all((sig > 0) & np.isfinite(sig)):
+            raise ValueError("sigma must be positive and finite")
+        self.proposal = proposal
+        self.sigma = sig
+        self._proposals = ProposalRegistry()
+
+    def attach(self, runtime):
+        """Gaussian proposals: per-temperature width tables for this runtime's ladder."""
+        if self.proposal == "gaussian":
+            self._proposals.attach(runtime.betas, GaussianTables(runtime.temperatures, runtime.L, self.sigma,
+                                                                 self.dtype))
+
+    def proposal_tables(self, runtime):
+        """Width tables and acceptance counters of a runtime (Gaussian proposals)."""
+        return self._proposals.get(runtime.betas)
 
     def initialize_state(self, L, replicas, seed):
         if self.start == "random":
@@ -51,7 +74,12 @@ class XYModel(Model):
         return np.zeros((replicas, L, L), dtype=self.dtype)
 
     def update_color(self, state, rng_states, betas, color, blocks, threads):
-        metropolis_color(state, rng_states, betas, color, self.J, self.delta)
+        if self.proposal == "gaussian":
+            tables = self._proposals.get(betas)
+            gaussian_color(state, rng_states, betas, color, self.J, tables)
+            tables.count_colour()
+        else:
+            metropolis_color(state, rng_states, betas, color, self.J, self.delta)
 
     def measure(self, state, temperature_indices=None, cached_energies=None):
         """Reduce the GPU state to the five host measurement arrays."""
@@ -64,6 +92,8 @@ class XYModel(Model):
                             cached_energies=energies)
 
     def parameters(self):
+        if self.proposal == "gaussian":
+            return {"J": self.J, "proposal": "gaussian", "sigma": self.sigma.tolist()}
         return {"J": self.J, "delta": self.delta}
 
     def measure_energy(self, state, output):

@@ -99,6 +99,17 @@ int mc_ising_energy(void *stream, const int8_t *spins, int R, int L, float J, do
 /* models/xy/update.py:6-54; betas have the state's dtype */
 int mc_xy_update_color(void *stream, int dtype, void *theta, uint64_t *rng, const void *betas,
                        int R, int L, int color, double J, double delta);
+/* Gaussian-proposal XY update; no reference counterpart (the reference's
+ * uniform window is models/xy/update.py:24-31, replaced here by
+ * theta + sigma_t z, z = sqrt(-2 log u1) cos(2 pi u2), wrapped by whole turns;
+ * dE and acceptance as models/xy/update.py:33-54 with a third draw).
+ * table_betas and sigmas (T entries, state dtype) give each ladder
+ * temperature's sigma; a walker uses the temperature whose beta is nearest
+ * to its own.  accepted (T entries, may be NULL) counts accepted moves per
+ * temperature. */
+int mc_xy_gauss_update_color(void *stream, int dtype, void *theta, uint64_t *rng, const void *betas,
+                             int R, int L, int color, double J, const void *table_betas,
+                             const void *sigmas, int T, unsigned long long *accepted);
 /* models/xy/measure.py:17-68 (energy, mx, my, twist_cos_x, twist_sin_x) */
 int mc_xy_measure(void *stream, int dtype, const void *theta, int R, int L, double J,
                   const int32_t *temperature_indices, const double *cached_energies,
@@ -189,16 +200,20 @@ int mc_kawamura_twist_grad(void *stream, int dtype, const void *state, int R, in
  * core/parallel_tempering.py:245-269, samples every sample_interval) for
  * one lattice size inside one cooperative launch.  model 0 = Ising (the
  * reference update, bit-identical to mc_ising_update_color), model 1 = XY
- * float32 fast update (cached cos/sin, float32 sincos/exp).  The state is
- * split into L/H strips of H rows per replica; each workgroup keeps its
- * strip on chip.  Workspace sizes come from mc_batch_workspace_size. */
+ * float32 fast update (cached cos/sin, float32 sincos/exp), model 2 = the
+ * same with Gaussian proposals (sigmas: R entries per ladder temperature).
+ * accepted (R entries per ladder temperature, may be NULL) counts accepted
+ * moves.  The state is split into L/H strips of H rows per replica; each
+ * workgroup keeps its strip on chip.  Workspace sizes come from
+ * mc_batch_workspace_size. */
 int mc_batch_workspace_size(int model, int L, int R, int H, size_t *halo_words,
                             size_t *barrier_words, size_t *epart_doubles);
 /* Largest grid (R * L / H workgroups) that can be co-resident for this
  * model and strip height. */
 int mc_batch_max_workgroups(int model, int L, int R, int H, int exchange, int *max_workgroups);
 int mc_batch_run(void *stream, int model, int L, int R, int H, void *state, uint64_t *rng,
-                 float *betas, float J, float delta, int nsweeps, long long sweep0,
+                 float *betas, float J, float delta, const float *sigmas,
+                 unsigned long long *accepted, int nsweeps, long long sweep0,
                  int exchange_interval, int *parity, double *energies, int32_t *walker_to_temperature,
                  int32_t *temperature_to_walker, uint64_t *pt_rng, long long *counts,
                  long long *trips, int sample_interval, long long sample_first, int max_samples,

@@ -20,12 +20,12 @@ __all__ = [
     "load", "build", "splitmix64", "rng_init", "rng_draw_u64", "rng_draw_f32",
     "rng_draw_f64", "rng_jump", "set_threads",
     "ising_color", "ising_sweeps", "ising_measure", "ising_energy",
-    "xy_color", "xy_sweeps", "xy_measure", "xy_energy",
+    "xy_color", "xy_gauss_color", "xy_sweeps", "xy_measure", "xy_energy",
     "kawamura_local", "kawamura_color", "kawamura_sweeps", "kawamura_measure",
     "kawamura_energy",
     "he3_potential", "he3_gradient_difference", "he3_local_energy", "he3_color",
     "he3_sweeps", "he3_gaussian_color", "he3_measure", "he3_energy",
-    "pt_exchange", "fast_sincosf", "fast_expf", "xyfast_cache", "xyfast_color",
+    "pt_exchange", "fast_sincosf", "fast_expf", "fast_logf", "xyfast_cache", "xyfast_color", "xyfast_gauss_color",
     "xyfast_sweeps", "xyfast_observables",
 ]
 
@@ -70,6 +70,8 @@ def _declare(lib):
         "orc_ising_energy": (None, [P, I32, I32, F32, P]),
         "orc_xy_color_f32": (None, [P, P, P, I32, I32, I32, F32, F32]),
         "orc_xy_color_f64": (None, [P, P, P, I32, I32, I32, F64, F64]),
+        "orc_xy_gauss_color_f32": (None, [P, P, P, I32, I32, I32, F32, P, P, I32, P]),
+        "orc_xy_gauss_color_f64": (None, [P, P, P, I32, I32, I32, F64, P, P, I32, P]),
         "orc_xy_sweeps_f32": (None, [P, P, P, I32, I32, F32, F32, I64]),
         "orc_xy_sweeps_f64": (None, [P, P, P, I32, I32, F64, F64, I64]),
         "orc_xy_measure_f32": (None, [P, I32, I32, F64, P, P, P]),
@@ -105,9 +107,11 @@ def _declare(lib):
         "orc_fast_expf": (F32, [F32]),
         "orc_fast_sincosf_n": (None, [P, I64, P, P]),
         "orc_fast_expf_n": (None, [P, I64, P]),
+        "orc_fast_logf_n": (None, [P, I64, P]),
         "orc_xyfast_init_cache": (None, [P, I64, P, P]),
         "orc_xyfast_color": (None, [P, P, P, P, P, I32, I32, I32, F32, F32]),
         "orc_xyfast_sweeps": (None, [P, P, P, P, P, I32, I32, F32, F32, I64]),
+        "orc_xyfast_gauss_color": (None, [P, P, P, P, P, P, I32, I32, I32, F32, P]),
         "orc_xyfast_observables": (None, [P, P, I32, I32, F32, P]),
         "orc_pt_exchange_f64": (None, [P, P, P, P, P, P, P, I32, I32, I64]),
     }
@@ -225,6 +229,18 @@ def xy_color(theta, rng, betas, color, J, delta):
     R, L = theta.shape[0], theta.shape[1]
     getattr(load(), f"orc_xy_color_{sfx}")(_ptr(theta, real, True), _rng(rng), _ptr(betas, real),
                                            R, L, int(color), float(J), float(delta))
+
+
+def xy_gauss_color(theta, rng, betas, color, J, table_betas, sigmas, accepted=None):
+    """Gaussian-proposal colour update; `accepted` (int64, per temperature) is incremented."""
+    sfx, real = _real(theta)
+    R, L = theta.shape[0], theta.shape[1]
+    table_betas = np.ascontiguousarray(table_betas, dtype=real)
+    sigmas = np.ascontiguousarray(sigmas, dtype=real)
+    acc = None if accepted is None else _ptr(accepted, np.int64, True)
+    getattr(load(), f"orc_xy_gauss_color_{sfx}")(_ptr(theta, real, True), _rng(rng), _ptr(betas, real), R, L,
+                                                 int(color), float(J), table_betas.ctypes.data,
+                                                 sigmas.ctypes.data, len(sigmas), acc)
 
 
 def xy_sweeps(theta, rng, betas, J, delta, nsweeps):
@@ -403,6 +419,13 @@ def fast_expf(x):
     return y
 
 
+def fast_logf(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.empty_like(x)
+    load().orc_fast_logf_n(x.ctypes.data, x.size, y.ctypes.data)
+    return y
+
+
 def xyfast_cache(theta):
     """(cos, sin) cache arrays of the fast XY update."""
     c = np.empty_like(theta, dtype=np.float32)
@@ -416,6 +439,16 @@ def xyfast_color(theta, c, s, rng, betas, color, J, delta):
     load().orc_xyfast_color(_ptr(theta, np.float32, True), _ptr(c, np.float32, True),
                             _ptr(s, np.float32, True), _rng(rng), _ptr(betas, np.float32), R, L,
                             int(color), float(J), float(delta))
+
+
+def xyfast_gauss_color(theta, c, s, rng, betas, sigmas, color, J, accepted=None):
+    """Fast-path Gaussian colour update; sigmas and accepted are per walker."""
+    R, L = theta.shape[0], theta.shape[1]
+    acc = None if accepted is None else _ptr(accepted, np.int64, True)
+    sig = np.ascontiguousarray(sigmas, np.float32)  # keep alive across the call
+    load().orc_xyfast_gauss_color(_ptr(theta, np.float32, True), _ptr(c, np.float32, True),
+                                  _ptr(s, np.float32, True), _rng(rng), _ptr(betas, np.float32),
+                                  _ptr(sig, np.float32), R, L, int(color), float(J), acc)
 
 
 def xyfast_sweeps(theta, c, s, rng, betas, J, delta, nsweeps):

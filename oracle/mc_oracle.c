@@ -101,12 +101,16 @@ static inline float rsin_f32(float x) { return (float)sin((double)x); }
 static inline float rexp_f32(float x) { return (float)exp((double)x); }
 static inline float rlog_f32(float x) { return (float)log((double)x); }
 static inline float rsqrt_f32(float x) { return sqrtf(x); }
+static inline float rfloor_f32(float x) { return floorf(x); }
+static inline float rfabs_f32(float x) { return fabsf(x); }
 static inline float draw_f32(uint64_t *s) { return orc_uniform_f32(s); }
 static inline double rcos_f64(double x) { return cos(x); }
 static inline double rsin_f64(double x) { return sin(x); }
 static inline double rexp_f64(double x) { return exp(x); }
 static inline double rlog_f64(double x) { return log(x); }
 static inline double rsqrt_f64(double x) { return sqrt(x); }
+static inline double rfloor_f64(double x) { return floor(x); }
+static inline double rfabs_f64(double x) { return fabs(x); }
 static inline double draw_f64(uint64_t *s) { return orc_uniform_f64(s); }
 
 /* ---------------------------------------------------------------- Ising */

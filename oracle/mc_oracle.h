@@ -54,6 +54,12 @@ void orc_xy_color_f32(float *theta, uint64_t *rng, const float *betas,
                       int R, int L, int color, float J, float delta);
 void orc_xy_color_f64(double *theta, uint64_t *rng, const double *betas,
                       int R, int L, int color, double J, double delta);
+void orc_xy_gauss_color_f32(float *theta, uint64_t *rng, const float *betas, int R, int L, int color,
+                            float J, const float *table_betas, const float *sigmas, int T,
+                            int64_t *accepted);
+void orc_xy_gauss_color_f64(double *theta, uint64_t *rng, const double *betas, int R, int L, int color,
+                            double J, const double *table_betas, const double *sigmas, int T,
+                            int64_t *accepted);
 void orc_xy_sweeps_f32(float *theta, uint64_t *rng, const float *betas,
                        int R, int L, float J, float delta, int64_t nsweeps);
 void orc_xy_sweeps_f64(double *theta, uint64_t *rng, const double *betas,
@@ -131,6 +137,10 @@ void orc_fast_sincosf(float x, float *sn, float *cs);
 float orc_fast_expf(float x);
 void orc_fast_sincosf_n(const float *x, int64_t n, float *sn, float *cs);
 void orc_fast_expf_n(const float *x, int64_t n, float *y);
+float orc_fast_logf(float x);
+void orc_fast_logf_n(const float *x, int64_t n, float *y);
+void orc_xyfast_gauss_color(float *th, float *c, float *s, uint64_t *rng, const float *betas, const float *sig,
+                            int R, int L, int color, float J, int64_t *acc);
 void orc_xyfast_init_cache(const float *th, int64_t n, float *c, float *s);
 void orc_xyfast_color(float *th, float *c, float *s, uint64_t *rng, const float *betas, int R, int L,
                       int color, float J, float delta);

@@ -81,6 +81,35 @@ float orc_fast_expf(float x) {
     return p * two_e.f;
 }
 
+/* log for positive normal x: x = 2^e m, m in [sqrt(1/2), sqrt(2)), then
+ * log m = 2 atanh(s), s = (m-1)/(m+1), summed to s^11 as f - s (f - r) */
+float orc_fast_logf(float x) {
+    union { float f; uint32_t u; } v;
+    v.f = x;
+    int e = (int)(v.u >> 23) - 127;
+    v.u = (v.u & 0x7fffffu) | 0x3f800000u;
+    float m = v.f;
+    if (m > 0x1.6a09e6p+0f) {
+        m *= 0.5f;
+        e += 1;
+    }
+    const float f = m - 1.0f;
+    const float s = f / (2.0f + f);
+    const float z = s * s;
+    float p = fmaf(z, 0x1.745d18p-3f, 0x1.c71c72p-3f);
+    p = fmaf(z, p, 0x1.24924ap-2f);
+    p = fmaf(z, p, 0x1.99999ap-2f);
+    p = fmaf(z, p, 0x1.555556p-1f);
+    const float r = z * p;
+    const float lm = fmaf(-s, f - r, f);
+    const float ef = (float)e;
+    return fmaf(ef, LN2_HI, fmaf(ef, LN2_LO, lm));
+}
+
+void orc_fast_logf_n(const float *x, int64_t n, float *y) {
+    for (int64_t k = 0; k < n; k++) y[k] = orc_fast_logf(x[k]);
+}
+
 void orc_fast_sincosf_n(const float *x, int64_t n, float *sn, float *cs) {
     for (int64_t k = 0; k < n; k++) orc_fast_sincosf(x[k], sn + k, cs + k);
 }
@@ -129,6 +158,63 @@ void orc_xyfast_color(float *th, float *c, float *s, uint64_t *rng, const float 
                 }
             }
         }
+    }
+}
+
+/* Gaussian proposal of the fast path: u1, u2 -> z = sqrt(-2 log u1) cos(2 pi u2)
+ * with this file's log and sincos (u1 = 0 -> 2^-53), p = theta + sigma z
+ * wrapped to [-pi, pi) by whole turns, then the fast dE and acceptance with a
+ * third draw.  sig holds each walker's sigma; acc counts accepted moves per
+ * walker (may be NULL). */
+static float fast_wrap(float p) {
+    const float pi = (float)M_PI, two_pi = (float)(2.0 * M_PI);
+    if (p >= pi || p < -pi) {
+        p -= two_pi * floorf((p + pi) / two_pi);
+        if (p >= pi) p -= two_pi;
+        else if (p < -pi) p += two_pi;
+    }
+    return p;
+}
+
+void orc_xyfast_gauss_color(float *th, float *c, float *s, uint64_t *rng, const float *betas, const float *sig,
+                            int R, int L, int color, float J, int64_t *acc) {
+    const float two_pi = (float)(2.0 * M_PI);
+    const int h = L / 2;
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < R; r++) {
+        int64_t n_acc = 0;
+        for (int row = 0; row < h; row++) {
+            for (int col = 0; col < h; col++) {
+                const int i = 2 * row + color / 2, j = 2 * col + color % 2;
+                const size_t k = FSITE(L, r, i, j);
+                uint64_t *st = rng + 2 * k;
+                const float theta = th[k];
+                float u1 = orc_uniform_f32(st);
+                const float u2 = orc_uniform_f32(st);
+                if (u1 <= 0.0f) u1 = 0x1p-53f;
+                float s2, c2;
+                orc_fast_sincosf(two_pi * u2, &s2, &c2);
+                const float z = sqrtf(-2.0f * orc_fast_logf(u1)) * c2;
+                const float p = fast_wrap(theta + sig[r] * z);
+                float sp, cp;
+                orc_fast_sincosf(p, &sp, &cp);
+                const size_t ku = FSITE(L, r, (i + L - 1) % L, j), kd = FSITE(L, r, (i + 1) % L, j);
+                const size_t kl = FSITE(L, r, i, (j + L - 1) % L), kr = FSITE(L, r, i, (j + 1) % L);
+                const float C = ((c[ku] + c[kd]) + c[kl]) + c[kr];
+                const float S = ((s[ku] + s[kd]) + s[kl]) + s[kr];
+                float t = (cp - c[k]) * C;
+                t = fmaf(sp - s[k], S, t);
+                const float dE = -J * t;
+                const float v = orc_uniform_f32(st);
+                if (dE <= 0.0f || v < orc_fast_expf(-betas[r] * dE)) {
+                    th[k] = p;
+                    c[k] = cp;
+                    s[k] = sp;
+                    n_acc++;
+                }
+            }
+        }
+        if (acc) acc[r] += n_acc;
     }
 }
 

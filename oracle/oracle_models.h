@@ -58,6 +58,68 @@ void FN(orc_xy_color)(REAL *th, uint64_t *rng, const REAL *betas, int R, int L,
     }
 }
 
+/* Gaussian-proposal XY colour update (no reference counterpart; the
+ * reference draws a uniform window, models/xy/update.py:24-31): u1, u2 ->
+ * z = sqrt(-2 log u1) cos(2 pi u2) (u1 = 0 -> 2^-53), p = theta + sigma z
+ * wrapped to [-pi, pi) by whole turns, then the reference dE and acceptance
+ * with a third draw.  sigma comes from the ladder temperature whose beta is
+ * nearest to the walker's; accepted moves are counted per temperature. */
+static inline REAL FN(wrap_angle)(REAL p, REAL pi, REAL two_pi) {
+    if (p >= pi || p < -pi) {
+        p -= two_pi * FN(rfloor)((p + pi) / two_pi);
+        if (p >= pi) p -= two_pi;
+        else if (p < -pi) p += two_pi;
+    }
+    return p;
+}
+
+static inline int FN(nearest_temperature)(const REAL *table, int T, REAL beta) {
+    int best = 0;
+    REAL bd = FN(rfabs)(table[0] - beta);
+    for (int t = 1; t < T; t++) {
+        const REAL d = FN(rfabs)(table[t] - beta);
+        if (d < bd) { bd = d; best = t; }
+    }
+    return best;
+}
+
+void FN(orc_xy_gauss_color)(REAL *th, uint64_t *rng, const REAL *betas, int R, int L, int color, REAL J,
+                            const REAL *table_betas, const REAL *sigmas, int T, int64_t *accepted) {
+    const REAL two = (REAL)2, zero = (REAL)0;
+    const REAL pi = (REAL)M_PI, two_pi = (REAL)(2.0 * M_PI);
+    const int h = L / 2;
+    for (int r = 0; r < R; r++) {
+        const int t = FN(nearest_temperature)(table_betas, T, betas[r]);
+        const REAL sigma = sigmas[t];
+        int64_t acc = 0;
+        for (int row = 0; row < h; row++) {
+            for (int col = 0; col < h; col++) {
+                const int i = 2 * row + color / 2, j = 2 * col + color % 2;
+                const size_t k = SITE(L, r, i, j);
+                uint64_t *s = rng + 2 * k;
+                const REAL theta = th[k];
+                REAL u1 = FN(draw)(s);
+                const REAL u2 = FN(draw)(s);
+                if (u1 <= zero) u1 = (REAL)0x1p-53;
+                const REAL z = FN(rsqrt)(-two * FN(rlog)(u1)) * FN(rcos)(two_pi * u2);
+                const REAL p = FN(wrap_angle)(theta + sigma * z, pi, two_pi);
+                const REAL up = th[SITE(L, r, (i + L - 1) % L, j)];
+                const REAL down = th[SITE(L, r, (i + 1) % L, j)];
+                const REAL left = th[SITE(L, r, i, (j + L - 1) % L)];
+                const REAL right = th[SITE(L, r, i, (j + 1) % L)];
+                const REAL e_old = -J * (((FN(rcos)(theta - up) + FN(rcos)(theta - down))
+                                          + FN(rcos)(theta - left)) + FN(rcos)(theta - right));
+                const REAL e_new = -J * (((FN(rcos)(p - up) + FN(rcos)(p - down))
+                                          + FN(rcos)(p - left)) + FN(rcos)(p - right));
+                const REAL dE = e_new - e_old;
+                const REAL v = FN(draw)(s);
+                if (dE <= zero || v < FN(rexp)(-betas[r] * dE)) { th[k] = p; acc++; }
+            }
+        }
+        if (accepted) accepted[t] += acc;
+    }
+}
+
 /* core/runtime.py:87-99: colours 0..3 in order, per sweep */
 void FN(orc_xy_sweeps)(REAL *th, uint64_t *rng, const REAL *betas, int R, int L,
                        REAL J, REAL delta, int64_t nsweeps) {
