@@ -1,25 +1,29 @@
 #!/usr/bin/env python3
 """Headline benchmark: site-updates/ns on one MI355X, batched FP32 vs plain FP64.
 
-Workload (one lattice size, the XY benchmark configuration): XY model, J = 1,
-L = 128 periodic, 32 replicas on a geometric temperature ladder 0.7-1.1,
-angles uniform from seed 170, uniform-window proposals (delta = 1.5),
-replica exchange every 10 sweeps.  One step = 10 sweeps + 1 exchange.
+Workload (the XY benchmark configuration): XY model, J = 1, L = 128 periodic,
+32 replicas on a geometric temperature ladder 0.7-1.1, angles uniform from
+seed 170, Gaussian-MH angle proposals whose widths are adapted toward 0.5
+acceptance per temperature (2,000 sweeps, then frozen), replica exchange
+every 10 sweeps.  One step = 10 sweeps + 1 exchange.
 
 * value: the batched FP32 engine (core.batched.BatchedRun, one persistent
   launch for all timed steps), site-updates per ns of wall-clock time with the
   state already resident on the GPU.  A site-update is one visit of one site
   in one colour pass (L*L*R per sweep).
-* plain_fp64: the drop-in path (Runtime + ParallelTempering, float64 XY,
-  four colour launches per sweep from Python) on the same workload.
+* plain_fp64: the drop-in path (Runtime + ParallelTempering, float64 XY with
+  the same Gaussian widths, four colour launches per sweep from Python).
+* uniform_window: the batched engine with the reference's uniform window
+  (delta = 1.5) on the same lattice, for comparison.
 * roofline: algorithmic bytes of the batched kernel, 40 B per site-update
   (float32 angle read + write, 16 B xoroshiro128+ state read + write), over
   its HIP-event time, against 8 TB/s.  `traffic` is the HBM bytes of the timed
   launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE measurement
   (per sweep, scaled by a copy kernel of known size; profiles/) times the
-  launch's sweeps, when that measurement used the same strip height.
-* cpu_baseline: the CPU oracle's restatement of the reference XY update on
-  the same lattice, a bounded number of sweeps on the host cores.
+  launch's sweeps, when that measurement used the same workload and strip height.
+* cpu_baseline: the CPU oracle's restatement of the same Gaussian update
+  (float32, OpenMP over replicas) on the same lattice, a bounded number of
+  sweeps on the host cores.
 
 Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build measures one GPU.
 """
@@ -40,9 +44,10 @@ for p in (str(REPO / "gpu-monte-carlo_amd"), str(REPO)):
 
 METRIC = "site-updates/ns on 1 GPU (batched FP32 vs plain FP64) + % HBM roofline"
 L, R, SEED, EXCHANGE = 128, 32, 170, 10
+ADAPT_SWEEPS, ADAPT_INTERVAL = 2000, 100
 BYTES_PER_SITE_UPDATE = 40          # XY float32: 2 * 4 B angle + 2 * 16 B RNG state
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = REPO / "profiles" / "batch_kernel_xy_L128_R32.traffic.json"
+TRAFFIC_FILE = REPO / "profiles" / "batch_kernel_xy_gauss_L128_R32.traffic.json"
 
 
 def temperatures():
@@ -53,20 +58,32 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def bench_batched(steps, warmup, strip_rows, autotune):
+def _model(proposal, sigma=1.0, dtype=np.float32):
+    from models.xy.model import XYModel
+    if proposal == "gaussian":
+        return XYModel(J=1.0, dtype=dtype, start="random", proposal="gaussian", sigma=sigma)
+    return XYModel(J=1.0, delta=1.5, dtype=dtype, start="random")
+
+
+def bench_batched(steps, warmup, strip_rows, autotune, proposal="gaussian"):
     from core.batched import BatchedRun, autotune_strip_rows
     from mcgpu import synchronize
     from mcgpu.timing import EventTimer
-    from models.xy.model import XYModel
 
-    model = XYModel(J=1.0, delta=1.5, start="random")
+    model = _model(proposal)
     if strip_rows is None and autotune:
-        log("autotuning strip height")
+        log(f"autotuning strip height ({proposal})")
         strip_rows = autotune_strip_rows(model, L, temperatures(), seed=SEED, exchange_interval=EXCHANGE,
                                          sweeps=400, verbose=True)
     run = BatchedRun(model, L, temperatures(), seed=SEED, exchange_interval=EXCHANGE, strip_rows=strip_rows)
-    log(f"batched FP32: strip rows {run.H}, {run.R * run.strips} workgroups; warmup {warmup} steps")
+    sigmas = None
+    if proposal == "gaussian":
+        log(f"adapting Gaussian widths over {ADAPT_SWEEPS} sweeps")
+        run.adapt_sigmas(ADAPT_SWEEPS, interval=ADAPT_INTERVAL)
+        sigmas = run._sigmas_host.copy()
+    log(f"batched FP32 ({proposal}): strip rows {run.H}, {run.R * run.strips} workgroups; warmup {warmup} steps")
     run.run(warmup * EXCHANGE)
+    run.reset_acceptance()
     synchronize()
     timer = EventTimer(run._stream)
     sweeps = steps * EXCHANGE
@@ -82,25 +99,25 @@ def bench_batched(steps, warmup, strip_rows, autotune):
     energy = run.measure()["energy"]
     if not np.all(np.isfinite(energy)):
         raise RuntimeError("batched run produced nonfinite energies")
-    site_updates = sweeps * L * L * R
+    d = run.diagnostics()
     return {
         "wall_s": wall,
         "kernel_ms": kernel_ms,
-        "site_updates": site_updates,
+        "site_updates": sweeps * L * L * R,
         "strip_rows": run.H,
         "workgroups": run.R * run.strips,
-        "acceptance": run.diagnostics()["acceptance_rates"].mean(),
+        "pt_acceptance": float(d["acceptance_rates"].mean()),
+        "move_acceptance": d["acceptance_by_temperature"],
+        "sigmas": sigmas,
     }
 
 
-def bench_plain(dtype, sweeps, warmup=20):
+def bench_plain(dtype, sweeps, sigmas, warmup=20):
     from core.parallel_tempering import ParallelTempering
     from core.runtime import Runtime
     from mcgpu import synchronize
-    from models.xy.model import XYModel
 
-    model = XYModel(J=1.0, delta=1.5, dtype=dtype, start="random")
-    rt = Runtime(model, L, temperatures(), seed=SEED)
+    rt = Runtime(_model("gaussian", sigmas, dtype), L, temperatures(), seed=SEED)
     pt = ParallelTempering(rt, exchange_interval=EXCHANGE, seed=SEED + 1)
     for _ in range(warmup):
         pt.sweep()
@@ -136,19 +153,25 @@ def measured_traffic(strip_rows, sweeps):
     return float(t["traffic_bytes_per_sweep"]) * sweeps, t.get("summary")
 
 
-def cpu_baseline(target_s):
+def cpu_baseline(target_s, sigmas):
     import oracle
     threads = min(16, os.cpu_count() or 1)
     oracle.set_threads(threads)
-    from models.xy.model import XYModel
-    theta = XYModel(start="random").initialize_state(L, R, SEED)
+    theta = _model("gaussian").initialize_state(L, R, SEED)
     rng = oracle.rng_init(R * L * L, SEED)
     betas = (1.0 / temperatures()).astype(np.float32)
-    oracle.xy_sweeps(theta, rng, betas, 1.0, 1.5, 2)  # warm caches and threads
+    sig = np.asarray(sigmas, dtype=np.float32)
+
+    def sweep():
+        for color in range(4):
+            oracle.xy_gauss_color(theta, rng, betas, color, 1.0, betas, sig)
+
+    sweep()  # warm caches and threads
     sweeps = 0
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < target_s:
-        oracle.xy_sweeps(theta, rng, betas, 1.0, 1.5, EXCHANGE)
+        for _ in range(EXCHANGE):
+            sweep()
         oracle.xy_energy(theta, 1.0)
         sweeps += EXCHANGE
     wall = time.perf_counter() - t0
@@ -157,9 +180,9 @@ def cpu_baseline(target_s):
         "unit": "site-updates/ns",
         "cores": threads,
         "kind": "port",
-        "sample": (f"{sweeps} sweeps of the reference XY float32 update (C restatement, OpenMP over "
-                   f"replicas) on L={L}, R={R}, with a float64 energy pass every {EXCHANGE} sweeps; "
-                   f"{wall:.1f} s"),
+        "sample": (f"{sweeps} sweeps of the Gaussian-proposal XY float32 update (C restatement of the "
+                   f"engine's plain path, OpenMP over replicas) on L={L}, R={R} with the adapted widths, "
+                   f"plus a float64 energy pass every {EXCHANGE} sweeps; {wall:.1f} s"),
     }
 
 
@@ -172,6 +195,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-uniform", action="store_true")
     ap.add_argument("--strip-rows", type=int, default=None)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -211,8 +235,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": (f"XY L={L} R={R}, T geometric 0.7-1.1, random start seed {SEED}, delta 1.5, "
-                         f"exchange every {EXCHANGE} sweeps; batched FP32 persistent launch"),
+            "workload": (f"XY L={L} R={R}, T geometric 0.7-1.1, random start seed {SEED}, Gaussian-MH "
+                         f"proposals with widths adapted toward 0.5 acceptance ({ADAPT_SWEEPS} sweeps, then "
+                         f"frozen), exchange every {EXCHANGE} sweeps; batched FP32 persistent launch"),
             "L": L, "replicas": R, "sweeps_per_step": EXCHANGE,
             "strip_rows": b["strip_rows"], "workgroups": b["workgroups"],
             "device": name, "compute_units": cus,
@@ -230,12 +255,22 @@ def main():
             "kernel_ms": round(b["kernel_ms"], 3),
             "kernel_site_updates_per_ns": round(kernel_rate, 3),
         },
-        "acceptance_rate_pt": round(float(b["acceptance"]), 4),
+        "move_acceptance": [round(float(x), 4) for x in b["move_acceptance"]],
+        "sigmas": [round(float(x), 4) for x in b["sigmas"]],
+        "acceptance_rate_pt": round(b["pt_acceptance"], 4),
     }
+    if not args.no_uniform:
+        u = bench_batched(max(args.steps // 3, 1), max(args.warmup // 3, 1), args.strip_rows, not args.no_autotune,
+                          proposal="uniform")
+        out["uniform_window"] = {
+            "value": round(u["site_updates"] / (u["wall_s"] * 1e9), 4), "unit": "site-updates/ns",
+            "kernel_site_updates_per_ns": round(u["site_updates"] / (u["kernel_ms"] * 1e6), 3),
+            "strip_rows": u["strip_rows"],
+            "note": "batched FP32 with the reference's uniform window, delta = 1.5"}
     if not args.no_plain:
-        p64 = bench_plain(np.float64, args.plain_sweeps)
+        p64 = bench_plain(np.float64, args.plain_sweeps, b["sigmas"])
         v64 = p64["site_updates"] / (p64["wall_s"] * 1e9)
-        p32 = bench_plain(np.float32, args.plain_sweeps)
+        p32 = bench_plain(np.float32, args.plain_sweeps, b["sigmas"])
         v32 = p32["site_updates"] / (p32["wall_s"] * 1e9)
         log(f"plain FP64 {v64:.3f}, plain FP32 {v32:.3f} site-updates/ns")
         out["plain_fp64"] = {"value": round(v64, 4), "unit": "site-updates/ns",
@@ -246,7 +281,7 @@ def main():
         out["speedup_vs_plain_fp64"] = round(value / v64, 2)
     if not args.no_cpu:
         log("CPU baseline")
-        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, b["sigmas"])
     print(json.dumps(out), flush=True)
 
 

@@ -88,6 +88,7 @@ void FN(orc_xy_gauss_color)(REAL *th, uint64_t *rng, const REAL *betas, int R, i
     const REAL two = (REAL)2, zero = (REAL)0;
     const REAL pi = (REAL)M_PI, two_pi = (REAL)(2.0 * M_PI);
     const int h = L / 2;
+#pragma omp parallel for schedule(static)
     for (int r = 0; r < R; r++) {
         const int t = FN(nearest_temperature)(table_betas, T, betas[r]);
         const REAL sigma = sigmas[t];
@@ -116,7 +117,10 @@ void FN(orc_xy_gauss_color)(REAL *th, uint64_t *rng, const REAL *betas, int R, i
                 if (dE <= zero || v < FN(rexp)(-betas[r] * dE)) { th[k] = p; acc++; }
             }
         }
-        if (accepted) accepted[t] += acc;
+        if (accepted) {
+#pragma omp atomic
+            accepted[t] += acc;
+        }
     }
 }
 
