@@ -157,12 +157,10 @@ class BatchedRun:
             self._stream = p
 
     def _alloc_workspace(self):
-        hw, bw, ed = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
-        _lib.call("mc_batch_workspace_size", self.code, self.L, self.R, self.H, ctypes.byref(hw),
-                  ctypes.byref(bw), ctypes.byref(ed))
+        hw, ew = ctypes.c_size_t(), ctypes.c_size_t()
+        _lib.call("mc_batch_workspace_size", self.code, self.L, self.R, self.H, ctypes.byref(hw), ctypes.byref(ew))
         self._halo = device_array(max(hw.value, 1), np.uint64)
-        self._barrier = device_zeros(bw.value, np.uint32)
-        self._epart = device_zeros(max(ed.value, 1), np.float64)
+        self._energy_words = device_array(max(ew.value, 1), np.uint64)
         self._error = device_zeros(1, np.int32)
         limit = max_workgroups(self.code, self.L, self.R, self.H, self.exchange_interval)
         if self.R * (self.L // self.H) > limit:
@@ -208,8 +206,8 @@ class BatchedRun:
             as_pointer(self._parity), as_pointer(self.energies), as_pointer(self.walker_to_temperature),
             as_pointer(self.temperature_to_walker), as_pointer(self._pt_rng), as_pointer(self._swap_counts),
             as_pointer(self._trips), int(sample_interval or 0), int(sample_first), n_samples,
-            as_pointer(partials), as_pointer(sw2t), as_pointer(self._halo), as_pointer(self._barrier),
-            as_pointer(self._epart), as_pointer(self._error))
+            as_pointer(partials), as_pointer(sw2t), as_pointer(self._halo), as_pointer(self._energy_words),
+            as_pointer(self._error))
         self.sweeps += nsweeps
         self._counted_sweeps += nsweeps
         if self.exchange_interval:

@@ -27,10 +27,11 @@
 // trip per hand-off, no flags.
 //
 // Strips of one replica are placed on one XCD.  Every `exchange_interval`
-// sweeps the strips publish energy partials and meet at a grid barrier; every
-// workgroup then evaluates the same exchange matching on identical inputs
-// (identical decisions, no second barrier) and workgroup 0 writes the PT
-// state back to global memory.
+// sweeps each strip publishes its energy partial as two tagged words (float
+// hi + lo, about 48 bits); every workgroup polls all R*S partials into LDS,
+// sums each replica's strips in strip order and evaluates the same exchange
+// matching on identical inputs (identical decisions, no grid barrier); only
+// workgroup 0 writes the PT state back to global memory.
 //
 // Every wait is bounded: past a watchdog count a workgroup records an error
 // and leaves, so a broken invariant ends the launch instead of hanging it.
@@ -41,8 +42,9 @@
 //  * MODEL_XY_FAST: the XY update with dE from cached (cos, sin) of every site
 //    and the bit-reproducible float32 sincos/exp of mc_fastmath.h.
 //  * MODEL_XY_GAUSS: the same with a Gaussian proposal theta + sigma_t z,
-//    z = sqrt(-2 log u1) cos(2 pi u2), sigma_t per ladder temperature (it
-//    follows the walker's temperature through exchanges).
+//    sigma_t per ladder temperature (it follows the walker's temperature
+//    through exchanges); z and the acceptance uniform come from one 64-bit
+//    draw per site (gauss_draw in mc_fastmath.h).
 // Accepted moves are counted per ladder temperature when `accepted` is given.
 #include <algorithm>
 
@@ -85,8 +87,8 @@ struct BatchArgs {
     // synchronisation
     uint64_t *halo;             // [R][S][2 (row 2, row H)][2 (sweep parity)][L] words
                                 //   (sweep << 32 | payload)
-    unsigned *barrier;          // [2] count, generation
-    double *epart;              // [2][R][S]
+    uint64_t *energy;           // [2 (exchange parity)][R][S][2 (hi, lo)] words
+                                //   (exchange tag << 32 | float bits)
     int *error;
 };
 
@@ -95,32 +97,16 @@ __device__ __forceinline__ int lds_index(int row, int j, int L) {
     return row * L + (j & 1) * (L >> 1) + (j >> 1);
 }
 
-// Cross-workgroup protocol.  Every word that crosses workgroups (halo words,
-// energy partials, barrier counters) is written and read with agent-scope
-// relaxed atomics, which are device-coherent on their own (gfx950 sets the
-// device-scope cache bits on them), so no L2 write-back or invalidate is
-// needed.  Halo words are single-copy atomic 64-bit values whose high half
-// names the sweep, so a reader that sees the expected sweep also sees the
-// payload written with it; energy partials are published before the grid
-// barrier's arrival, whose counter update is ordered after them by a full
-// s_waitcnt and a workgroup barrier.
+// Cross-workgroup protocol.  Every word that crosses workgroups (halo and
+// energy words) is written and read with agent-scope relaxed atomics, which
+// are device-coherent on their own (gfx950 sets the device-scope cache bits
+// on them), so no L2 write-back or invalidate is needed.  Each word is a
+// single-copy atomic 64-bit value whose high half names the sweep (halo) or
+// the exchange (energy) it belongs to, so a reader that sees the expected
+// tag also sees the payload written with it.
 struct Sync {
     const BatchArgs &a;
-    // thread 0: poll until *f != target, bounded by the watchdog
-    __device__ bool poll_changed(unsigned *f, unsigned target) const {
-        unsigned n = 0;
-        for (;;) {
-            const unsigned v = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (v != target) return true;
-            __builtin_amdgcn_s_sleep(1);
-            if ((++n & 1023u) == 0 &&
-                (n > kWatchdog || __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-                atomicOr(a.error, BATCH_ERR_WATCHDOG);
-                return false;
-            }
-        }
-    }
-    // one thread: poll a halo word until it carries sweep `seq`
+    // one thread: poll a word until it carries tag `seq`
     __device__ bool poll_word(const uint64_t *p, unsigned seq, uint32_t &payload) const {
         unsigned n = 0;
         for (;;) {
@@ -135,28 +121,6 @@ struct Sync {
                 (n > kWatchdog || __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
                 return false;
         }
-    }
-    // sense-reversing grid barrier over all workgroups
-    __device__ bool grid_barrier(unsigned nwg) const {
-        __shared__ int ok;
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned *count = a.barrier, *gen = a.barrier + 1;
-            const unsigned g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int good = 1;
-            if (arrived == nwg - 1) {
-                __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __builtin_amdgcn_s_waitcnt(0);
-                __hip_atomic_store(gen, g + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            } else {
-                good = poll_changed(gen, g);
-            }
-            ok = good;
-        }
-        __syncthreads();
-        return ok;
     }
 };
 
@@ -230,6 +194,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
     int32_t *pt_w2t = pt_t2w + R;                                         // R
     double *pt_e = reinterpret_cast<double *>(pt_base + ((size_t)(3 * R * 4 + 7) & ~(size_t)7));  // R
     uint64_t *pt_g = reinterpret_cast<uint64_t *>(pt_e + R);             // 2 (R-1)
+    float *pt_part = reinterpret_cast<float *>(pt_g + 2 * (R - 1));      // R S (hi, lo)
     __shared__ double red[kChannels][kBatchThreads / 64];
     __shared__ float thr[5];
     __shared__ float my_beta, my_sigma;
@@ -310,21 +275,11 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
         } else {
             const float th = lth[own];
             float pr;
+            float v;
             if (MODEL == MODEL_XY_GAUSS) {
-                float u1 = g.uniform_f32();
-                const float u2 = g.uniform_f32();
-                if (u1 <= 0.0f) u1 = 0x1p-53f;
-                float s2, c2;
-                fast_sincosf(two_pi * u2, s2, c2);
-                const float z = sqrtf(-2.0f * fast_logf(u1)) * c2;
-                pr = th + sigma * z;
-                if (pr >= pi || pr < -pi) {  // whole turns, then the last one's rounding
-                    pr -= two_pi * floorf((pr + pi) / two_pi);
-                    if (pr >= pi)
-                        pr -= two_pi;
-                    else if (pr < -pi)
-                        pr += two_pi;
-                }
+                const GaussDraw d = gauss_draw(g.next());
+                pr = wrap_turns(th + sigma * d.z);
+                v = d.v;
             } else {
                 const float u = g.uniform_f32();
                 pr = th + delta * (two * u - one);
@@ -340,7 +295,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
             float t = (cp - lc[self]) * Cs;
             t = fmaf(sp - ls[self], Ss, t);
             const float dE = -J * t;
-            const float v = g.uniform_f32();
+            if (MODEL != MODEL_XY_GAUSS) v = g.uniform_f32();
             if (dE <= 0.0f || v < fast_expf(-beta * dE)) {
                 lth[own] = pr;
                 lc[self] = cp;
@@ -512,7 +467,8 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
         __syncthreads();  // the observables read every row
 
         // ---- strip observables: right and down bonds (the row below row H is
-        // the ghost row H+1, current for this sweep)
+        // the ghost row H+1, current for this sweep); the energy alone when
+        // only an exchange needs it
         double v[kChannels];
         {
             double e = 0, m1 = 0, m2 = 0, cx = 0, qx = 0;
@@ -524,20 +480,21 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
                 if (MODEL == MODEL_ISING) {
                     const int sv = lsp[self];
                     e += -(double)J * sv * ((int)lsp[dn] + (int)lsp[rt]);
-                    m1 += sv;
+                    if (do_sample) m1 += sv;
                 } else {
                     const float cs = lc[self], ss = ls[self];
                     const float bxc = fmaf(ss, ls[dn], cs * lc[dn]);
                     const float byc = fmaf(ss, ls[rt], cs * lc[rt]);
-                    const float bxs = fmaf(ss, lc[dn], -(cs * ls[dn]));
                     const float y = -J * (bxc + byc) - ec;
                     const float t = ek + y;
                     ec = (t - ek) - y;
                     ek = t;
-                    m1 += cs;
-                    m2 += ss;
-                    cx += bxc;
-                    qx += bxs;
+                    if (do_sample) {
+                        m1 += cs;
+                        m2 += ss;
+                        cx += bxc;
+                        qx += fmaf(ss, lc[dn], -(cs * ls[dn]));
+                    }
                 }
             }
             if (MODEL != MODEL_ISING) e = (double)ek - (double)ec;
@@ -547,21 +504,49 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
             v[3] = cx;
             v[4] = qx;
         }
-        block_sum<kChannels>(v, red);
+        if (do_sample) {
+            block_sum<kChannels>(v, red);
+        } else {
+            double v1[1] = {v[0]};
+            block_sum<1>(v1, red);
+            v[0] = v1[0];
+        }
 
         if (do_exchange) {
             flush_accepted();  // moves so far belong to the current temperature
-            const int buf = (int)((sweep_global / a.exchange_interval) & 1);
-            double *ep = a.epart + ((size_t)buf * R + r) * S + s;
-            if (tid == 0) __hip_atomic_store(ep, v[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (!sync.grid_barrier(nwg)) goto done;
-            // every workgroup: the same total energies, in the same order
+            const long long xi = sweep_global / a.exchange_interval;
+            const unsigned tag = (unsigned)xi + 1u;
+            uint64_t *ew = a.energy + (size_t)(xi & 1) * R * S * 2;
+            if (tid == 0) {
+                const float hi = (float)v[0];
+                const float lo = (float)(v[0] - (double)hi);
+                uint64_t *dst = ew + ((size_t)r * S + s) * 2;
+                __hip_atomic_store(dst, ((uint64_t)tag << 32) | __float_as_uint(hi), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(dst + 1, ((uint64_t)tag << 32) | __float_as_uint(lo), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+            // every workgroup gathers every strip's partial (the same values
+            // everywhere) and sums each replica's strips in strip order
+            int bad = 0;
+            for (int k = tid; k < R * S * 2; k += kBatchThreads) {
+                uint32_t payload;
+                if (!sync.poll_word(ew + k, tag, payload)) {
+                    bad = 1;
+                    break;
+                }
+                pt_part[k] = __uint_as_float(payload);
+            }
+            if (__syncthreads_or(bad)) {
+                if (tid == 0) atomicOr(a.error, BATCH_ERR_WATCHDOG);
+                goto done;
+            }
             for (int k = tid; k < R; k += kBatchThreads) {
                 double tot = 0.0;
-                const double *src = a.epart + ((size_t)buf * R + k) * S;
-                for (int q = 0; q < S; q++)
-                    tot += __hip_atomic_load(const_cast<double *>(src + q), __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
+                for (int q = 0; q < S; q++) {
+                    const float *pp = pt_part + ((size_t)k * S + q) * 2;
+                    tot += (double)pp[0] + (double)pp[1];
+                }
                 pt_e[k] = tot;
             }
             __syncthreads();
@@ -667,23 +652,22 @@ done:
     if (blockIdx.x == 0 && tid == 0 && use_pt) *a.parity = parity;
 }
 
-// Zeroes the launch-local synchronisation words: halo words (their sweep tags
-// restart at 1 every launch), the grid barrier and the error word.  A kernel
-// rather than memset nodes, so a launch can be captured into a graph.
-__global__ void batch_reset_kernel(uint64_t *halo, size_t nhalo, unsigned *barrier, int *error) {
+// Zeroes the launch-local synchronisation words: halo and energy words
+// (their tags restart every launch) and the error word.  A kernel rather than
+// memset nodes, so a launch can be captured into a graph.
+__global__ void batch_reset_kernel(uint64_t *halo, size_t nhalo, uint64_t *energy, size_t nenergy, int *error) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < nhalo; k += stride) halo[k] = 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        barrier[0] = 0;
-        barrier[1] = 0;
-        *error = 0;
-    }
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < nenergy; k += stride) energy[k] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *error = 0;
 }
 
 static size_t halo_words(int L, int R, int H) { return (size_t)R * (L / H) * 2 * 2 * L; }
+static size_t energy_words(int L, int R, int H) { return (size_t)2 * R * (L / H) * 2; }
 
 static size_t batch_lds(int model, int L, int R, int H, int exchange) {
-    const size_t pt_bytes = exchange ? (size_t)3 * R * 4 + 8 + (size_t)R * 8 + (size_t)2 * R * 8 : 0;
+    const size_t pt_bytes =
+        exchange ? (size_t)3 * R * 4 + 8 + (size_t)R * 8 + (size_t)2 * R * 8 + (size_t)R * (L / H) * 2 * 4 : 0;
     return ((strip_bytes(model, L, H) + 15) & ~(size_t)15) + pt_bytes + 64;
 }
 
@@ -710,14 +694,11 @@ using namespace mc;
 
 extern "C" {
 
-int mc_batch_workspace_size(int model, int L, int R, int H, size_t *halo_words_out, size_t *barrier_words,
-                            size_t *epart_doubles) {
+int mc_batch_workspace_size(int model, int L, int R, int H, size_t *halo_words_out, size_t *energy_words_out) {
     int rc = check_shape(model, L, R, H);
     if (rc) return rc;
-    const int S = L / H;
     *halo_words_out = halo_words(L, R, H);
-    *barrier_words = 2;
-    *epart_doubles = (size_t)2 * R * S;
+    *energy_words_out = energy_words(L, R, H);
     return MC_OK;
 }
 
@@ -743,8 +724,8 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, void *state, uint
                  float J, float delta, const float *sigmas, unsigned long long *accepted, int nsweeps, long long sweep0, int exchange_interval, int *parity,
                  double *energies, int32_t *w2t, int32_t *t2w, uint64_t *pt_rng, long long *counts,
                  long long *trips, int sample_interval, long long sample_first, int max_samples,
-                 double *sample_partials, int32_t *sample_w2t, uint64_t *halo, unsigned *barrier,
-                 double *epart, int *error) {
+                 double *sample_partials, int32_t *sample_w2t, uint64_t *halo, uint64_t *energy,
+                 int *error) {
     int rc = check_shape(model, L, R, H);
     if (rc) return rc;
     MC_REQUIRE(nsweeps >= 0, "nsweeps must be nonnegative");
@@ -766,14 +747,14 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, void *state, uint
     a.w2t = w2t; a.t2w = t2w; a.pt_rng = pt_rng; a.counts = counts; a.trips = trips;
     a.sample_interval = sample_interval; a.sample_first = sample_first; a.max_samples = max_samples;
     a.sample_partials = sample_partials; a.sample_w2t = sample_w2t;
-    a.halo = halo; a.barrier = barrier; a.epart = epart; a.error = error;
+    a.halo = halo; a.energy = energy; a.error = error;
 
     if (lds > 64 * 1024)
         MC_TRY_HIP(hipFuncSetAttribute(kernel_ptr(model), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const size_t nh = halo_words(L, R, H);
     const unsigned reset_blocks = (unsigned)std::min<size_t>((nh + 255) / 256, 1024);
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(reset_blocks), dim3(256), 0, as_stream(stream), halo, nh, barrier,
-                       error);
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(reset_blocks), dim3(256), 0, as_stream(stream), halo, nh, energy,
+                       energy_words(L, R, H), error);
     MC_TRY_HIP(hipGetLastError());
     void *kargs[] = {&a};
     MC_TRY_HIP(hipLaunchCooperativeKernel(kernel_ptr(model), dim3((unsigned)(R * S)), dim3(kBatchThreads),

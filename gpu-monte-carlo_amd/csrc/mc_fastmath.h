@@ -75,4 +75,42 @@ __device__ __forceinline__ float fast_logf(float x) {
     return fmaf(ef, 0x1.62e430p-1f, fmaf(ef, -0x1.05c610p-29f, lm));
 }
 
+// Gaussian angle step of the batched fast path from ONE 64-bit draw x:
+//   bits 63..43 -> k1, u1 = (k1 + 1) 2^-21 in (0, 1]
+//   bits 42..22 -> sign bit b and 20-bit a: angle pi a 2^-20 in [0, pi)
+//   bits 21..1  -> k3, acceptance uniform v = k3 2^-21 in [0, 1)
+// z = (-1)^b sqrt(-2 log u1) cos(pi a 2^-20): Box-Muller with the cosine's sign
+// taken from an independent bit, so the proposal is exactly symmetric.
+struct GaussDraw {
+    float z, v;
+};
+__device__ __forceinline__ GaussDraw gauss_draw(uint64_t x) {
+    const unsigned hi = (unsigned)(x >> 32), lo = (unsigned)x;
+    const unsigned k1 = hi >> 11;
+    const unsigned k2 = ((hi & 0x7ffu) << 10) | (lo >> 22);
+    const unsigned k3 = (lo >> 1) & 0x1fffffu;
+    const float u1 = (float)(k1 + 1u) * 0x1p-21f;
+    float sn, cs;
+    fast_sincosf((float)(k2 & 0xfffffu) * 0x1.921fb6p-19f, sn, cs);  // pi 2^-20
+    const float r = sqrtf(-2.0f * fast_logf(u1));
+    const float z = r * cs;
+    GaussDraw d;
+    d.z = (k2 >> 20) ? -z : z;
+    d.v = (float)k3 * 0x1p-21f;
+    return d;
+}
+
+// Angle into [-pi, pi) after an unbounded move: whole turns, then the last one's rounding.
+__device__ __forceinline__ float wrap_turns(float p) {
+    const float pi = 3.14159274101257324f, two_pi = 6.28318548202514648f;
+    if (p >= pi || p < -pi) {
+        p -= two_pi * floorf((p + pi) * 0x1.45f306p-3f);  // 1 / (2 pi)
+        if (p >= pi)
+            p -= two_pi;
+        else if (p < -pi)
+            p += two_pi;
+    }
+    return p;
+}
+
 }  // namespace mc

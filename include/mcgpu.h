@@ -207,7 +207,7 @@ int mc_kawamura_twist_grad(void *stream, int dtype, const void *state, int R, in
  * workgroup keeps its strip on chip.  Workspace sizes come from
  * mc_batch_workspace_size. */
 int mc_batch_workspace_size(int model, int L, int R, int H, size_t *halo_words,
-                            size_t *barrier_words, size_t *epart_doubles);
+                            size_t *energy_words);
 /* Largest grid (R * L / H workgroups) that can be co-resident for this
  * model and strip height. */
 int mc_batch_max_workgroups(int model, int L, int R, int H, int exchange, int *max_workgroups);
@@ -217,8 +217,8 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, void *state, uint
                  int exchange_interval, int *parity, double *energies, int32_t *walker_to_temperature,
                  int32_t *temperature_to_walker, uint64_t *pt_rng, long long *counts,
                  long long *trips, int sample_interval, long long sample_first, int max_samples,
-                 double *sample_partials, int32_t *sample_w2t, uint64_t *halo, unsigned *barrier,
-                 double *epart, int *error);
+                 double *sample_partials, int32_t *sample_w2t, uint64_t *halo, uint64_t *energy,
+                 int *error);
 
 #ifdef __cplusplus
 }

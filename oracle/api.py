@@ -25,7 +25,7 @@ __all__ = [
     "kawamura_energy",
     "he3_potential", "he3_gradient_difference", "he3_local_energy", "he3_color",
     "he3_sweeps", "he3_gaussian_color", "he3_measure", "he3_energy",
-    "pt_exchange", "fast_sincosf", "fast_expf", "fast_logf", "xyfast_cache", "xyfast_color", "xyfast_gauss_color",
+    "pt_exchange", "fast_sincosf", "fast_expf", "fast_logf", "xyfast_cache", "xyfast_color", "xyfast_gauss_color", "fast_gauss_draw",
     "xyfast_sweeps", "xyfast_observables",
 ]
 
@@ -112,6 +112,7 @@ def _declare(lib):
         "orc_xyfast_color": (None, [P, P, P, P, P, I32, I32, I32, F32, F32]),
         "orc_xyfast_sweeps": (None, [P, P, P, P, P, I32, I32, F32, F32, I64]),
         "orc_xyfast_gauss_color": (None, [P, P, P, P, P, P, I32, I32, I32, F32, P]),
+        "orc_fast_gauss_draw": (None, [ctypes.c_uint64, P, P]),
         "orc_xyfast_observables": (None, [P, P, I32, I32, F32, P]),
         "orc_pt_exchange_f64": (None, [P, P, P, P, P, P, P, I32, I32, I64]),
     }
@@ -449,6 +450,19 @@ def xyfast_gauss_color(theta, c, s, rng, betas, sigmas, color, J, accepted=None)
     load().orc_xyfast_gauss_color(_ptr(theta, np.float32, True), _ptr(c, np.float32, True),
                                   _ptr(s, np.float32, True), _rng(rng), _ptr(betas, np.float32),
                                   _ptr(sig, np.float32), R, L, int(color), float(J), acc)
+
+
+def fast_gauss_draw(x):
+    """(z, v) of the batched engine's Gaussian step for 64-bit draws x."""
+    x = np.ascontiguousarray(x, dtype=np.uint64).reshape(-1)
+    z = np.empty(len(x), np.float32)
+    v = np.empty(len(x), np.float32)
+    zf, vf = F32(), F32()
+    f = load().orc_fast_gauss_draw
+    for k, xv in enumerate(x):
+        f(int(xv), ctypes.byref(zf), ctypes.byref(vf))
+        z[k], v[k] = zf.value, vf.value
+    return z, v
 
 
 def xyfast_sweeps(theta, c, s, rng, betas, J, delta, nsweeps):

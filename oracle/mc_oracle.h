@@ -139,6 +139,7 @@ void orc_fast_sincosf_n(const float *x, int64_t n, float *sn, float *cs);
 void orc_fast_expf_n(const float *x, int64_t n, float *y);
 float orc_fast_logf(float x);
 void orc_fast_logf_n(const float *x, int64_t n, float *y);
+void orc_fast_gauss_draw(uint64_t x, float *z, float *v);
 void orc_xyfast_gauss_color(float *th, float *c, float *s, uint64_t *rng, const float *betas, const float *sig,
                             int R, int L, int color, float J, int64_t *acc);
 void orc_xyfast_init_cache(const float *th, int64_t n, float *c, float *s);

@@ -161,15 +161,31 @@ void orc_xyfast_color(float *th, float *c, float *s, uint64_t *rng, const float 
     }
 }
 
-/* Gaussian proposal of the fast path: u1, u2 -> z = sqrt(-2 log u1) cos(2 pi u2)
- * with this file's log and sincos (u1 = 0 -> 2^-53), p = theta + sigma z
- * wrapped to [-pi, pi) by whole turns, then the fast dE and acceptance with a
- * third draw.  sig holds each walker's sigma; acc counts accepted moves per
- * walker (may be NULL). */
+/* Gaussian step of the fast path from ONE 64-bit draw x (the batched
+ * engine's gauss_draw): bits 63..43 -> u1 = (k1 + 1) 2^-21 in (0, 1];
+ * bits 42..22 -> sign bit and a 20-bit angle pi a 2^-20 in [0, pi);
+ * bits 21..1 -> acceptance uniform v = k3 2^-21.  z = +-sqrt(-2 log u1) cos(angle)
+ * with this file's log and sincos; p = theta + sigma z wrapped by whole turns
+ * (multiplying by 1/(2 pi)); then the fast dE and acceptance.  sig holds each
+ * walker's sigma; acc counts accepted moves per walker (may be NULL). */
+void orc_fast_gauss_draw(uint64_t x, float *z, float *v) {
+    const uint32_t hi = (uint32_t)(x >> 32), lo = (uint32_t)x;
+    const uint32_t k1 = hi >> 11;
+    const uint32_t k2 = ((hi & 0x7ffu) << 10) | (lo >> 22);
+    const uint32_t k3 = (lo >> 1) & 0x1fffffu;
+    const float u1 = (float)(k1 + 1u) * 0x1p-21f;
+    float sn, cs;
+    orc_fast_sincosf((float)(k2 & 0xfffffu) * 0x1.921fb6p-19f, &sn, &cs);
+    const float r = sqrtf(-2.0f * orc_fast_logf(u1));
+    const float zz = r * cs;
+    *z = (k2 >> 20) ? -zz : zz;
+    *v = (float)k3 * 0x1p-21f;
+}
+
 static float fast_wrap(float p) {
     const float pi = (float)M_PI, two_pi = (float)(2.0 * M_PI);
     if (p >= pi || p < -pi) {
-        p -= two_pi * floorf((p + pi) / two_pi);
+        p -= two_pi * floorf((p + pi) * 0x1.45f306p-3f);
         if (p >= pi) p -= two_pi;
         else if (p < -pi) p += two_pi;
     }
@@ -178,7 +194,6 @@ static float fast_wrap(float p) {
 
 void orc_xyfast_gauss_color(float *th, float *c, float *s, uint64_t *rng, const float *betas, const float *sig,
                             int R, int L, int color, float J, int64_t *acc) {
-    const float two_pi = (float)(2.0 * M_PI);
     const int h = L / 2;
 #pragma omp parallel for schedule(static)
     for (int r = 0; r < R; r++) {
@@ -189,12 +204,8 @@ void orc_xyfast_gauss_color(float *th, float *c, float *s, uint64_t *rng, const 
                 const size_t k = FSITE(L, r, i, j);
                 uint64_t *st = rng + 2 * k;
                 const float theta = th[k];
-                float u1 = orc_uniform_f32(st);
-                const float u2 = orc_uniform_f32(st);
-                if (u1 <= 0.0f) u1 = 0x1p-53f;
-                float s2, c2;
-                orc_fast_sincosf(two_pi * u2, &s2, &c2);
-                const float z = sqrtf(-2.0f * orc_fast_logf(u1)) * c2;
+                float z, v;
+                orc_fast_gauss_draw(orc_rng_next(st), &z, &v);
                 const float p = fast_wrap(theta + sig[r] * z);
                 float sp, cp;
                 orc_fast_sincosf(p, &sp, &cp);
@@ -205,7 +216,6 @@ void orc_xyfast_gauss_color(float *th, float *c, float *s, uint64_t *rng, const 
                 float t = (cp - c[k]) * C;
                 t = fmaf(sp - s[k], S, t);
                 const float dE = -J * t;
-                const float v = orc_uniform_f32(st);
                 if (dE <= 0.0f || v < orc_fast_expf(-betas[r] * dE)) {
                     th[k] = p;
                     c[k] = cp;

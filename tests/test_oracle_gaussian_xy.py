@@ -122,3 +122,19 @@ def test_counters_equal_moves_and_huge_sigma_stays_wrapped():
     assert np.array_equal(acc, moved)
     assert np.all((theta >= -np.pi) & (theta < np.float32(np.pi)))
     assert 0 < acc[0] < L * L and 0 < acc[1] < L * L
+
+
+def test_fast_gauss_draw_moments_and_exact_symmetry():
+    """One 64-bit draw gives z ~ N(0, 1) (Box-Muller) and v ~ U[0, 1); flipping
+    the sign bit (bit 42) negates z exactly and leaves v."""
+    n = 100_000
+    x = np.random.default_rng(9).integers(0, 2**63, size=n, dtype=np.uint64) * np.uint64(2) + np.uint64(1)
+    z, v = oracle.fast_gauss_draw(x)
+    zd = z.astype(float)
+    assert abs(zd.mean()) < 5 / np.sqrt(n)
+    assert abs(zd.var() - 1) < 5 * np.sqrt(2 / n)
+    assert abs(np.mean(zd**4) - 3) < 5 * np.sqrt(96 / n)
+    assert abs(np.mean(np.abs(zd) > 1.959964) - 0.05) < 5 * np.sqrt(0.05 * 0.95 / n)
+    assert np.all((v >= 0) & (v < 1)) and abs(v.mean() - 0.5) < 5 * np.sqrt(1 / 12 / n)
+    z2, v2 = oracle.fast_gauss_draw(x ^ np.uint64(1 << 42))
+    assert np.array_equal(z2, -z) and np.array_equal(v2, v)
