@@ -75,12 +75,35 @@ __device__ __forceinline__ float fast_logf(float x) {
     return fmaf(ef, 0x1.62e430p-1f, fmaf(ef, -0x1.05c610p-29f, lm));
 }
 
+// sqrt(y) for y >= 0 as y * rsqrt(y): seed from the exponent/mantissa bits
+// (Lomont's constant 0x5f375a86), three Newton steps x <- x (1.5 - y/2 x^2) with
+// fused multiply-adds.  Relative error < 2e-7; exactly reproducible on a CPU.
+__device__ __forceinline__ float newton_sqrtf(float y) {
+    float x = __uint_as_float(0x5f375a86u - (__float_as_uint(y) >> 1));
+    const float h = 0.5f * y;
+    x = x * fmaf(-(h * x), x, 1.5f);
+    x = x * fmaf(-(h * x), x, 1.5f);
+    x = x * fmaf(-(h * x), x, 1.5f);
+    return y * x;
+}
+
+// sin(x) for |x| <= pi/2: odd Taylor series to x^13 in Horner form (error < 7e-10).
+__device__ __forceinline__ float halfpi_sinf(float x) {
+    const float x2 = x * x;
+    float p = fmaf(x2, 0x1.612462p-33f, -0x1.ae6456p-26f);  // 1/13!, -1/11!
+    p = fmaf(x2, p, 0x1.71de3ap-19f);                        // 1/9!
+    p = fmaf(x2, p, -0x1.a01a02p-13f);                       // -1/7!
+    p = fmaf(x2, p, 0x1.111112p-7f);                         // 1/5!
+    p = fmaf(x2, p, -0x1.555556p-3f);                        // -1/3!
+    return fmaf(x * x2, p, x);
+}
+
 // Gaussian angle step of the batched fast path from ONE 64-bit draw x:
 //   bits 63..43 -> k1, u1 = (k1 + 1) 2^-21 in (0, 1]
-//   bits 42..22 -> sign bit b and 20-bit a: angle pi a 2^-20 in [0, pi)
+//   bits 42..22 -> sign bit b and 20-bit a: phi = pi (a 2^-20 - 1/2) in [-pi/2, pi/2)
 //   bits 21..1  -> k3, acceptance uniform v = k3 2^-21 in [0, 1)
-// z = (-1)^b sqrt(-2 log u1) cos(pi a 2^-20): Box-Muller with the cosine's sign
-// taken from an independent bit, so the proposal is exactly symmetric.
+// z = (-1)^b sqrt(-2 log u1) sin(phi): Box-Muller with the sine's sign taken
+// from an independent bit, so the proposal is exactly symmetric.
 struct GaussDraw {
     float z, v;
 };
@@ -90,27 +113,21 @@ __device__ __forceinline__ GaussDraw gauss_draw(uint64_t x) {
     const unsigned k2 = ((hi & 0x7ffu) << 10) | (lo >> 22);
     const unsigned k3 = (lo >> 1) & 0x1fffffu;
     const float u1 = (float)(k1 + 1u) * 0x1p-21f;
-    float sn, cs;
-    fast_sincosf((float)(k2 & 0xfffffu) * 0x1.921fb6p-19f, sn, cs);  // pi 2^-20
-    const float r = sqrtf(-2.0f * fast_logf(u1));
-    const float z = r * cs;
+    const float phi = fmaf((float)(k2 & 0xfffffu), 0x1.921fb6p-19f, -0x1.921fb6p+0f);  // pi 2^-20, -pi/2
+    const float z = newton_sqrtf(-2.0f * fast_logf(u1)) * halfpi_sinf(phi);
     GaussDraw d;
     d.z = (k2 >> 20) ? -z : z;
     d.v = (float)k3 * 0x1p-21f;
     return d;
 }
 
-// Angle into [-pi, pi) after an unbounded move: whole turns, then the last one's rounding.
+// Angle into [-pi, pi) after an unbounded move: whole turns, then the last
+// one's rounding (branch-free).
 __device__ __forceinline__ float wrap_turns(float p) {
     const float pi = 3.14159274101257324f, two_pi = 6.28318548202514648f;
-    if (p >= pi || p < -pi) {
-        p -= two_pi * floorf((p + pi) * 0x1.45f306p-3f);  // 1 / (2 pi)
-        if (p >= pi)
-            p -= two_pi;
-        else if (p < -pi)
-            p += two_pi;
-    }
-    return p;
+    p -= two_pi * floorf((p + pi) * 0x1.45f306p-3f);  // 1 / (2 pi)
+    p = p >= pi ? p - two_pi : p;
+    return p < -pi ? p + two_pi : p;
 }
 
 }  // namespace mc

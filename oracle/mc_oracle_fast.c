@@ -161,35 +161,56 @@ void orc_xyfast_color(float *th, float *c, float *s, uint64_t *rng, const float 
     }
 }
 
+/* sqrt(y) = y rsqrt(y): seed 0x5f375a86 - (bits >> 1) (Lomont), three Newton
+ * steps x <- x (1.5 - y/2 x^2) with fused multiply-adds */
+static float newton_sqrtf(float y) {
+    union { float f; uint32_t u; } v;
+    v.f = y;
+    v.u = 0x5f375a86u - (v.u >> 1);
+    float x = v.f;
+    const float h = 0.5f * y;
+    x = x * fmaf(-(h * x), x, 1.5f);
+    x = x * fmaf(-(h * x), x, 1.5f);
+    x = x * fmaf(-(h * x), x, 1.5f);
+    return y * x;
+}
+
+/* sin(x), |x| <= pi/2: odd Taylor series to x^13, Horner form */
+static float halfpi_sinf(float x) {
+    const float x2 = x * x;
+    float p = fmaf(x2, 0x1.612462p-33f, -0x1.ae6456p-26f);
+    p = fmaf(x2, p, 0x1.71de3ap-19f);
+    p = fmaf(x2, p, -0x1.a01a02p-13f);
+    p = fmaf(x2, p, 0x1.111112p-7f);
+    p = fmaf(x2, p, -0x1.555556p-3f);
+    return fmaf(x * x2, p, x);
+}
+
 /* Gaussian step of the fast path from ONE 64-bit draw x (the batched
  * engine's gauss_draw): bits 63..43 -> u1 = (k1 + 1) 2^-21 in (0, 1];
- * bits 42..22 -> sign bit and a 20-bit angle pi a 2^-20 in [0, pi);
- * bits 21..1 -> acceptance uniform v = k3 2^-21.  z = +-sqrt(-2 log u1) cos(angle)
- * with this file's log and sincos; p = theta + sigma z wrapped by whole turns
- * (multiplying by 1/(2 pi)); then the fast dE and acceptance.  sig holds each
- * walker's sigma; acc counts accepted moves per walker (may be NULL). */
+ * bits 42..22 -> sign bit and a 20-bit phi = pi (a 2^-20 - 1/2) in [-pi/2, pi/2);
+ * bits 21..1 -> acceptance uniform v = k3 2^-21.  z = +-sqrt(-2 log u1) sin(phi)
+ * with this file's log, the Newton square root and the half-period sine;
+ * p = theta + sigma z wrapped by whole turns (multiplying by 1/(2 pi), branch
+ * free); then the fast dE and acceptance.  sig holds each walker's sigma; acc
+ * counts accepted moves per walker (may be NULL). */
 void orc_fast_gauss_draw(uint64_t x, float *z, float *v) {
     const uint32_t hi = (uint32_t)(x >> 32), lo = (uint32_t)x;
     const uint32_t k1 = hi >> 11;
     const uint32_t k2 = ((hi & 0x7ffu) << 10) | (lo >> 22);
     const uint32_t k3 = (lo >> 1) & 0x1fffffu;
     const float u1 = (float)(k1 + 1u) * 0x1p-21f;
-    float sn, cs;
-    orc_fast_sincosf((float)(k2 & 0xfffffu) * 0x1.921fb6p-19f, &sn, &cs);
-    const float r = sqrtf(-2.0f * orc_fast_logf(u1));
-    const float zz = r * cs;
+    const float phi = fmaf((float)(k2 & 0xfffffu), 0x1.921fb6p-19f, -0x1.921fb6p+0f);
+    const float zz = newton_sqrtf(-2.0f * orc_fast_logf(u1)) * halfpi_sinf(phi);
     *z = (k2 >> 20) ? -zz : zz;
     *v = (float)k3 * 0x1p-21f;
 }
 
 static float fast_wrap(float p) {
     const float pi = (float)M_PI, two_pi = (float)(2.0 * M_PI);
-    if (p >= pi || p < -pi) {
-        p -= two_pi * floorf((p + pi) * 0x1.45f306p-3f);
-        if (p >= pi) p -= two_pi;
-        else if (p < -pi) p += two_pi;
-    }
-    return p;
+    p -= two_pi * floorf((p + pi) * 0x1.45f306p-3f);
+    p = p >= pi ? p - two_pi : p;
+    return p < -pi ? p + two_pi : p;
 }
 
 void orc_xyfast_gauss_color(float *th, float *c, float *s, uint64_t *rng, const float *betas, const float *sig,

@@ -138,3 +138,17 @@ def test_fast_gauss_draw_moments_and_exact_symmetry():
     assert np.all((v >= 0) & (v < 1)) and abs(v.mean() - 0.5) < 5 * np.sqrt(1 / 12 / n)
     z2, v2 = oracle.fast_gauss_draw(x ^ np.uint64(1 << 42))
     assert np.array_equal(z2, -z) and np.array_equal(v2, v)
+
+
+def test_fast_gauss_draw_matches_box_muller_from_its_bit_fields():
+    x = np.random.default_rng(21).integers(0, 2**63, size=20_000, dtype=np.uint64) * np.uint64(2)
+    z, v = oracle.fast_gauss_draw(x)
+    xi = x.astype(object)
+    k1 = np.array([(int(a) >> 43) & 0x1FFFFF for a in xi], float)
+    k2 = np.array([(int(a) >> 22) & 0x1FFFFF for a in xi])
+    k3 = np.array([(int(a) >> 1) & 0x1FFFFF for a in xi], float)
+    u1 = (k1 + 1) * 2.0**-21
+    phi = np.pi * ((k2 & 0xFFFFF) * 2.0**-20 - 0.5)
+    ref = np.sqrt(-2 * np.log(u1)) * np.sin(phi) * np.where(k2 >> 20, -1.0, 1.0)
+    assert np.max(np.abs(z - ref)) < 2e-6 * max(1.0, np.abs(ref).max())
+    np.testing.assert_array_equal(v, (k3 * 2.0**-21).astype(np.float32))
