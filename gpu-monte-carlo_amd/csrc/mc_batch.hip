@@ -296,7 +296,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
             t = fmaf(sp - ls[self], Ss, t);
             const float dE = -J * t;
             if (MODEL != MODEL_XY_GAUSS) v = g.uniform_f32();
-            if (dE <= 0.0f || v < fast_expf(-beta * dE)) {
+            if (v < accept_weight(-beta * dE)) {  // dE <= 0 or v < exp(-beta dE)
                 lth[own] = pr;
                 lc[self] = cp;
                 ls[self] = sp;

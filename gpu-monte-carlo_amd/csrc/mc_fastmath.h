@@ -53,6 +53,25 @@ __device__ __forceinline__ float fast_expf(float x) {
     return p * __int_as_float((e + 127) << 23);
 }
 
+// Metropolis acceptance weight exp(min(x, 0)) for the fast path, branch free:
+// x is clamped to [-87, 0], where every result is a normal float (exp(0) = 1
+// exactly, so "dE <= 0 or v < exp(-beta dE)" is "v < accept_weight(-beta dE)"
+// for any v in [0, 1)).  Same reduction and polynomial as fast_expf.
+__device__ __forceinline__ float accept_weight(float x) {
+    x = fminf(fmaxf(x, -87.0f), 0.0f);
+    const float k = rintf(x * 0x1.715476p+0f);
+    float r = fmaf(-k, 0x1.62e430p-1f, x);
+    r = fmaf(-k, -0x1.05c610p-29f, r);
+    float p = fmaf(r, 0x1.a01a02p-13f, 0x1.6c16c2p-10f);
+    p = fmaf(p, r, 0x1.111112p-7f);
+    p = fmaf(p, r, 0x1.555556p-5f);
+    p = fmaf(p, r, 0x1.555556p-3f);
+    p = fmaf(p, r, 0x1p-1f);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
+    return p * __int_as_float(((int)k + 127) << 23);
+}
+
 // log for positive normal x (the engine calls it on uniforms in (0, 1])
 __device__ __forceinline__ float fast_logf(float x) {
     const unsigned b = __float_as_uint(x);

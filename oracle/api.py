@@ -25,7 +25,7 @@ __all__ = [
     "kawamura_energy",
     "he3_potential", "he3_gradient_difference", "he3_local_energy", "he3_color",
     "he3_sweeps", "he3_gaussian_color", "he3_measure", "he3_energy",
-    "pt_exchange", "fast_sincosf", "fast_expf", "fast_logf", "xyfast_cache", "xyfast_color", "xyfast_gauss_color", "fast_gauss_draw",
+    "pt_exchange", "fast_sincosf", "fast_expf", "fast_logf", "accept_weight", "xyfast_cache", "xyfast_color", "xyfast_gauss_color", "fast_gauss_draw",
     "xyfast_sweeps", "xyfast_observables",
 ]
 
@@ -108,6 +108,7 @@ def _declare(lib):
         "orc_fast_sincosf_n": (None, [P, I64, P, P]),
         "orc_fast_expf_n": (None, [P, I64, P]),
         "orc_fast_logf_n": (None, [P, I64, P]),
+        "orc_accept_weight": (F32, [F32]),
         "orc_xyfast_init_cache": (None, [P, I64, P, P]),
         "orc_xyfast_color": (None, [P, P, P, P, P, I32, I32, I32, F32, F32]),
         "orc_xyfast_sweeps": (None, [P, P, P, P, P, I32, I32, F32, F32, I64]),
@@ -418,6 +419,12 @@ def fast_expf(x):
     y = np.empty_like(x)
     load().orc_fast_expf_n(x.ctypes.data, x.size, y.ctypes.data)
     return y
+
+
+def accept_weight(x):
+    f = load().orc_accept_weight
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    return np.array([f(float(v)) for v in x.reshape(-1)], dtype=np.float32).reshape(x.shape)
 
 
 def fast_logf(x):

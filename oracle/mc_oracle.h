@@ -138,6 +138,7 @@ float orc_fast_expf(float x);
 void orc_fast_sincosf_n(const float *x, int64_t n, float *sn, float *cs);
 void orc_fast_expf_n(const float *x, int64_t n, float *y);
 float orc_fast_logf(float x);
+float orc_accept_weight(float x);
 void orc_fast_logf_n(const float *x, int64_t n, float *y);
 void orc_fast_gauss_draw(uint64_t x, float *z, float *v);
 void orc_xyfast_gauss_color(float *th, float *c, float *s, uint64_t *rng, const float *betas, const float *sig,

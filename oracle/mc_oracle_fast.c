@@ -118,6 +118,26 @@ void orc_fast_expf_n(const float *x, int64_t n, float *y) {
     for (int64_t k = 0; k < n; k++) y[k] = orc_fast_expf(x[k]);
 }
 
+/* exp(min(x, 0)) with x clamped to [-87, 0] (normal results only): the fast
+ * path accepts iff v < accept_weight(-beta dE), which is "dE <= 0 or
+ * v < exp(-beta dE)" for v in [0, 1) */
+float orc_accept_weight(float x) {
+    x = fminf(fmaxf(x, -87.0f), 0.0f);
+    const float k = rintf(x * LOG2E);
+    float r = fmaf(-k, LN2_HI, x);
+    r = fmaf(-k, LN2_LO, r);
+    float p = fmaf(r, E7, E6);
+    p = fmaf(p, r, E5);
+    p = fmaf(p, r, E4);
+    p = fmaf(p, r, E3);
+    p = fmaf(p, r, E2);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
+    union { uint32_t u; float f; } two_k;
+    two_k.u = (uint32_t)((int)k + 127) << 23;
+    return p * two_k.f;
+}
+
 void orc_xyfast_init_cache(const float *th, int64_t n, float *c, float *s) {
     for (int64_t k = 0; k < n; k++) orc_fast_sincosf(th[k], s + k, c + k);
 }
@@ -151,7 +171,7 @@ void orc_xyfast_color(float *th, float *c, float *s, uint64_t *rng, const float 
                 t = fmaf(sp - s[k], S, t);
                 const float dE = -J * t;
                 const float v = orc_uniform_f32(st);
-                if (dE <= 0.0f || v < orc_fast_expf(-betas[r] * dE)) {
+                if (v < orc_accept_weight(-betas[r] * dE)) {
                     th[k] = p;
                     c[k] = cp;
                     s[k] = sp;
@@ -237,7 +257,7 @@ void orc_xyfast_gauss_color(float *th, float *c, float *s, uint64_t *rng, const 
                 float t = (cp - c[k]) * C;
                 t = fmaf(sp - s[k], S, t);
                 const float dE = -J * t;
-                if (dE <= 0.0f || v < orc_fast_expf(-betas[r] * dE)) {
+                if (v < orc_accept_weight(-betas[r] * dE)) {
                     th[k] = p;
                     c[k] = cp;
                     s[k] = sp;

@@ -132,3 +132,12 @@ def test_fast_update_hot_and_cold_limits(sign):
     assert theta[0, 0, 0] == p
     if sign > 0:  # ordered start: every proposal is uphill at beta = 1e30
         assert np.array_equal(theta[1], before[1])
+
+
+def test_accept_weight_is_clamped_exponential():
+    """v < accept_weight(-beta dE) is the Metropolis rule "dE <= 0 or v < exp(-beta dE)"."""
+    x = np.linspace(-87.0, 0.0, 200_001).astype(np.float32)
+    np.testing.assert_array_equal(oracle.accept_weight(x), oracle.fast_expf(x))
+    assert np.all(oracle.accept_weight(np.array([0.0, 1e-30, 3.0, 1e30, np.inf], np.float32)) == 1.0)
+    low = oracle.accept_weight(np.array([-87.0, -90.0, -1e4, -np.inf], np.float32))
+    assert np.all(low == low[0]) and 0 < low[0] < 2e-38 and low[0] >= np.finfo(np.float32).tiny
