@@ -65,23 +65,26 @@ def _model(proposal, sigma=1.0, dtype=np.float32):
     return XYModel(J=1.0, delta=1.5, dtype=dtype, start="random")
 
 
-def bench_batched(steps, warmup, strip_rows, autotune, proposal="gaussian"):
-    from core.batched import BatchedRun, autotune_strip_rows
+def bench_batched(steps, warmup, strip_rows, tune, proposal="gaussian"):
+    from core.batched import BatchedRun, autotune
     from mcgpu import synchronize
     from mcgpu.timing import EventTimer
 
     model = _model(proposal)
-    if strip_rows is None and autotune:
-        log(f"autotuning strip height ({proposal})")
-        strip_rows = autotune_strip_rows(model, L, temperatures(), seed=SEED, exchange_interval=EXCHANGE,
-                                         sweeps=400, verbose=True)
-    run = BatchedRun(model, L, temperatures(), seed=SEED, exchange_interval=EXCHANGE, strip_rows=strip_rows)
+    threads = None
+    if strip_rows is None and tune:
+        log(f"autotuning strip height and workgroup size ({proposal})")
+        strip_rows, threads = autotune(model, L, temperatures(), seed=SEED, exchange_interval=EXCHANGE,
+                                       sweeps=400, verbose=True)
+    run = BatchedRun(model, L, temperatures(), seed=SEED, exchange_interval=EXCHANGE, strip_rows=strip_rows,
+                     threads=threads)
     sigmas = None
     if proposal == "gaussian":
         log(f"adapting Gaussian widths over {ADAPT_SWEEPS} sweeps")
         run.adapt_sigmas(ADAPT_SWEEPS, interval=ADAPT_INTERVAL)
         sigmas = run._sigmas_host.copy()
-    log(f"batched FP32 ({proposal}): strip rows {run.H}, {run.R * run.strips} workgroups; warmup {warmup} steps")
+    log(f"batched FP32 ({proposal}): strip rows {run.H}, {run.R * run.strips} workgroups of {run.threads} "
+        f"threads; warmup {warmup} steps")
     run.run(warmup * EXCHANGE)
     run.reset_acceptance()
     synchronize()
@@ -105,6 +108,7 @@ def bench_batched(steps, warmup, strip_rows, autotune, proposal="gaussian"):
         "kernel_ms": kernel_ms,
         "site_updates": sweeps * L * L * R,
         "strip_rows": run.H,
+        "threads": run.threads,
         "workgroups": run.R * run.strips,
         "pt_acceptance": float(d["acceptance_rates"].mean()),
         "move_acceptance": d["acceptance_by_temperature"],
@@ -142,13 +146,13 @@ def copy_probe(mib):
     log(f"copy probe: {mib} MiB device-to-device")
 
 
-def measured_traffic(strip_rows, sweeps):
+def measured_traffic(strip_rows, threads, sweeps):
     """HBM bytes of a launch of `sweeps` sweeps from the committed counter runs, or None."""
     try:
         t = json.loads(TRAFFIC_FILE.read_text())
     except (OSError, ValueError):
         return None, None
-    if int(t.get("strip_rows", -1)) != strip_rows:
+    if int(t.get("strip_rows", -1)) != strip_rows or int(t.get("threads", 256)) != threads:
         return None, None
     return float(t["traffic_bytes_per_sweep"]) * sweeps, t.get("summary")
 
@@ -219,7 +223,7 @@ def main():
     kernel_rate = b["site_updates"] / (b["kernel_ms"] * 1e6)
     achieved = BYTES_PER_SITE_UPDATE * b["site_updates"] / (b["kernel_ms"] * 1e-3) / 1e9
     log(f"batched FP32: {value:.2f} site-updates/ns wall, {kernel_rate:.2f} in-kernel")
-    traffic, traffic_src = measured_traffic(b["strip_rows"], args.steps * EXCHANGE)
+    traffic, traffic_src = measured_traffic(b["strip_rows"], b["threads"], args.steps * EXCHANGE)
 
     out = {
         "metric": METRIC,
@@ -239,7 +243,7 @@ def main():
                          f"proposals with widths adapted toward 0.5 acceptance ({ADAPT_SWEEPS} sweeps, then "
                          f"frozen), exchange every {EXCHANGE} sweeps; batched FP32 persistent launch"),
             "L": L, "replicas": R, "sweeps_per_step": EXCHANGE,
-            "strip_rows": b["strip_rows"], "workgroups": b["workgroups"],
+            "strip_rows": b["strip_rows"], "workgroup_threads": b["threads"], "workgroups": b["workgroups"],
             "device": name, "compute_units": cus,
         },
         "roofline": {

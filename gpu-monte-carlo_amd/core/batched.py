@@ -57,38 +57,63 @@ def model_code(model):
     raise TypeError(f"no batched kernel for {type(model).__name__}")
 
 
-def max_workgroups(code, L, R, H, exchange):
+WORKGROUP_SIZES = (256, 512)
+
+
+def max_workgroups(code, L, R, H, exchange, threads=256):
     n = ctypes.c_int(0)
-    _lib.call("mc_batch_max_workgroups", code, L, R, H, int(bool(exchange)), ctypes.byref(n))
+    _lib.call("mc_batch_max_workgroups", code, L, R, H, int(threads), int(bool(exchange)), ctypes.byref(n))
     return int(n.value)
 
 
-def candidate_strip_rows(code, L, R, exchange):
-    """Even divisors H of L whose grid (R * L / H workgroups) can be co-resident."""
+def candidate_configs(code, L, R, exchange):
+    """(strip height H, workgroup size) pairs whose grid (R * L / H workgroups)
+    can be co-resident: H an even divisor of L, at most 8 colour rows of work
+    per thread and pass, and at least one colour row per wave for 512 threads."""
     out = []
-    for H in range(4, L + 1, 2):
-        if L % H or (H // 2) * (L // 2) > 8 * 256:
+    for threads in WORKGROUP_SIZES:
+        if L // 2 > threads:
             continue
-        if R * (L // H) > max_workgroups(code, L, R, H, exchange):
-            continue
-        out.append(H)
+        for H in range(4, L + 1, 2):
+            if L % H or (H // 2) * (L // 2) > 8 * threads:
+                continue
+            if threads > 256 and (H // 2) * (L // 2) < threads:
+                continue
+            if R * (L // H) > max_workgroups(code, L, R, H, exchange, threads):
+                continue
+            out.append((H, threads))
     return out
 
 
-def default_strip_rows(code, L, R, exchange, target_workgroups=512):
-    """Smallest strip height >= 4 whose grid does not exceed `target_workgroups`."""
-    cands = candidate_strip_rows(code, L, R, exchange)
+def candidate_strip_rows(code, L, R, exchange):
+    """Strip heights usable with 256-thread workgroups (or 512 when L > 512)."""
+    return sorted({H for H, t in candidate_configs(code, L, R, exchange) if t == _min_threads(L)})
+
+
+def _min_threads(L):
+    return 256 if L // 2 <= 256 else 512
+
+
+def default_config(code, L, R, exchange, target_workgroups=512):
+    """Smallest strip height >= 4 (smallest workgroup) whose grid does not exceed `target_workgroups`."""
+    cands = candidate_configs(code, L, R, exchange)
     if not cands:
         raise ValueError(f"no strip height fits L={L}, R={R} on this device")
-    good = [H for H in cands if R * (L // H) <= target_workgroups]
-    return min(good) if good else max(cands)
+    t0 = min(t for _, t in cands)
+    base = [H for H, t in cands if t == t0]
+    good = [H for H in base if R * (L // H) <= target_workgroups]
+    return (min(good) if good else max(base)), t0
+
+
+def default_strip_rows(code, L, R, exchange, target_workgroups=512):
+    return default_config(code, L, R, exchange, target_workgroups)[0]
 
 
 class BatchedRun:
     """One model, one lattice size, all its replicas: state, streams and PT on the device."""
 
     def __init__(self, model, L, temperatures, seed=42, exchange_interval=10, pt_seed=None,
-                 strip_rows=None, stream=True):
+                 strip_rows=None, stream=True, threads=None):
         if L < 4 or L % 2 != 0:
             raise ValueError("L must be even and >= 4")
         temperatures = np.asarray(temperatures, dtype=np.float64)
@@ -147,8 +172,12 @@ class BatchedRun:
             self._sigmas_host = np.broadcast_to(np.asarray(model.sigma, dtype=np.float64), (R,)).copy()
             self.sigmas = to_device(self._sigmas_host.astype(np.float32))
 
-        self.H = int(strip_rows) if strip_rows else default_strip_rows(
-            self.code, L, R, exchange_interval)
+        if strip_rows:
+            self.H = int(strip_rows)
+            self.threads = int(threads) if threads else _min_threads(L)
+        else:
+            self.H, t = default_config(self.code, L, R, exchange_interval)
+            self.threads = int(threads) if threads else t
         self._alloc_workspace()
         self._stream = None
         if stream:
@@ -162,7 +191,7 @@ class BatchedRun:
         self._halo = device_array(max(hw.value, 1), np.uint64)
         self._energy_words = device_array(max(ew.value, 1), np.uint64)
         self._error = device_zeros(1, np.int32)
-        limit = max_workgroups(self.code, self.L, self.R, self.H, self.exchange_interval)
+        limit = max_workgroups(self.code, self.L, self.R, self.H, self.exchange_interval, self.threads)
         if self.R * (self.L // self.H) > limit:
             raise ValueError(f"grid of {self.R * (self.L // self.H)} workgroups exceeds the "
                              f"{limit} that can be co-resident; choose a taller strip")
@@ -199,7 +228,7 @@ class BatchedRun:
         partials = device_array((max(n_samples, 1), self.R, S, CHANNELS), np.float64)
         sw2t = device_array((max(n_samples, 1), self.R), np.int32)
         _lib.call(
-            "mc_batch_run", self._stream, self.code, self.L, self.R, self.H, as_pointer(self.state),
+            "mc_batch_run", self._stream, self.code, self.L, self.R, self.H, self.threads, as_pointer(self.state),
             as_pointer(self.rng_states), as_pointer(self.betas), float(self.model.J),
             float(getattr(self.model, "delta", 0.0)), as_pointer(self.sigmas), as_pointer(self._accepted),
             nsweeps, sweep0, self.exchange_interval,
@@ -321,14 +350,15 @@ class BatchedRun:
             "walker_round_trip_total_sweeps": trips[4].copy(),
             "mean_round_trip_sweeps": float(trips[4].sum()) / completed if completed else None,
             "strip_rows": self.H,
+            "workgroup_threads": self.threads,
             "acceptance_by_temperature": self.acceptance_by_temperature(),
             "sigmas": None if self.sigmas is None else self._sigmas_host.copy(),
         }
 
 
-def autotune_strip_rows(model, L, temperatures, seed=42, exchange_interval=10, sweeps=300,
-                        candidates=None, verbose=False):
-    """Time short runs for each candidate strip height and return the fastest.
+def autotune(model, L, temperatures, seed=42, exchange_interval=10, sweeps=300, candidates=None,
+             verbose=False):
+    """Time short runs for each candidate (strip height, workgroup size) and return the fastest.
 
     Results are cached per (model type, L, R, exchange) for the process.
     """
@@ -337,23 +367,29 @@ def autotune_strip_rows(model, L, temperatures, seed=42, exchange_interval=10, s
     key = (code, L, R, bool(exchange_interval))
     if key in _tuned and candidates is None:
         return _tuned[key]
-    cands = candidates or candidate_strip_rows(code, L, R, exchange_interval)
+    cands = candidates or candidate_configs(code, L, R, exchange_interval)
     best, best_t = None, None
-    timings = {}
-    for H in cands:
+    for H, threads in cands:
         run = BatchedRun(model, L, temperatures, seed=seed, exchange_interval=exchange_interval,
-                         strip_rows=H)
+                         strip_rows=H, threads=threads)
         run.run(20)
         t0 = time.perf_counter()
         run.run(sweeps)
         dt = time.perf_counter() - t0
-        timings[H] = dt
         if verbose:
-            print(f"  H={H:4d} strips={L // H:4d} {dt / sweeps * 1e6:9.2f} us/sweep", flush=True)
+            print(f"  H={H:4d} threads={threads:4d} strips={L // H:4d} {dt / sweeps * 1e6:9.2f} us/sweep",
+                  flush=True)
         if best_t is None or dt < best_t:
-            best, best_t = H, dt
+            best, best_t = (H, threads), dt
     _tuned[key] = best
     return best
+
+
+def autotune_strip_rows(model, L, temperatures, seed=42, exchange_interval=10, sweeps=300, candidates=None,
+                        verbose=False):
+    """The fastest strip height among 256-thread configurations (see autotune)."""
+    cands = candidates and [(H, _min_threads(L)) for H in candidates]
+    return autotune(model, L, temperatures, seed, exchange_interval, sweeps, cands, verbose)[0]
 
 
 class BatchedSimulation:

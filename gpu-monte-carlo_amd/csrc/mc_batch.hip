@@ -2,7 +2,8 @@
 // samples) in one cooperative launch.
 //
 // Decomposition: each replica lattice (L x L) is cut into S = L / H strips of
-// H full rows; one 256-thread workgroup owns one strip for the whole launch.
+// H full rows; one workgroup (256 or 512 threads) owns one strip for the
+// whole launch.
 // Everything the strip needs lives in LDS, rows stored colour-split (even
 // columns, then odd columns) so every access of a colour step is unit-stride:
 //   local row 0      bottom row of the strip above (received)
@@ -56,7 +57,6 @@ namespace mc {
 enum { MODEL_ISING = 0, MODEL_XY_FAST = 1, MODEL_XY_GAUSS = 2 };
 enum { BATCH_ERR_WATCHDOG = 1 };
 
-constexpr int kBatchThreads = 256;
 constexpr unsigned kWatchdog = 1u << 20;  // about a second of polling
 constexpr int kChannels = 5;               // XY: e, mx, my, cx, qx ; Ising: e, m
 
@@ -132,9 +132,10 @@ __device__ __forceinline__ float ising_threshold(int v, float beta, float J) {
     return rexp(-beta * dE);
 }
 
-// Block reduction of C float64 channels; the result is valid in thread 0.
-template <int C>
-__device__ void block_sum(double (&v)[C], double (*sh)[kBatchThreads / 64]) {
+// Block reduction of C float64 channels over NT threads; the result is valid
+// in thread 0 (waves added in order).
+template <int C, int NT>
+__device__ void block_sum(double (&v)[C], double (*sh)[NT / 64]) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int c = 0; c < C; c++) {
@@ -146,7 +147,12 @@ __device__ void block_sum(double (&v)[C], double (*sh)[kBatchThreads / 64]) {
     __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int c = 0; c < C; c++) v[c] = ((sh[c][0] + sh[c][1]) + sh[c][2]) + sh[c][3];
+        for (int c = 0; c < C; c++) {
+            double t = sh[c][0];
+#pragma unroll
+            for (int w = 1; w < NT / 64; w++) t += sh[c][w];
+            v[c] = t;
+        }
     }
     __syncthreads();
 }
@@ -167,8 +173,8 @@ struct Seg {
     int c, lo, hi;
 };
 
-template <int MODEL>
-__global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
+template <int MODEL, int NT>
+__global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const int L = a.L, H = a.H, S = a.S, R = a.R;
     const int half = L >> 1, HH = H >> 1;
@@ -195,18 +201,16 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
     double *pt_e = reinterpret_cast<double *>(pt_base + ((size_t)(3 * R * 4 + 7) & ~(size_t)7));  // R
     uint64_t *pt_g = reinterpret_cast<uint64_t *>(pt_e + R);             // 2 (R-1)
     float *pt_part = reinterpret_cast<float *>(pt_g + 2 * (R - 1));      // R S (hi, lo)
-    __shared__ double red[kChannels][kBatchThreads / 64];
+    __shared__ double red[kChannels + 1][NT / 64];
     __shared__ float thr[5];
-    __shared__ float my_beta, my_sigma;
-    __shared__ int my_t;
-    __shared__ int red_i[kBatchThreads / 64];
+    __shared__ int red_i[NT / 64];
 
     const size_t rep = (size_t)r * L * L;
     const int row0 = s * H;  // first global row of the strip
     const bool use_pt = a.exchange_interval > 0;
 
     // ---- load the strip: rows 0..H+2 (global rows row0-1 .. row0+H+1)
-    for (int idx = tid; idx < (H + 3) * L; idx += kBatchThreads) {
+    for (int idx = tid; idx < (H + 3) * L; idx += NT) {
         const int lr = idx / L, j = idx - lr * L;
         const int gi = (row0 + lr - 1 + L) % L;
         const size_t gk = rep + (size_t)gi * L + j;
@@ -224,30 +228,29 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
         }
     }
     if (use_pt) {
-        for (int k = tid; k < R; k += kBatchThreads) {
+        for (int k = tid; k < R; k += NT) {
             pt_beta[k] = a.betas[k];
             pt_t2w[k] = a.t2w[k];
             pt_w2t[k] = a.w2t[k];
         }
-        for (int k = tid; k < 2 * (R - 1); k += kBatchThreads) pt_g[k] = a.pt_rng[k];
+        for (int k = tid; k < 2 * (R - 1); k += NT) pt_g[k] = a.pt_rng[k];
     }
-    if (tid == 0) {
-        my_beta = a.betas[r];
-        my_t = use_pt ? a.w2t[r] : r;
-        my_sigma = MODEL == MODEL_XY_GAUSS ? a.sigmas[my_t] : 0.0f;
-    }
-    __syncthreads();
+    // this walker's beta, ladder temperature and proposal width (the same in
+    // every thread of the workgroup)
+    float my_beta = a.betas[r];
+    int my_t = use_pt ? a.w2t[r] : r;
+    float sigma = MODEL == MODEL_XY_GAUSS ? a.sigmas[my_t] : 0.0f;
     if (MODEL == MODEL_ISING && tid < 5) thr[tid] = ising_threshold(4 * tid - 8, my_beta, a.J);
     int parity = use_pt ? *a.parity : 0;
+    bool exchanged = false;  // pt_e holds this launch's last exchange energies
     __syncthreads();
 
     const float J = a.J, delta = a.delta;
-    float sigma = 0.0f;  // my_sigma for the current sweep
     int n_acc = 0;       // accepted moves of this thread since the last flush
     const float one = 1.0f, two = 2.0f;
     const float pi = 3.14159274101257324f, two_pi = 6.28318548202514648f;
     // fixed thread -> (column, colour-row offset) map of every pass
-    const int tcol = tid % half, trow = tid / half, rows_per_pass = kBatchThreads / half;
+    const int tcol = tid % half, trow = tid / half, rows_per_pass = NT / half;
 
     // Update site (colour row ri, column 2 tcol + (c & 1)) of colour c.
     auto site = [&](const int c, const int ri, const float beta) {
@@ -354,7 +357,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
     // Ends with a workgroup barrier; false if the watchdog expired.
     auto receive = [&](int n) -> bool {
         if (S == 1) {
-            for (int k = tid; k < 2 * L; k += kBatchThreads) {
+            for (int k = tid; k < 2 * L; k += NT) {
                 const bool top = k < L;
                 const int j = top ? k : k - L;
                 const int dst = lds_index(top ? 0 : H + 2, j, L), src = lds_index(top ? H : 2, j, L);
@@ -374,7 +377,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
         }
         const int par = (n - 1) & 1;
         int bad = 0;
-        for (int k = tid; k < 2 * L; k += kBatchThreads) {
+        for (int k = tid; k < 2 * L; k += NT) {
             const bool top = k < L;
             const int j = top ? k : k - L;
             const uint64_t *src = top ? halo_ptr(s_up, 1, par) : halo_ptr(s_dn, 0, par);
@@ -425,7 +428,9 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
         if ((tid & 63) == 0) red_i[tid >> 6] = x;
         __syncthreads();
         if (tid == 0) {
-            const int total = ((red_i[0] + red_i[1]) + red_i[2]) + red_i[3];
+            int total = 0;
+#pragma unroll
+            for (int w = 0; w < NT / 64; w++) total += red_i[w];
             if (total) atomicAdd(&a.accepted[my_t], (unsigned long long)total);
         }
         __syncthreads();
@@ -448,7 +453,6 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
     for (int n = 0; n < a.nsweeps; n++) {
         const long long sweep_global = a.sweep0 + n + 1;
         const float beta = my_beta;
-        sigma = my_sigma;
         pass(Seg{0, 1, HH - 1}, none, none, none, beta);
         __syncthreads();
         pass(Seg{1, 1, HH - 1}, none, none, none, beta);
@@ -473,7 +477,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
         {
             double e = 0, m1 = 0, m2 = 0, cx = 0, qx = 0;
             float ek = 0.0f, ec = 0.0f;  // compensated sum of the XY bond energies
-            for (int idx = tid; idx < H * L; idx += kBatchThreads) {
+            for (int idx = tid; idx < H * L; idx += NT) {
                 const int lr = 1 + idx / L, j = idx % L;
                 const int self = lds_index(lr, j, L), dn = lds_index(lr + 1, j, L);
                 const int rt = lds_index(lr, j == L - 1 ? 0 : j + 1, L);
@@ -504,16 +508,26 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
             v[3] = cx;
             v[4] = qx;
         }
+        // the workgroup's accepted moves travel with the sums (channel kChannels)
+        double nacc = 0.0;
         if (do_sample) {
-            block_sum<kChannels>(v, red);
+            double w[kChannels + 1] = {v[0], v[1], v[2], v[3], v[4], (double)n_acc};
+            block_sum<kChannels + 1, NT>(w, red);
+#pragma unroll
+            for (int c = 0; c < kChannels; c++) v[c] = w[c];
+            nacc = w[kChannels];
         } else {
-            double v1[1] = {v[0]};
-            block_sum<1>(v1, red);
-            v[0] = v1[0];
+            double w[2] = {v[0], (double)n_acc};
+            block_sum<2, NT>(w, red);
+            v[0] = w[0];
+            nacc = w[1];
         }
+        // moves so far belong to the current temperature
+        if (tid == 0 && a.accepted && nacc > 0.0) atomicAdd(&a.accepted[my_t], (unsigned long long)nacc);
+        n_acc = 0;
 
         if (do_exchange) {
-            flush_accepted();  // moves so far belong to the current temperature
+            exchanged = true;
             const long long xi = sweep_global / a.exchange_interval;
             const unsigned tag = (unsigned)xi + 1u;
             uint64_t *ew = a.energy + (size_t)(xi & 1) * R * S * 2;
@@ -529,7 +543,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
             // every workgroup gathers every strip's partial (the same values
             // everywhere) and sums each replica's strips in strip order
             int bad = 0;
-            for (int k = tid; k < R * S * 2; k += kBatchThreads) {
+            for (int k = tid; k < R * S * 2; k += NT) {
                 uint32_t payload;
                 if (!sync.poll_word(ew + k, tag, payload)) {
                     bad = 1;
@@ -541,7 +555,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
                 if (tid == 0) atomicOr(a.error, BATCH_ERR_WATCHDOG);
                 goto done;
             }
-            for (int k = tid; k < R; k += kBatchThreads) {
+            for (int k = tid; k < R; k += NT) {
                 double tot = 0.0;
                 for (int q = 0; q < S; q++) {
                     const float *pp = pt_part + ((size_t)k * S + q) * 2;
@@ -550,9 +564,11 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
                 pt_e[k] = tot;
             }
             __syncthreads();
-            // one matching of parity `parity` (core/parallel_tempering.py:44-69)
+            // one matching of parity `parity` (core/parallel_tempering.py:44-69);
+            // every workgroup evaluates all pairs on identical inputs, and the
+            // counters of pair `left` are kept by workgroup left mod grid
             const int P = R - 1;
-            for (int left = parity + 2 * tid; left < R - 1; left += 2 * kBatchThreads) {
+            for (int left = parity + 2 * tid; left < R - 1; left += 2 * NT) {
                 const int right = left + 1;
                 const int wa = pt_t2w[left], wb = pt_t2w[right];
                 const double ea = pt_e[wa], eb = pt_e[wb];
@@ -577,48 +593,39 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
                     pt_beta[wa] = pt_beta[wb];
                     pt_beta[wb] = tb;
                 }
-                if (blockIdx.x == 0) {
+                if ((unsigned)left % nwg == blockIdx.x) {
                     a.counts[left] += 1;
                     if (!finite) a.counts[2 * P + left] += 1;
                     if (accept) a.counts[P + left] += 1;
                 }
             }
             __syncthreads();
-            if (blockIdx.x == 0) {
-                // round-trip bookkeeping (core/parallel_tempering.py:71-91)
-                for (int w = tid; w < R; w += kBatchThreads) {
-                    const int t = pt_w2t[w];
-                    if (t != 0 && t != R - 1) continue;
-                    const long long origin = a.trips[w];
-                    if (origin < 0) {
-                        a.trips[w] = t;
-                        a.trips[2 * R + w] = sweep_global;
-                    } else if (t != origin) {
-                        a.trips[R + w] = 1;
-                    } else if (a.trips[R + w]) {
-                        a.trips[3 * R + w] += 1;
-                        a.trips[4 * R + w] += sweep_global - a.trips[2 * R + w];
-                        a.trips[2 * R + w] = sweep_global;
-                        a.trips[R + w] = 0;
-                    }
+            // round-trip bookkeeping (core/parallel_tempering.py:71-91), walker w
+            // kept by workgroup w mod grid
+            for (int w = blockIdx.x + tid * (int)nwg; w < R; w += NT * (int)nwg) {
+                const int t = pt_w2t[w];
+                if (t != 0 && t != R - 1) continue;
+                const long long origin = a.trips[w];
+                if (origin < 0) {
+                    a.trips[w] = t;
+                    a.trips[2 * R + w] = sweep_global;
+                } else if (t != origin) {
+                    a.trips[R + w] = 1;
+                } else if (a.trips[R + w]) {
+                    a.trips[3 * R + w] += 1;
+                    a.trips[4 * R + w] += sweep_global - a.trips[2 * R + w];
+                    a.trips[2 * R + w] = sweep_global;
+                    a.trips[R + w] = 0;
                 }
-                for (int k = tid; k < R; k += kBatchThreads) {
-                    a.betas[k] = pt_beta[k];
-                    a.t2w[k] = pt_t2w[k];
-                    a.w2t[k] = pt_w2t[k];
-                    a.energies[k] = pt_e[k];
-                }
-                for (int k = tid; k < 2 * (R - 1); k += kBatchThreads) a.pt_rng[k] = pt_g[k];
             }
             if (R > 2) parity = 1 - parity;
-            if (tid == 0) {
-                my_beta = pt_beta[r];
-                my_t = pt_w2t[r];
-                if (MODEL == MODEL_XY_GAUSS) my_sigma = a.sigmas[my_t];
+            my_beta = pt_beta[r];
+            my_t = pt_w2t[r];
+            if (MODEL == MODEL_XY_GAUSS) sigma = a.sigmas[my_t];
+            if (MODEL == MODEL_ISING) {
+                if (tid < 5) thr[tid] = ising_threshold(4 * tid - 8, my_beta, a.J);
+                __syncthreads();
             }
-            __syncthreads();
-            if (MODEL == MODEL_ISING && tid < 5) thr[tid] = ising_threshold(4 * tid - 8, my_beta, a.J);
-            __syncthreads();
         }
 
         if (do_sample) {
@@ -629,18 +636,25 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(BatchArgs a) {
 #pragma unroll
                     for (int c = 0; c < kChannels; c++) dst[c] = v[c];
                 }
-                if (blockIdx.x == 0)
-                    for (int k = tid; k < R; k += kBatchThreads)
-                        a.sample_w2t[(size_t)idx * R + k] = use_pt ? pt_w2t[k] : k;
+                if (s == 0 && tid == 0) a.sample_w2t[(size_t)idx * R + r] = my_t;
             }
         }
     }
 
 done:
     flush_accepted();
+    if (use_pt && blockIdx.x == 0) {  // the PT state after the last exchange
+        for (int k = tid; k < R; k += NT) {
+            a.betas[k] = pt_beta[k];
+            a.t2w[k] = pt_t2w[k];
+            a.w2t[k] = pt_w2t[k];
+            if (exchanged) a.energies[k] = pt_e[k];
+        }
+        for (int k = tid; k < 2 * (R - 1); k += NT) a.pt_rng[k] = pt_g[k];
+    }
     // ---- write back own rows: state and RNG streams
     __syncthreads();
-    for (int idx = tid; idx < H * L; idx += kBatchThreads) {
+    for (int idx = tid; idx < H * L; idx += NT) {
         const int lr = 1 + idx / L, j = idx - (lr - 1) * L;
         const size_t gk = rep + (size_t)(row0 + lr - 1) * L + j;
         reinterpret_cast<ulonglong2 *>(a.rng)[gk] = lrng[lds_index(lr - 1, j, L)];
@@ -671,19 +685,25 @@ static size_t batch_lds(int model, int L, int R, int H, int exchange) {
     return ((strip_bytes(model, L, H) + 15) & ~(size_t)15) + pt_bytes + 64;
 }
 
-static const void *kernel_ptr(int model) {
-    if (model == MODEL_ISING) return reinterpret_cast<const void *>(&batch_kernel<MODEL_ISING>);
-    if (model == MODEL_XY_FAST) return reinterpret_cast<const void *>(&batch_kernel<MODEL_XY_FAST>);
-    return reinterpret_cast<const void *>(&batch_kernel<MODEL_XY_GAUSS>);
+template <int NT>
+static const void *kernel_ptr_nt(int model) {
+    if (model == MODEL_ISING) return reinterpret_cast<const void *>(&batch_kernel<MODEL_ISING, NT>);
+    if (model == MODEL_XY_FAST) return reinterpret_cast<const void *>(&batch_kernel<MODEL_XY_FAST, NT>);
+    return reinterpret_cast<const void *>(&batch_kernel<MODEL_XY_GAUSS, NT>);
 }
 
-static int check_shape(int model, int L, int R, int H) {
+static const void *kernel_ptr(int model, int threads) {
+    return threads == 512 ? kernel_ptr_nt<512>(model) : kernel_ptr_nt<256>(model);
+}
+
+static int check_shape(int model, int L, int R, int H, int threads) {
+    MC_REQUIRE(threads == 256 || threads == 512, "workgroup size must be 256 or 512, got %d", threads);
     MC_REQUIRE(model == MODEL_ISING || model == MODEL_XY_FAST || model == MODEL_XY_GAUSS,
                "unknown batch model %d", model);
     MC_REQUIRE(L >= 4 && L % 2 == 0, "L must be even and >= 4, got %d", L);
     MC_REQUIRE(H >= 4 && H % 2 == 0 && L % H == 0, "strip height %d must be even, >= 4 and divide L=%d",
                H, L);
-    MC_REQUIRE(L / 2 <= kBatchThreads, "L=%d is too wide for one workgroup (L/2 <= %d)", L, kBatchThreads);
+    MC_REQUIRE(L / 2 <= threads, "L=%d is too wide for a %d-thread workgroup (L/2 <= threads)", L, threads);
     MC_REQUIRE(R >= 1, "need at least one replica");
     return MC_OK;
 }
@@ -695,15 +715,15 @@ using namespace mc;
 extern "C" {
 
 int mc_batch_workspace_size(int model, int L, int R, int H, size_t *halo_words_out, size_t *energy_words_out) {
-    int rc = check_shape(model, L, R, H);
+    int rc = check_shape(model, L, R, H, 256);
     if (rc) return rc;
     *halo_words_out = halo_words(L, R, H);
     *energy_words_out = energy_words(L, R, H);
     return MC_OK;
 }
 
-int mc_batch_max_workgroups(int model, int L, int R, int H, int exchange, int *max_workgroups) {
-    int rc = check_shape(model, L, R, H);
+int mc_batch_max_workgroups(int model, int L, int R, int H, int threads, int exchange, int *max_workgroups) {
+    int rc = check_shape(model, L, R, H, threads);
     if (rc) return rc;
     const size_t lds = batch_lds(model, L, R, H, exchange);
     if (lds > 160 * 1024) {
@@ -711,22 +731,24 @@ int mc_batch_max_workgroups(int model, int L, int R, int H, int exchange, int *m
         return MC_OK;
     }
     if (lds > 64 * 1024)
-        MC_TRY_HIP(hipFuncSetAttribute(kernel_ptr(model), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        MC_TRY_HIP(hipFuncSetAttribute(kernel_ptr(model, threads), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds));
     int per_cu = 0, dev = 0, cus = 0;
-    MC_TRY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(model), kBatchThreads, lds));
+    MC_TRY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(model, threads), threads, lds));
     MC_TRY_HIP(hipGetDevice(&dev));
     MC_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     *max_workgroups = per_cu * cus;
     return MC_OK;
 }
 
-int mc_batch_run(void *stream, int model, int L, int R, int H, void *state, uint64_t *rng, float *betas,
-                 float J, float delta, const float *sigmas, unsigned long long *accepted, int nsweeps, long long sweep0, int exchange_interval, int *parity,
+int mc_batch_run(void *stream, int model, int L, int R, int H, int threads, void *state, uint64_t *rng,
+                 float *betas, float J, float delta, const float *sigmas, unsigned long long *accepted,
+                 int nsweeps, long long sweep0, int exchange_interval, int *parity,
                  double *energies, int32_t *w2t, int32_t *t2w, uint64_t *pt_rng, long long *counts,
                  long long *trips, int sample_interval, long long sample_first, int max_samples,
                  double *sample_partials, int32_t *sample_w2t, uint64_t *halo, uint64_t *energy,
                  int *error) {
-    int rc = check_shape(model, L, R, H);
+    int rc = check_shape(model, L, R, H, threads);
     if (rc) return rc;
     MC_REQUIRE(nsweeps >= 0, "nsweeps must be nonnegative");
     MC_REQUIRE(exchange_interval >= 0, "exchange_interval must be nonnegative");
@@ -750,14 +772,15 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, void *state, uint
     a.halo = halo; a.energy = energy; a.error = error;
 
     if (lds > 64 * 1024)
-        MC_TRY_HIP(hipFuncSetAttribute(kernel_ptr(model), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        MC_TRY_HIP(hipFuncSetAttribute(kernel_ptr(model, threads), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds));
     const size_t nh = halo_words(L, R, H);
     const unsigned reset_blocks = (unsigned)std::min<size_t>((nh + 255) / 256, 1024);
     hipLaunchKernelGGL(batch_reset_kernel, dim3(reset_blocks), dim3(256), 0, as_stream(stream), halo, nh, energy,
                        energy_words(L, R, H), error);
     MC_TRY_HIP(hipGetLastError());
     void *kargs[] = {&a};
-    MC_TRY_HIP(hipLaunchCooperativeKernel(kernel_ptr(model), dim3((unsigned)(R * S)), dim3(kBatchThreads),
+    MC_TRY_HIP(hipLaunchCooperativeKernel(kernel_ptr(model, threads), dim3((unsigned)(R * S)), dim3(threads),
                                           kargs, lds, as_stream(stream)));
     return MC_OK;
 }

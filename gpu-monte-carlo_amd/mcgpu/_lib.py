@@ -73,8 +73,8 @@ SIGNATURES = {
     "mc_kawamura_twist_terms": [P, INT, P, INT, INT, P, P],
     "mc_kawamura_twist_grad": [P, INT, P, INT, INT, P, P, P, P, P],
     "mc_batch_workspace_size": [INT, INT, INT, INT, ctypes.POINTER(SZ), ctypes.POINTER(SZ)],
-    "mc_batch_max_workgroups": [INT, INT, INT, INT, INT, ctypes.POINTER(INT)],
-    "mc_batch_run": [P, INT, INT, INT, INT, P, P, P, F32, F32, P, P, INT, I64, INT, P, P, P, P, P, P, P,
+    "mc_batch_max_workgroups": [INT, INT, INT, INT, INT, INT, ctypes.POINTER(INT)],
+    "mc_batch_run": [P, INT, INT, INT, INT, INT, P, P, P, F32, F32, P, P, INT, I64, INT, P, P, P, P, P, P, P,
                      INT, I64, INT, P, P, P, P, P],
 }
 

@@ -204,14 +204,15 @@ int mc_kawamura_twist_grad(void *stream, int dtype, const void *state, int R, in
  * same with Gaussian proposals (sigmas: R entries per ladder temperature).
  * accepted (R entries per ladder temperature, may be NULL) counts accepted
  * moves.  The state is split into L/H strips of H rows per replica; each
- * workgroup keeps its strip on chip.  Workspace sizes come from
- * mc_batch_workspace_size. */
+ * workgroup (threads = 256 or 512) keeps its strip on chip.  Workspace sizes
+ * come from mc_batch_workspace_size. */
 int mc_batch_workspace_size(int model, int L, int R, int H, size_t *halo_words,
                             size_t *energy_words);
 /* Largest grid (R * L / H workgroups) that can be co-resident for this
  * model and strip height. */
-int mc_batch_max_workgroups(int model, int L, int R, int H, int exchange, int *max_workgroups);
-int mc_batch_run(void *stream, int model, int L, int R, int H, void *state, uint64_t *rng,
+int mc_batch_max_workgroups(int model, int L, int R, int H, int threads, int exchange,
+                            int *max_workgroups);
+int mc_batch_run(void *stream, int model, int L, int R, int H, int threads, void *state, uint64_t *rng,
                  float *betas, float J, float delta, const float *sigmas,
                  unsigned long long *accepted, int nsweeps, long long sweep0,
                  int exchange_interval, int *parity, double *energies, int32_t *walker_to_temperature,

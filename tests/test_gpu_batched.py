@@ -27,14 +27,15 @@ def _plain_ising(L, temps, seed, interval, pt_seed, nsweeps, start="random"):
     return rt, pt
 
 
-@pytest.mark.parametrize("L,R,H", [(8, 2, 8), (16, 5, 4), (32, 8, 32), (32, 8, 8), (64, 3, 16)])
-def test_batched_ising_matches_drop_in_path_bit_for_bit(gpu, L, R, H):
+@pytest.mark.parametrize("L,R,H,threads", [(8, 2, 8, 256), (16, 5, 4, 256), (32, 8, 32, 256), (32, 8, 8, 256),
+                                           (64, 3, 16, 256), (64, 3, 32, 512), (128, 2, 16, 512)])
+def test_batched_ising_matches_drop_in_path_bit_for_bit(gpu, L, R, H, threads):
     from core.batched import BatchedRun
     from models.ising.model import IsingModel
     temps = np.linspace(2.0, 2.6, R)
     seed, pt_seed, interval, n = 40 + L, 43, 3, 37
     br = BatchedRun(IsingModel(start="random"), L, temps, seed=seed, exchange_interval=interval,
-                    pt_seed=pt_seed, strip_rows=H)
+                    pt_seed=pt_seed, strip_rows=H, threads=threads)
     br.run(n)
     rt, pt = _plain_ising(L, temps, seed, interval, pt_seed, n)
     np.testing.assert_array_equal(br.snapshot(), rt.snapshot())
@@ -78,13 +79,14 @@ def test_batched_ising_simulation_samples_equal_drop_in_simulation(gpu):
         np.testing.assert_array_equal(batched["samples"][name], plain["samples"][name], err_msg=name)
 
 
-@pytest.mark.parametrize("L,R,H", [(16, 3, 16), (32, 4, 8), (128, 2, 16)])
-def test_batched_xy_fast_trajectory_matches_oracle(gpu, L, R, H):
+@pytest.mark.parametrize("L,R,H,threads", [(16, 3, 16, 256), (32, 4, 8, 256), (128, 2, 16, 256),
+                                           (128, 2, 16, 512), (64, 3, 32, 512)])
+def test_batched_xy_fast_trajectory_matches_oracle(gpu, L, R, H, threads):
     from core.batched import BatchedRun
     from models.xy.model import XYModel
     temps = np.geomspace(0.7, 1.1, R)
     model = XYModel(start="random")
-    br = BatchedRun(model, L, temps, seed=170, exchange_interval=0, strip_rows=H)
+    br = BatchedRun(model, L, temps, seed=170, exchange_interval=0, strip_rows=H, threads=threads)
     theta = model.initialize_state(L, R, 170).copy()
     c, s = oracle.xyfast_cache(theta)
     rng = oracle.rng_init(R * L * L, 170)

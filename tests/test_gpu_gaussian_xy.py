@@ -100,14 +100,15 @@ def _fast_reference(theta, c, s, rng, betas, sig_walker, nsweeps, acc_walker):
             oracle.xyfast_gauss_color(theta, c, s, rng, betas, sig_walker, color, 1.0, acc_walker)
 
 
-@pytest.mark.parametrize("L,R,H", [(16, 3, 16), (32, 4, 8), (64, 2, 4)])
-def test_batched_gaussian_trajectory_matches_oracle(gpu, L, R, H):
+@pytest.mark.parametrize("L,R,H,threads", [(16, 3, 16, 256), (32, 4, 8, 256), (64, 2, 4, 256),
+                                           (128, 2, 16, 512)])
+def test_batched_gaussian_trajectory_matches_oracle(gpu, L, R, H, threads):
     from core.batched import BatchedRun
     from models.xy.model import XYModel
     temps = np.geomspace(0.7, 1.1, R)
     sig = np.linspace(0.5, 2.5, R)
     model = XYModel(start="random", proposal="gaussian", sigma=sig)
-    br = BatchedRun(model, L, temps, seed=77, exchange_interval=0, strip_rows=H)
+    br = BatchedRun(model, L, temps, seed=77, exchange_interval=0, strip_rows=H, threads=threads)
     br.run(16)
     theta = model.initialize_state(L, R, 77).copy()
     c, s = oracle.xyfast_cache(theta)
