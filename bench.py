@@ -65,13 +65,12 @@ def _model(proposal, sigma=1.0, dtype=np.float32):
     return XYModel(J=1.0, delta=1.5, dtype=dtype, start="random")
 
 
-def bench_batched(steps, warmup, strip_rows, tune, proposal="gaussian"):
+def bench_batched(steps, warmup, strip_rows, tune, proposal="gaussian", threads=None):
     from core.batched import BatchedRun, autotune
     from mcgpu import synchronize
     from mcgpu.timing import EventTimer
 
     model = _model(proposal)
-    threads = None
     if strip_rows is None and tune:
         log(f"autotuning strip height and workgroup size ({proposal})")
         strip_rows, threads = autotune(model, L, temperatures(), seed=SEED, exchange_interval=EXCHANGE,
@@ -201,6 +200,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-uniform", action="store_true")
     ap.add_argument("--strip-rows", type=int, default=None)
+    ap.add_argument("--threads", type=int, default=None, help="workgroup size with --strip-rows (256 or 512)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device once, so a counter run holds a "
@@ -218,7 +218,7 @@ def main():
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
 
-    b = bench_batched(args.steps, args.warmup, args.strip_rows, not args.no_autotune)
+    b = bench_batched(args.steps, args.warmup, args.strip_rows, not args.no_autotune, threads=args.threads)
     value = b["site_updates"] / (b["wall_s"] * 1e9)
     kernel_rate = b["site_updates"] / (b["kernel_ms"] * 1e6)
     achieved = BYTES_PER_SITE_UPDATE * b["site_updates"] / (b["kernel_ms"] * 1e-3) / 1e9
@@ -265,11 +265,11 @@ def main():
     }
     if not args.no_uniform:
         u = bench_batched(max(args.steps // 3, 1), max(args.warmup // 3, 1), args.strip_rows, not args.no_autotune,
-                          proposal="uniform")
+                          proposal="uniform", threads=args.threads)
         out["uniform_window"] = {
             "value": round(u["site_updates"] / (u["wall_s"] * 1e9), 4), "unit": "site-updates/ns",
             "kernel_site_updates_per_ns": round(u["site_updates"] / (u["kernel_ms"] * 1e6), 3),
-            "strip_rows": u["strip_rows"],
+            "strip_rows": u["strip_rows"], "workgroup_threads": u["threads"],
             "note": "batched FP32 with the reference's uniform window, delta = 1.5"}
     if not args.no_plain:
         p64 = bench_plain(np.float64, args.plain_sweeps, b["sigmas"])
