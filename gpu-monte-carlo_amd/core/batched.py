@@ -109,11 +109,13 @@ def default_strip_rows(code, L, R, exchange, target_workgroups=512):
     return default_config(code, L, R, exchange, target_workgroups)[0]
 
 
-class BatchedRun:
-    """One model, one lattice size, all its replicas: state, streams and PT on the device."""
+class Ensemble:
+    """One model, one lattice size, all its replicas: state, RNG streams, PT
+    state, proposal widths and acceptance counters on the device, with the
+    host bookkeeping that launches advance.  ``BatchedRun`` runs one ensemble
+    per launch; ``core.packed.PackedRun`` runs several in one launch."""
 
-    def __init__(self, model, L, temperatures, seed=42, exchange_interval=10, pt_seed=None,
-                 strip_rows=None, stream=True, threads=None):
+    def __init__(self, model, L, temperatures, seed=42, exchange_interval=10, pt_seed=None):
         if L < 4 or L % 2 != 0:
             raise ValueError("L must be even and >= 4")
         temperatures = np.asarray(temperatures, dtype=np.float64)
@@ -171,12 +173,132 @@ class BatchedRun:
         if self.code == MODEL_XY_GAUSS:
             self._sigmas_host = np.broadcast_to(np.asarray(model.sigma, dtype=np.float64), (R,)).copy()
             self.sigmas = to_device(self._sigmas_host.astype(np.float32))
+        self._owner = None  # weak reference to the engine that runs this ensemble
 
+    # ------------------------------------------------------- bookkeeping
+    def _sample_plan(self, nsweeps, sample_interval, sample_first):
+        """(first sampled sweep, number of samples) of a launch of `nsweeps`."""
+        if sample_interval and sample_interval < 1:
+            raise ValueError("sample_interval must be positive")
+        if not sample_interval:
+            return 0, 0
+        sweep0 = self.sweeps
+        if sample_first is None:
+            sample_first = sweep0 + sample_interval
+        last = sweep0 + nsweeps
+        return sample_first, (0 if last < sample_first else (last - sample_first) // sample_interval + 1)
+
+    def _advance(self, nsweeps):
+        sweep0 = self.sweeps
+        self.sweeps += nsweeps
+        self._counted_sweeps += nsweeps
+        if self.exchange_interval:
+            self.exchange_rounds += (sweep0 + nsweeps) // self.exchange_interval - sweep0 // self.exchange_interval
+
+    def _samples(self, partials, sw2t, n, first, interval):
+        """Samples in temperature order from per-strip partials (n, R, S, C),
+        strips added in order, and the walkers' temperatures (n, R)."""
+        names = NAMES[self.code]
+        if n == 0:
+            return {name: np.zeros((0, self.R)) for name in names}, np.zeros(0, dtype=np.int64)
+        walker = partials[:n].sum(axis=2)
+        w2t = sw2t[:n]
+        out = {}
+        rows = np.arange(n)[:, None]
+        for c, name in enumerate(names):
+            values = np.empty((n, self.R))
+            values[rows, w2t] = walker[:, :, c]
+            out[name] = values
+        return out, first + interval * np.arange(n, dtype=np.int64)
+
+    @property
+    def owner(self):
+        """The packed engine that runs this ensemble, if any."""
+        return self._owner() if self._owner is not None else None
+
+    def synchronize(self):
+        owner = self.owner
+        if owner is not None:
+            owner.synchronize()
+        else:
+            synchronize()
+
+    # -------------------------------------------------- proposals, acceptance
+    def acceptance_by_temperature(self):
+        """Accepted fraction of site updates per ladder temperature since the last reset."""
+        self.synchronize()
+        acc = self._accepted.copy_to_host().astype(np.float64)
+        attempts = float(self.L * self.L * self._counted_sweeps)
+        return acc / attempts if attempts else np.full(self.R, np.nan)
+
+    def reset_acceptance(self):
+        self._accepted.fill_zero()
+        self._counted_sweeps = 0
+
+    def set_sigmas(self, sigmas):
+        from models.xy.proposals import SIGMA_MAX, SIGMA_MIN
+        if self.code != MODEL_XY_GAUSS:
+            raise ValueError("proposal widths exist only for Gaussian XY proposals")
+        sig = np.clip(np.broadcast_to(np.asarray(sigmas, dtype=np.float64), (self.R,)), SIGMA_MIN, SIGMA_MAX)
+        self._sigmas_host = sig.copy()
+        self.sigmas.copy_to_device(sig.astype(np.float32))
+
+    def _adapt_step(self, target, rate):
+        rates = self.acceptance_by_temperature()
+        self.set_sigmas(self._sigmas_host * np.exp(rate * (rates - target)))
+        self.reset_acceptance()
+        return self._sigmas_host.copy(), rates
+
+    # ---------------------------------------------------------- inspection
+    def snapshot(self):
+        """Host state in walker order."""
+        return self.state.copy_to_host()
+
+    def measure(self):
+        """Observables of the current state through the drop-in measurement kernels,
+        in temperature order."""
+        return self.model.measure(self.state, temperature_indices=self.walker_to_temperature)
+
+    def diagnostics(self):
+        counts = self._swap_counts.copy_to_host()
+        if self.R > 1 and np.any(counts[2]):
+            raise FloatingPointError("Nonfinite energies encountered during replica exchange")
+        trips = self._trips.copy_to_host()
+        attempts, accepted = counts[0, :self.R - 1], counts[1, :self.R - 1]
+        rates = np.divide(accepted, attempts, out=np.zeros(self.R - 1), where=attempts > 0)
+        completed = int(trips[3].sum())
+        return {
+            "exchange_interval": self.exchange_interval,
+            "exchange_seed": self.exchange_seed,
+            "sweeps": self.sweeps,
+            "exchange_rounds": self.exchange_rounds,
+            "next_parity": int(self._parity.copy_to_host()[0]),
+            "diagnostics_scope": "since_wrapper_creation_including_thermalization",
+            "temperature_betas": self._temperature_betas.copy(),
+            "attempts": attempts.copy(),
+            "accepted": accepted.copy(),
+            "acceptance_rates": rates,
+            "walker_to_temperature": self.walker_to_temperature.copy_to_host(),
+            "temperature_to_walker": self.temperature_to_walker.copy_to_host(),
+            "walker_round_trips": trips[3].copy(),
+            "walker_round_trip_total_sweeps": trips[4].copy(),
+            "mean_round_trip_sweeps": float(trips[4].sum()) / completed if completed else None,
+            "acceptance_by_temperature": self.acceptance_by_temperature(),
+            "sigmas": None if self.sigmas is None else self._sigmas_host.copy(),
+        }
+
+
+class BatchedRun(Ensemble):
+    """One ensemble in one persistent launch: every strip resident in LDS."""
+
+    def __init__(self, model, L, temperatures, seed=42, exchange_interval=10, pt_seed=None,
+                 strip_rows=None, stream=True, threads=None):
+        super().__init__(model, L, temperatures, seed, exchange_interval, pt_seed)
         if strip_rows:
             self.H = int(strip_rows)
             self.threads = int(threads) if threads else _min_threads(L)
         else:
-            self.H, t = default_config(self.code, L, R, exchange_interval)
+            self.H, t = default_config(self.code, L, self.R, self.exchange_interval)
             self.threads = int(threads) if threads else t
         self._alloc_workspace()
         self._stream = None
@@ -214,16 +336,8 @@ class BatchedRun:
         nsweeps = int(nsweeps)
         if nsweeps < 0:
             raise ValueError("nsweeps must be nonnegative")
-        if sample_interval and sample_interval < 1:
-            raise ValueError("sample_interval must be positive")
+        sample_first, n_samples = self._sample_plan(nsweeps, sample_interval, sample_first)
         sweep0 = self.sweeps
-        if sample_interval:
-            if sample_first is None:
-                sample_first = sweep0 + sample_interval
-            last = sweep0 + nsweeps
-            n_samples = 0 if last < sample_first else (last - sample_first) // sample_interval + 1
-        else:
-            sample_first, n_samples = 0, 0
         S = self.strips
         partials = device_array((max(n_samples, 1), self.R, S, CHANNELS), np.float64)
         sw2t = device_array((max(n_samples, 1), self.R), np.int32)
@@ -237,10 +351,7 @@ class BatchedRun:
             as_pointer(self._trips), int(sample_interval or 0), int(sample_first), n_samples,
             as_pointer(partials), as_pointer(sw2t), as_pointer(self._halo), as_pointer(self._energy_words),
             as_pointer(self._error))
-        self.sweeps += nsweeps
-        self._counted_sweeps += nsweeps
-        if self.exchange_interval:
-            self.exchange_rounds += (sweep0 + nsweeps) // self.exchange_interval - sweep0 // self.exchange_interval
+        self._advance(nsweeps)
         return (partials, sw2t, n_samples, sample_first, sample_interval)
 
     def synchronize(self):
@@ -257,42 +368,12 @@ class BatchedRun:
         """Wait for a launch and return its samples in temperature order."""
         partials, sw2t, n, first, interval = handle
         self.synchronize()
-        names = NAMES[self.code]
         if n == 0:
-            return {name: np.zeros((0, self.R)) for name in names}, np.zeros(0, dtype=np.int64)
-        per_strip = partials.copy_to_host()[:n]
-        w2t = sw2t.copy_to_host()[:n]
-        walker = per_strip.sum(axis=2)  # (n, R, C), strips added in order
-        out = {}
-        rows = np.arange(n)[:, None]
-        for c, name in enumerate(names):
-            values = np.empty((n, self.R))
-            values[rows, w2t] = walker[:, :, c]
-            out[name] = values
-        return out, first + interval * np.arange(n, dtype=np.int64)
+            return self._samples(None, None, 0, first, interval)
+        return self._samples(partials.copy_to_host(), sw2t.copy_to_host(), n, first, interval)
 
     def run(self, nsweeps, sample_interval=0, sample_first=None):
         return self.collect(self.launch(nsweeps, sample_interval, sample_first))
-
-    # -------------------------------------------------- proposals, acceptance
-    def acceptance_by_temperature(self):
-        """Accepted fraction of site updates per ladder temperature since the last reset."""
-        self.synchronize()
-        acc = self._accepted.copy_to_host().astype(np.float64)
-        attempts = float(self.L * self.L * self._counted_sweeps)
-        return acc / attempts if attempts else np.full(self.R, np.nan)
-
-    def reset_acceptance(self):
-        self._accepted.fill_zero()
-        self._counted_sweeps = 0
-
-    def set_sigmas(self, sigmas):
-        from models.xy.proposals import SIGMA_MAX, SIGMA_MIN
-        if self.code != MODEL_XY_GAUSS:
-            raise ValueError("proposal widths exist only for Gaussian XY proposals")
-        sig = np.clip(np.broadcast_to(np.asarray(sigmas, dtype=np.float64), (self.R,)), SIGMA_MIN, SIGMA_MAX)
-        self._sigmas_host = sig.copy()
-        self.sigmas.copy_to_device(sig.astype(np.float32))
 
     def adapt_sigmas(self, sweeps, interval=50, target=0.5, rate=1.0):
         """Thermalise `sweeps` sweeps, moving each temperature's width toward `target`
@@ -309,51 +390,14 @@ class BatchedRun:
             n = min(interval, sweeps - done)
             self.run(n)
             done += n
-            rates = self.acceptance_by_temperature()
-            self.set_sigmas(self._sigmas_host * np.exp(rate * (rates - target)))
-            history.append((self._sigmas_host.copy(), rates))
-            self.reset_acceptance()
+            history.append(self._adapt_step(target, rate))
         return history
 
-    # ---------------------------------------------------------- inspection
-    def snapshot(self):
-        """Host state in walker order."""
-        return self.state.copy_to_host()
-
-    def measure(self):
-        """Observables of the current state through the drop-in measurement kernels,
-        in temperature order."""
-        return self.model.measure(self.state, temperature_indices=self.walker_to_temperature)
-
     def diagnostics(self):
-        counts = self._swap_counts.copy_to_host()
-        if self.R > 1 and np.any(counts[2]):
-            raise FloatingPointError("Nonfinite energies encountered during replica exchange")
-        trips = self._trips.copy_to_host()
-        attempts, accepted = counts[0, :self.R - 1], counts[1, :self.R - 1]
-        rates = np.divide(accepted, attempts, out=np.zeros(self.R - 1), where=attempts > 0)
-        completed = int(trips[3].sum())
-        return {
-            "exchange_interval": self.exchange_interval,
-            "exchange_seed": self.exchange_seed,
-            "sweeps": self.sweeps,
-            "exchange_rounds": self.exchange_rounds,
-            "next_parity": int(self._parity.copy_to_host()[0]),
-            "diagnostics_scope": "since_wrapper_creation_including_thermalization",
-            "temperature_betas": self._temperature_betas.copy(),
-            "attempts": attempts.copy(),
-            "accepted": accepted.copy(),
-            "acceptance_rates": rates,
-            "walker_to_temperature": self.walker_to_temperature.copy_to_host(),
-            "temperature_to_walker": self.temperature_to_walker.copy_to_host(),
-            "walker_round_trips": trips[3].copy(),
-            "walker_round_trip_total_sweeps": trips[4].copy(),
-            "mean_round_trip_sweeps": float(trips[4].sum()) / completed if completed else None,
-            "strip_rows": self.H,
-            "workgroup_threads": self.threads,
-            "acceptance_by_temperature": self.acceptance_by_temperature(),
-            "sigmas": None if self.sigmas is None else self._sigmas_host.copy(),
-        }
+        d = super().diagnostics()
+        d["strip_rows"] = self.H
+        d["workgroup_threads"] = self.threads
+        return d
 
 
 def autotune(model, L, temperatures, seed=42, exchange_interval=10, sweeps=300, candidates=None,
@@ -392,6 +436,21 @@ def autotune_strip_rows(model, L, temperatures, seed=42, exchange_interval=10, s
     return autotune(model, L, temperatures, seed, exchange_interval, sweeps, cands, verbose)[0]
 
 
+def run_metadata(ens, sample_sweeps, thermalization_sweeps, measurement_sweeps, sample_interval, engine):
+    """Run metadata in the reference's layout (core/simulation.py:80-91)."""
+    return {
+        "L": ens.L,
+        "temperatures": ens.temperatures.copy(),
+        "seed": ens.seed,
+        "model_parameters": ens.model.parameters(),
+        "thermalization_sweeps": thermalization_sweeps,
+        "measurement_sweeps": measurement_sweeps,
+        "sample_interval": sample_interval,
+        "sample_sweeps": sample_sweeps,
+        "engine": engine,
+    }
+
+
 class BatchedSimulation:
     """Thermalise and sample several BatchedRuns concurrently (one stream each).
 
@@ -420,17 +479,8 @@ class BatchedSimulation:
         out = {}
         for br, h in zip(self.runs, handles):
             samples, sweeps = br.collect(h)
-            metadata = {
-                "L": br.L,
-                "temperatures": br.temperatures.copy(),
-                "seed": br.seed,
-                "model_parameters": br.model.parameters(),
-                "thermalization_sweeps": self.thermalization_sweeps,
-                "measurement_sweeps": self.measurement_sweeps,
-                "sample_interval": self.sample_interval,
-                "sample_sweeps": sweeps - (br.sweeps - total),
-                "engine": "batched",
-            }
+            metadata = run_metadata(br, sweeps - (br.sweeps - total), self.thermalization_sweeps,
+                                    self.measurement_sweeps, self.sample_interval, "batched")
             if br.exchange_interval:
                 metadata["parallel_tempering"] = br.diagnostics()
             key = br.L

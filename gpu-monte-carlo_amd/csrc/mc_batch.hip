@@ -49,16 +49,9 @@
 // Accepted moves are counted per ladder temperature when `accepted` is given.
 #include <algorithm>
 
-#include "mc_common.h"
-#include "mc_fastmath.h"
+#include "mc_strip.h"
 
 namespace mc {
-
-enum { MODEL_ISING = 0, MODEL_XY_FAST = 1, MODEL_XY_GAUSS = 2 };
-enum { BATCH_ERR_WATCHDOG = 1 };
-
-constexpr unsigned kWatchdog = 1u << 20;  // about a second of polling
-constexpr int kChannels = 5;               // XY: e, mx, my, cx, qx ; Ising: e, m
 
 struct BatchArgs {
     int model, L, R, H, S;
@@ -87,90 +80,9 @@ struct BatchArgs {
     // synchronisation
     uint64_t *halo;             // [R][S][2 (row 2, row H)][2 (sweep parity)][L] words
                                 //   (sweep << 32 | payload)
-    uint64_t *energy;           // [2 (exchange parity)][R][S][2 (hi, lo)] words
-                                //   (exchange tag << 32 | float bits)
+    uint64_t *energy;           // [2 (exchange parity)][R][S][2 (lo, hi)] words
+                                //   (exchange tag << 32 | half of the int64 energy partial)
     int *error;
-};
-
-// LDS position of column j in LDS row `row` (colour-split layout)
-__device__ __forceinline__ int lds_index(int row, int j, int L) {
-    return row * L + (j & 1) * (L >> 1) + (j >> 1);
-}
-
-// Cross-workgroup protocol.  Every word that crosses workgroups (halo and
-// energy words) is written and read with agent-scope relaxed atomics, which
-// are device-coherent on their own (gfx950 sets the device-scope cache bits
-// on them), so no L2 write-back or invalidate is needed.  Each word is a
-// single-copy atomic 64-bit value whose high half names the sweep (halo) or
-// the exchange (energy) it belongs to, so a reader that sees the expected
-// tag also sees the payload written with it.
-struct Sync {
-    const BatchArgs &a;
-    // one thread: poll a word until it carries tag `seq`
-    __device__ bool poll_word(const uint64_t *p, unsigned seq, uint32_t &payload) const {
-        unsigned n = 0;
-        for (;;) {
-            const uint64_t w =
-                __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if ((unsigned)(w >> 32) == seq) {
-                payload = (uint32_t)w;
-                return true;
-            }
-            __builtin_amdgcn_s_sleep(1);
-            if ((++n & 1023u) == 0 &&
-                (n > kWatchdog || __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
-                return false;
-        }
-    }
-};
-
-// Ising acceptance threshold for 2 s sum_nn = v: always flip when dE <= 0,
-// else u < exp(-beta dE) in float32 through the float64 exponential.
-__device__ __forceinline__ float ising_threshold(int v, float beta, float J) {
-    const float dE = (float)v * J;
-    if (dE <= 0.0f) return __builtin_inff();
-    return rexp(-beta * dE);
-}
-
-// Block reduction of C float64 channels over NT threads; the result is valid
-// in thread 0 (waves added in order).
-template <int C, int NT>
-__device__ void block_sum(double (&v)[C], double (*sh)[NT / 64]) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int c = 0; c < C; c++) {
-        double x = v[c];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-        if (lane == 0) sh[c][wave] = x;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int c = 0; c < C; c++) {
-            double t = sh[c][0];
-#pragma unroll
-            for (int w = 1; w < NT / 64; w++) t += sh[c][w];
-            v[c] = t;
-        }
-    }
-    __syncthreads();
-}
-
-// LDS bytes of one strip's site data (everything but the PT scratch): RNG
-// streams and angles of rows 1..H+1, cos/sin (or spins) of rows 0..H+2
-__host__ __device__ inline size_t strip_bytes(int model, int L, int H) {
-    const size_t rng = (size_t)(H + 1) * L * 16;
-    return model == MODEL_ISING ? rng + (size_t)(H + 3) * L
-                                : rng + (size_t)(H + 3) * L * 8 + (size_t)(H + 1) * L * 4;
-}
-
-// One group of colour rows of a pass: colour c, colour rows lo..hi (empty if
-// hi < lo).  Colour rows of colours 0/1 are local rows 1, 3, ..., H+1
-// (ri = 0..H/2, ri = H/2 being the ghost row); of colours 2/3 local rows
-// 2, 4, ..., H (ri = 0..H/2-1).
-struct Seg {
-    int c, lo, hi;
 };
 
 template <int MODEL, int NT>
@@ -185,23 +97,24 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     int lin = blockIdx.x;
     if (nwg % 8 == 0 && (nwg / 8) % S == 0) lin = (blockIdx.x % 8) * (nwg / 8) + blockIdx.x / 8;
     const int r = lin / S, s = lin % S;
-    const Sync sync{a};
     const int s_up = (s + S - 1) % S, s_dn = (s + 1) % S;
 
     // ---- LDS carve-up
-    ulonglong2 *lrng = reinterpret_cast<ulonglong2 *>(smem);              // rows 1..H+1
-    float *lc = reinterpret_cast<float *>(lrng + (size_t)(H + 1) * L);     // rows 0..H+2
-    float *ls = lc + (H + 3) * L;                                          // rows 0..H+2
-    float *lth = ls + (H + 3) * L;                                         // rows 1..H+1
-    int8_t *lsp = reinterpret_cast<int8_t *>(lrng + (size_t)(H + 1) * L);  // rows 0..H+2 (Ising)
+    const StripView view = strip_view(smem, L, H);
+    ulonglong2 *lrng = view.rng;  // rows 1..H+1
+    float *lc = view.c;           // rows 0..H+2
+    float *ls = view.s;           // rows 0..H+2
+    float *lth = view.th;         // rows 1..H+1
+    int8_t *lsp = view.sp;        // rows 0..H+2 (Ising)
     unsigned char *pt_base = smem + ((strip_bytes(MODEL, L, H) + 15) & ~(size_t)15);
     float *pt_beta = reinterpret_cast<float *>(pt_base);                 // R
     int32_t *pt_t2w = reinterpret_cast<int32_t *>(pt_beta + R);          // R
     int32_t *pt_w2t = pt_t2w + R;                                         // R
     double *pt_e = reinterpret_cast<double *>(pt_base + ((size_t)(3 * R * 4 + 7) & ~(size_t)7));  // R
     uint64_t *pt_g = reinterpret_cast<uint64_t *>(pt_e + R);             // 2 (R-1)
-    float *pt_part = reinterpret_cast<float *>(pt_g + 2 * (R - 1));      // R S (hi, lo)
-    __shared__ double red[kChannels + 1][NT / 64];
+    uint32_t *pt_part = reinterpret_cast<uint32_t *>(pt_g + 2 * (R - 1));  // R S (lo, hi)
+    __shared__ double red[kChannels][NT / 64];
+    __shared__ long long red_e[NT / 64];
     __shared__ float thr[5];
     __shared__ int red_i[NT / 64];
 
@@ -246,70 +159,15 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     __syncthreads();
 
     const float J = a.J, delta = a.delta;
-    int n_acc = 0;       // accepted moves of this thread since the last flush
-    const float one = 1.0f, two = 2.0f;
-    const float pi = 3.14159274101257324f, two_pi = 6.28318548202514648f;
+    int n_acc = 0;  // accepted moves of this thread since the last flush
     // fixed thread -> (column, colour-row offset) map of every pass
     const int tcol = tid % half, trow = tid / half, rows_per_pass = NT / half;
 
-    // Update site (colour row ri, column 2 tcol + (c & 1)) of colour c.
+    // Update site (colour row ri, column 2 tcol + (c & 1)) of colour c.  The
+    // ghost row's moves belong to the strip below and are not counted.
     auto site = [&](const int c, const int ri, const float beta) {
-        const int p = c & 1;
-        const int lr = 1 + 2 * ri + (c >> 1);
-        const int base = lr * L;
-        const int self = base + p * half + tcol;
-        const int up = self - L, dn = self + L;
-        const int lf = p ? base + tcol : base + half + (tcol == 0 ? half - 1 : tcol - 1);
-        const int rt = p ? base + (tcol == half - 1 ? 0 : tcol + 1) : base + half + tcol;
-        const int own = self - L;  // RNG and angle arrays start at local row 1
-        const int counted = !((c < 2) & (ri == HH));  // the ghost row's moves are its owner's
-        const ulonglong2 st = lrng[own];
-        Rng g;
-        g.s0 = st.x;
-        g.s1 = st.y;
-        if (MODEL == MODEL_ISING) {
-            const int sv = lsp[self];
-            const int nb = (int)lsp[up] + (int)lsp[dn] + (int)lsp[lf] + (int)lsp[rt];
-            const float u = g.uniform_f32();
-            if (u < thr[(2 * sv * nb + 8) >> 2]) {
-                lsp[self] = (int8_t)(-sv);
-                n_acc += counted;
-            }
-        } else {
-            const float th = lth[own];
-            float pr;
-            float v;
-            if (MODEL == MODEL_XY_GAUSS) {
-                const GaussDraw d = gauss_draw(g.next());
-                pr = wrap_turns(th + sigma * d.z);
-                v = d.v;
-            } else {
-                const float u = g.uniform_f32();
-                pr = th + delta * (two * u - one);
-                if (pr >= pi)
-                    pr -= two_pi;
-                else if (pr < -pi)
-                    pr += two_pi;
-            }
-            float sp, cp;
-            fast_sincosf(pr, sp, cp);
-            const float Cs = ((lc[up] + lc[dn]) + lc[lf]) + lc[rt];
-            const float Ss = ((ls[up] + ls[dn]) + ls[lf]) + ls[rt];
-            float t = (cp - lc[self]) * Cs;
-            t = fmaf(sp - ls[self], Ss, t);
-            const float dE = -J * t;
-            if (MODEL != MODEL_XY_GAUSS) v = g.uniform_f32();
-            if (v < accept_weight(-beta * dE)) {  // dE <= 0 or v < exp(-beta dE)
-                lth[own] = pr;
-                lc[self] = cp;
-                ls[self] = sp;
-                n_acc += counted;
-            }
-        }
-        ulonglong2 out;
-        out.x = g.s0;
-        out.y = g.s1;
-        lrng[own] = out;
+        const int counted = !((c < 2) & (ri == HH));
+        if (site_update<MODEL>(view, c, ri, tcol, L, beta, sigma, J, delta, thr)) n_acc += counted;
     };
     // One pass: up to four groups of colour rows, spread over the threads in
     // the fixed map (a thread takes colour rows trow, trow + rows_per_pass, ...
@@ -382,7 +240,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             const int j = top ? k : k - L;
             const uint64_t *src = top ? halo_ptr(s_up, 1, par) : halo_ptr(s_dn, 0, par);
             uint32_t payload;
-            if (!sync.poll_word(src + j, (unsigned)n, payload)) {
+            if (!poll_word(src + j, (unsigned)n, payload, a.error)) {
                 bad = 1;
                 break;
             }
@@ -473,54 +331,25 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
         // ---- strip observables: right and down bonds (the row below row H is
         // the ghost row H+1, current for this sweep); the energy alone when
         // only an exchange needs it
+        long long ef = 0;  // integer energy sum of the strip
         double v[kChannels];
+        double nacc = 0.0;  // the workgroup's accepted moves travel with the sums
         {
-            double e = 0, m1 = 0, m2 = 0, cx = 0, qx = 0;
-            float ek = 0.0f, ec = 0.0f;  // compensated sum of the XY bond energies
-            for (int idx = tid; idx < H * L; idx += NT) {
-                const int lr = 1 + idx / L, j = idx % L;
-                const int self = lds_index(lr, j, L), dn = lds_index(lr + 1, j, L);
-                const int rt = lds_index(lr, j == L - 1 ? 0 : j + 1, L);
-                if (MODEL == MODEL_ISING) {
-                    const int sv = lsp[self];
-                    e += -(double)J * sv * ((int)lsp[dn] + (int)lsp[rt]);
-                    if (do_sample) m1 += sv;
-                } else {
-                    const float cs = lc[self], ss = ls[self];
-                    const float bxc = fmaf(ss, ls[dn], cs * lc[dn]);
-                    const float byc = fmaf(ss, ls[rt], cs * lc[rt]);
-                    const float y = -J * (bxc + byc) - ec;
-                    const float t = ek + y;
-                    ec = (t - ek) - y;
-                    ek = t;
-                    if (do_sample) {
-                        m1 += cs;
-                        m2 += ss;
-                        cx += bxc;
-                        qx += fmaf(ss, lc[dn], -(cs * ls[dn]));
-                    }
-                }
-            }
-            if (MODEL != MODEL_ISING) e = (double)ek - (double)ec;
-            v[0] = e;
-            v[1] = m1;
-            v[2] = m2;
-            v[3] = cx;
-            v[4] = qx;
-        }
-        // the workgroup's accepted moves travel with the sums (channel kChannels)
-        double nacc = 0.0;
-        if (do_sample) {
-            double w[kChannels + 1] = {v[0], v[1], v[2], v[3], v[4], (double)n_acc};
-            block_sum<kChannels + 1, NT>(w, red);
+            double m[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int idx = tid; idx < H * L; idx += NT)
+                site_observables<MODEL>(view, 1 + idx / L, idx % L, L, do_sample, ef, m);
+            if (do_sample) {
+                double w[5] = {m[0], m[1], m[2], m[3], (double)n_acc};
+                block_sum<5, NT>(ef, w, red_e, red);
 #pragma unroll
-            for (int c = 0; c < kChannels; c++) v[c] = w[c];
-            nacc = w[kChannels];
-        } else {
-            double w[2] = {v[0], (double)n_acc};
-            block_sum<2, NT>(w, red);
-            v[0] = w[0];
-            nacc = w[1];
+                for (int c = 0; c < 4; c++) v[c + 1] = w[c];
+                nacc = w[4];
+            } else {
+                double w[1] = {(double)n_acc};
+                block_sum<1, NT>(ef, w, red_e, red);
+                nacc = w[0];
+            }
+            v[0] = energy_from_fixed(MODEL, J, ef);
         }
         // moves so far belong to the current temperature
         if (tid == 0 && a.accepted && nacc > 0.0) atomicAdd(&a.accepted[my_t], (unsigned long long)nacc);
@@ -532,36 +361,32 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             const unsigned tag = (unsigned)xi + 1u;
             uint64_t *ew = a.energy + (size_t)(xi & 1) * R * S * 2;
             if (tid == 0) {
-                const float hi = (float)v[0];
-                const float lo = (float)(v[0] - (double)hi);
                 uint64_t *dst = ew + ((size_t)r * S + s) * 2;
-                __hip_atomic_store(dst, ((uint64_t)tag << 32) | __float_as_uint(hi), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(dst + 1, ((uint64_t)tag << 32) | __float_as_uint(lo), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+                publish_word(dst, tag, (uint32_t)(unsigned long long)ef);
+                publish_word(dst + 1, tag, (uint32_t)((unsigned long long)ef >> 32));
             }
             // every workgroup gathers every strip's partial (the same values
-            // everywhere) and sums each replica's strips in strip order
+            // everywhere) and adds each replica's strips as integers
             int bad = 0;
             for (int k = tid; k < R * S * 2; k += NT) {
                 uint32_t payload;
-                if (!sync.poll_word(ew + k, tag, payload)) {
+                if (!poll_word(ew + k, tag, payload, a.error)) {
                     bad = 1;
                     break;
                 }
-                pt_part[k] = __uint_as_float(payload);
+                pt_part[k] = payload;
             }
             if (__syncthreads_or(bad)) {
                 if (tid == 0) atomicOr(a.error, BATCH_ERR_WATCHDOG);
                 goto done;
             }
             for (int k = tid; k < R; k += NT) {
-                double tot = 0.0;
+                long long tot = 0;
                 for (int q = 0; q < S; q++) {
-                    const float *pp = pt_part + ((size_t)k * S + q) * 2;
-                    tot += (double)pp[0] + (double)pp[1];
+                    const uint32_t *pp = pt_part + ((size_t)k * S + q) * 2;
+                    tot += (long long)(((unsigned long long)pp[1] << 32) | pp[0]);
                 }
-                pt_e[k] = tot;
+                pt_e[k] = energy_from_fixed(MODEL, J, tot);
             }
             __syncthreads();
             // one matching of parity `parity` (core/parallel_tempering.py:44-69);

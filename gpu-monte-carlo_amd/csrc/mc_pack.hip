@@ -1,0 +1,891 @@
+// Packed persistent engine: the runs of several lattice sizes (and models) --
+// a finite-size-scaling set -- in one launch.
+//
+// A group is one (model, L, ladder) ensemble of R replicas; every group keeps
+// the reference's schedule of sweeps, exchanges every `exchange_interval`
+// sweeps (core/parallel_tempering.py:245-269) and samples.  A group's
+// replicas are cut into strips of H rows (mc_strip.h), and a task is one
+// strip position s of a block of K consecutive replicas.  Each workgroup runs
+// a fixed list of tasks of one group, all of them every sweep: the sizes of a
+// whole study do not fit on chip together (the RNG streams alone are 16 bytes
+// per site), so a task is streamed through LDS each sweep -- rows 1..H from
+// the lattice arrays, the ghost row from its own buffer, rows 0 and H+2 from
+// the neighbours' halo words -- then swept with the five-pass schedule of the
+// resident engine and written back.  One sweep of a task reads and writes
+// each of its sites' RNG stream and state once, plus one ghost row and two
+// halo rows per strip.
+//
+// Ordering.  A task at sweep n needs only its neighbours' rows as of the end
+// of sweep n-1, published as tagged halo words (tag n+1 = "at the start of
+// sweep n", slot n & 1), so any placement of tasks on co-resident workgroups
+// makes progress: the dependencies of (sweep n, task i of a workgroup) are
+// tasks of sweep n-1 and earlier tasks of the same workgroup.  The launch
+// must be fully co-resident (checked on the host against the occupancy of
+// its LDS size); every wait is bounded by a watchdog regardless.
+//
+// Exchanges.  Energies are integers (mc_strip.h), so a group sums its strip
+// energies with device-scope atomic adds into two running totals per replica
+// (by exchange parity) and counts arriving workgroups on a monotone counter;
+// after every workgroup of the group has arrived, each reads the R totals,
+// takes the difference to the totals it saw two exchanges earlier, and runs
+// the identical matching (core/parallel_tempering.py:44-91) on identical
+// inputs.  Counters of pair `left` and trip records of walker w are kept by
+// the group's workgroup left (w) mod its workgroup count; the group's first
+// workgroup writes the PT state back at the end of the launch.
+//
+// Graphs.  The sweep count lives in device memory (`clock`), so a captured
+// prologue + launch can be replayed: each replay continues the chain.  The
+// prologue (a kernel, not memset nodes) refreshes the ghost rows, publishes
+// the halo words for sweep 0 and clears the exchange totals.
+#include <algorithm>
+#include <vector>
+
+#include "mc_strip.h"
+
+namespace mc {
+
+struct PackGroup {
+    int model, L, R, H, S, K;
+    int ntasks;                // (R / K) * S; task t = (replica block t / S, strip t % S)
+    int wg_first, wg_count;    // the group's workgroups
+    int exchange_interval;     // 0: no exchanges
+    float J, delta;
+    void *state;               // int8 or float32 (R, L, L)
+    uint64_t *rng;             // R*L*L records
+    float *betas;              // (R) walker betas
+    const float *sigmas;       // (R) per temperature, MODEL_XY_GAUSS
+    unsigned long long *accepted;  // (R) per temperature, may be NULL
+    int *parity;
+    double *energies;          // (R)
+    int32_t *w2t, *t2w;
+    uint64_t *pt_rng;          // (R-1) records
+    long long *counts;         // (3, R-1)
+    long long *trips;          // (5, R)
+    double *sample_partials;   // [max_samples][R][S][kChannels]
+    int32_t *sample_w2t;       // [max_samples][R]
+    // workspace owned by the pack
+    uint64_t *halo;            // [R][S][2 (row 2, row H)][2 (slot)][L] words
+    uint64_t *ghost_rng;       // [R][S][L] records
+    void *ghost_state;         // [R][S][L]
+    long long *energy_acc;     // [2][R] running integer energy totals
+    unsigned *arrive;          // workgroups arrived at this launch's exchanges
+};
+
+struct PackWork {
+    int group, task_first, task_count, pad;
+};
+
+struct PackArgs {
+    const PackGroup *groups;
+    const PackWork *work;
+    int ngroups;
+    int nsweeps;
+    long long *clock;          // [0] sweeps done, [1] sweeps done before this launch
+    int sample_interval;
+    long long sample_first;
+    int max_samples;
+    int *error;
+};
+
+// Dynamic LDS of a workgroup of a group: K strips, then the group's PT state
+struct PackLds {
+    size_t stride, beta, w2t, t2w, thr, sig, moves, e, g, prev, total;
+};
+
+__host__ __device__ inline PackLds pack_lds(int model, int L, int R, int H, int K) {
+    PackLds o;
+    o.stride = (strip_bytes(model, L, H) + 15) & ~(size_t)15;
+    size_t p = (size_t)K * o.stride;
+    o.beta = p;
+    p += 4 * (size_t)R;
+    o.w2t = p;
+    p += 4 * (size_t)R;
+    o.t2w = p;
+    p += 4 * (size_t)R;
+    o.thr = p;
+    p += 4 * 5 * (size_t)K;
+    o.sig = p;
+    p += 4 * (size_t)K;
+    o.moves = p;
+    p += 4 * (size_t)R;
+    p = (p + 7) & ~(size_t)7;
+    o.e = p;
+    p += 8 * (size_t)R;
+    o.g = p;
+    p += 16 * (size_t)(R > 1 ? R - 1 : 1);
+    o.prev = p;
+    p += 16 * (size_t)R;
+    o.total = p;
+    return o;
+}
+
+template <int MODEL, int NT>
+__device__ void pack_run(const PackArgs &a, const PackGroup &g, const PackWork &w, unsigned char *smem,
+                         long long *red_e, double (*red)[NT / 16]) {
+    const int L = g.L, H = g.H, S = g.S, R = g.R, K = g.K;
+    const int HH = H >> 1, half = L >> 1;
+    const int tid = threadIdx.x;
+    const int lw = (int)blockIdx.x - g.wg_first;  // workgroup index within the group
+    const bool use_pt = g.exchange_interval > 0;
+    const float J = g.J, delta = g.delta;
+
+    const PackLds lay = pack_lds(MODEL, L, R, H, K);
+    float *pt_beta = reinterpret_cast<float *>(smem + lay.beta);
+    int32_t *pt_w2t = reinterpret_cast<int32_t *>(smem + lay.w2t);
+    int32_t *pt_t2w = reinterpret_cast<int32_t *>(smem + lay.t2w);
+    float *thr = reinterpret_cast<float *>(smem + lay.thr);      // [K][5]
+    float *sig = reinterpret_cast<float *>(smem + lay.sig);      // [K]
+    unsigned *moves = reinterpret_cast<unsigned *>(smem + lay.moves);  // [R] accepted since the last flush
+    double *pt_e = reinterpret_cast<double *>(smem + lay.e);
+    uint64_t *pt_g = reinterpret_cast<uint64_t *>(smem + lay.g);
+    long long *prev = reinterpret_cast<long long *>(smem + lay.prev);  // [2][R]
+
+    for (int k = tid; k < R; k += NT) {
+        pt_beta[k] = g.betas[k];
+        pt_w2t[k] = use_pt ? g.w2t[k] : k;
+        pt_t2w[k] = use_pt ? g.t2w[k] : k;
+        moves[k] = 0;
+        prev[k] = 0;
+        prev[R + k] = 0;
+    }
+    if (use_pt)
+        for (int k = tid; k < 2 * (R - 1); k += NT) pt_g[k] = g.pt_rng[k];
+    int parity = use_pt ? *g.parity : 0;
+    const long long sweep0 = a.clock[1];
+    bool exchanged = false;
+    int q = 0;  // exchanges of this launch
+    __syncthreads();
+
+    // pass map: colour rows of the K strips spread over the threads
+    const int rows_all = NT / half, rpp = rows_all / K;
+    const int tcol = tid % half, trow = tid / half;
+    const int pk = trow / rpp, prow = trow - pk * rpp;
+    const bool pass_on = pk < K;
+    // row map of loads, stores, halo rows: whole rows of L sites
+    const int lj = tid % L, lrow = tid / L, lrows = NT / L;
+    // observables map: NT / K threads per replica
+    const int TPR = NT / K, ok = tid / TPR, ot = tid - ok * TPR;
+    const int W = TPR < 64 ? TPR : 64;
+
+    auto view_of = [&](int k) { return strip_view(smem + (size_t)k * lay.stride, L, H); };
+    auto halo_ptr = [&](int r, int ss, int which, int slot) {
+        return g.halo + ((((size_t)r * S + ss) * 2 + which) * 2 + slot) * L;
+    };
+    auto flush_moves = [&]() {
+        __syncthreads();
+        if (g.accepted)
+            for (int k = tid; k < R; k += NT)
+                if (moves[k]) {
+                    atomicAdd(&g.accepted[pt_w2t[k]], (unsigned long long)moves[k]);
+                    moves[k] = 0;
+                }
+        __syncthreads();
+    };
+
+    bool failed = false;
+    for (int n = 0; n < a.nsweeps && !failed; n++) {
+        const long long sweep_global = sweep0 + n + 1;
+        const bool do_exchange = use_pt && (sweep_global % g.exchange_interval == 0);
+        const bool do_sample = a.sample_interval > 0 && sweep_global >= a.sample_first &&
+                               ((sweep_global - a.sample_first) % a.sample_interval == 0);
+        const long long sidx = do_sample ? (sweep_global - a.sample_first) / a.sample_interval : 0;
+
+        for (int ti = 0; ti < w.task_count; ti++) {
+            const int t = w.task_first + ti;
+            const int b = t / S, s = t - (t / S) * S;
+            const int r0 = b * K;
+            const int row0 = s * H;
+            const int s_up = (s + S - 1) % S, s_dn = (s + 1) % S;
+            if (tid < K) {
+                const int r = r0 + tid;
+                if (MODEL == MODEL_ISING)
+                    for (int m = 0; m < 5; m++) thr[tid * 5 + m] = ising_threshold(4 * m - 8, pt_beta[r], J);
+                if (MODEL == MODEL_XY_GAUSS) sig[tid] = g.sigmas[pt_w2t[r]];
+            }
+            // ---- load rows 1..H (lattice) and H+1 (ghost buffer)
+            if (lrow < lrows)
+                for (int qr = lrow; qr < K * (H + 1); qr += lrows) {
+                    const int k = qr / (H + 1), lr = 1 + qr - k * (H + 1);
+                    const int r = r0 + k;
+                    const StripView v = view_of(k);
+                    size_t gk;
+                    const uint64_t *rsrc;
+                    const void *ssrc;
+                    if (lr <= H) {
+                        gk = (size_t)r * L * L + (size_t)(row0 + lr - 1) * L + lj;
+                        rsrc = g.rng;
+                        ssrc = g.state;
+                    } else {
+                        gk = ((size_t)r * S + s) * L + lj;
+                        rsrc = g.ghost_rng;
+                        ssrc = g.ghost_state;
+                    }
+                    v.rng[lds_index(lr - 1, lj, L)] = reinterpret_cast<const ulonglong2 *>(rsrc)[gk];
+                    if (MODEL == MODEL_ISING) {
+                        v.sp[lds_index(lr, lj, L)] = reinterpret_cast<const int8_t *>(ssrc)[gk];
+                    } else {
+                        const float th = reinterpret_cast<const float *>(ssrc)[gk];
+                        float sv, cv;
+                        fast_sincosf(th, sv, cv);
+                        v.c[lds_index(lr, lj, L)] = cv;
+                        v.s[lds_index(lr, lj, L)] = sv;
+                        v.th[lds_index(lr - 1, lj, L)] = th;
+                    }
+                }
+            __syncthreads();
+
+            // ---- the five passes (the schedule of the resident engine)
+            const StripView pv = view_of(pass_on ? pk : 0);
+            const float pbeta = pass_on ? pt_beta[r0 + pk] : 0.0f;
+            const float psig = MODEL == MODEL_XY_GAUSS && pass_on ? sig[pk] : 0.0f;
+            const float *pthr = thr + 5 * (pass_on ? pk : 0);
+            int n_acc = 0;
+            auto pass = [&](const Seg g0, const Seg g1, const Seg g2, const Seg g3) {
+                if (!pass_on) return;
+                const int n0 = max(g0.hi - g0.lo + 1, 0), n1 = max(g1.hi - g1.lo + 1, 0);
+                const int n2 = max(g2.hi - g2.lo + 1, 0), n3 = max(g3.hi - g3.lo + 1, 0);
+                const int total = n0 + n1 + n2 + n3;
+                for (int qq = prow; qq < total; qq += rpp) {
+                    int k = qq;
+                    Seg sg = g0;
+                    if (k >= n0) {
+                        k -= n0;
+                        sg = g1;
+                        if (k >= n1) {
+                            k -= n1;
+                            sg = g2;
+                            if (k >= n2) {
+                                k -= n2;
+                                sg = g3;
+                            }
+                        }
+                    }
+                    const int ri = sg.lo + k;
+                    const int counted = !((sg.c < 2) & (ri == HH));  // ghost moves are the owner's
+                    if (site_update<MODEL>(pv, sg.c, ri, tcol, L, pbeta, psig, J, delta, pthr)) n_acc += counted;
+                }
+            };
+            const Seg none{0, 0, -1};
+            pass(Seg{0, 1, HH - 1}, none, none, none);
+            __syncthreads();
+            pass(Seg{1, 1, HH - 1}, none, none, none);
+            // receive rows 0 and H+2 as of the start of sweep n (rows H and 2
+            // of this strip itself when S == 1; they are not updated yet)
+            {
+                int bad = 0;
+                if (lrow < lrows)
+                    for (int qr = lrow; qr < 2 * K; qr += lrows) {
+                        const int k = qr >> 1, top = !(qr & 1);
+                        const StripView v = view_of(k);
+                        const int dst_row = top ? 0 : H + 2;
+                        if (S == 1) {
+                            const int src_row = top ? H : 2;
+                            if (MODEL == MODEL_ISING) {
+                                v.sp[lds_index(dst_row, lj, L)] = v.sp[lds_index(src_row, lj, L)];
+                            } else {
+                                v.c[lds_index(dst_row, lj, L)] = v.c[lds_index(src_row, lj, L)];
+                                v.s[lds_index(dst_row, lj, L)] = v.s[lds_index(src_row, lj, L)];
+                            }
+                            continue;
+                        }
+                        const uint64_t *src = top ? halo_ptr(r0 + k, s_up, 1, n & 1) : halo_ptr(r0 + k, s_dn, 0, n & 1);
+                        uint32_t payload;
+                        if (!poll_word(src + lj, (unsigned)n + 1u, payload, a.error)) {
+                            bad = 1;
+                            break;
+                        }
+                        if (MODEL == MODEL_ISING) {
+                            v.sp[lds_index(dst_row, lj, L)] = (int8_t)(uint8_t)payload;
+                        } else {
+                            float sv, cv;
+                            fast_sincosf(__uint_as_float(payload), sv, cv);
+                            v.c[lds_index(dst_row, lj, L)] = cv;
+                            v.s[lds_index(dst_row, lj, L)] = sv;
+                        }
+                    }
+                if (__syncthreads_or(bad)) {
+                    if (tid == 0) atomicOr(a.error, BATCH_ERR_WATCHDOG);
+                    failed = true;
+                    break;
+                }
+            }
+            pass(Seg{0, 0, 0}, Seg{0, HH, HH}, Seg{2, 1, HH - 2}, none);
+            __syncthreads();
+            pass(Seg{1, 0, 0}, Seg{1, HH, HH}, Seg{2, 0, 0}, Seg{2, HH - 1, HH - 1});
+            __syncthreads();
+            pass(Seg{3, 0, HH - 1}, none, none, none);
+            if (pass_on && n_acc) atomicAdd(&moves[r0 + pk], (unsigned)n_acc);
+            __syncthreads();
+
+            // ---- publish rows 2 and H (tag n + 2, slot (n + 1) & 1); write back
+            if (lrow < lrows) {
+                if (S > 1)
+                    for (int qr = lrow; qr < 2 * K; qr += lrows) {
+                        const int k = qr >> 1, which = qr & 1;
+                        const StripView v = view_of(k);
+                        const int lr = which ? H : 2;
+                        const uint32_t payload = MODEL == MODEL_ISING
+                                                     ? (uint32_t)(uint8_t)v.sp[lds_index(lr, lj, L)]
+                                                     : __float_as_uint(v.th[lds_index(lr - 1, lj, L)]);
+                        publish_word(halo_ptr(r0 + k, s, which, (n + 1) & 1) + lj, (unsigned)n + 2u, payload);
+                    }
+                for (int qr = lrow; qr < K * (H + 1); qr += lrows) {
+                    const int k = qr / (H + 1), lr = 1 + qr - k * (H + 1);
+                    const int r = r0 + k;
+                    const StripView v = view_of(k);
+                    size_t gk;
+                    uint64_t *rdst;
+                    void *sdst;
+                    if (lr <= H) {
+                        gk = (size_t)r * L * L + (size_t)(row0 + lr - 1) * L + lj;
+                        rdst = g.rng;
+                        sdst = g.state;
+                    } else {
+                        gk = ((size_t)r * S + s) * L + lj;
+                        rdst = g.ghost_rng;
+                        sdst = g.ghost_state;
+                    }
+                    reinterpret_cast<ulonglong2 *>(rdst)[gk] = v.rng[lds_index(lr - 1, lj, L)];
+                    if (MODEL == MODEL_ISING)
+                        reinterpret_cast<int8_t *>(sdst)[gk] = v.sp[lds_index(lr, lj, L)];
+                    else
+                        reinterpret_cast<float *>(sdst)[gk] = v.th[lds_index(lr - 1, lj, L)];
+                }
+            }
+
+            // ---- strip observables per replica (segmented reduction over the
+            // replica's NT / K threads)
+            if (do_exchange || do_sample) {
+                long long ef = 0;
+                double m[4] = {0.0, 0.0, 0.0, 0.0};
+                const StripView v = view_of(ok);
+                const int prl = TPR >= L ? TPR / L : 1;
+                const int c0 = TPR >= L ? ot % L : ot, rr0 = TPR >= L ? ot / L : 0;
+                const int cstep = TPR >= L ? L : TPR;
+                if (rr0 < prl)
+                    for (int lr = 1 + rr0; lr <= H; lr += prl)
+                        for (int j = c0; j < L; j += cstep) site_observables<MODEL>(v, lr, j, L, do_sample, ef, m);
+                for (int off = W >> 1; off > 0; off >>= 1) {
+                    ef += __shfl_down(ef, off, W);
+#pragma unroll
+                    for (int c = 0; c < 4; c++) m[c] += __shfl_down(m[c], off, W);
+                }
+                if (ot % W == 0) {
+                    const int slot = tid / W;
+                    red_e[slot] = ef;
+#pragma unroll
+                    for (int c = 0; c < 4; c++) red[c][slot] = m[c];
+                }
+                __syncthreads();
+                if (tid < K) {
+                    const int per = TPR / W, first = tid * per;
+                    long long E = red_e[first];
+                    double M[4] = {red[0][first], red[1][first], red[2][first], red[3][first]};
+                    for (int x = 1; x < per; x++) {
+                        E += red_e[first + x];
+#pragma unroll
+                        for (int c = 0; c < 4; c++) M[c] += red[c][first + x];
+                    }
+                    const int r = r0 + tid;
+                    if (do_exchange)
+                        __hip_atomic_fetch_add(g.energy_acc + (size_t)(q & 1) * R + r, E, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                    if (do_sample && sidx < a.max_samples) {
+                        double *dst = g.sample_partials + (((size_t)sidx * R + r) * S + s) * kChannels;
+                        dst[0] = energy_from_fixed(MODEL, J, E);
+#pragma unroll
+                        for (int c = 0; c < 4; c++) dst[c + 1] = M[c];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (failed) break;
+
+        if (do_exchange) {
+            flush_moves();  // moves so far belong to the current temperatures
+            // make this workgroup's energy adds visible before it arrives
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __syncthreads();
+            int bad = 0;
+            if (tid == 0) {
+                __hip_atomic_fetch_add(g.arrive, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned target = (unsigned)(q + 1) * (unsigned)g.wg_count;
+                unsigned spins = 0;
+                while (__hip_atomic_load(g.arrive, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if ((++spins & 1023u) == 0 &&
+                        (spins > kWatchdog ||
+                         __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                        bad = 1;
+                        break;
+                    }
+                }
+            }
+            if (__syncthreads_or(bad)) {
+                if (tid == 0) atomicOr(a.error, BATCH_ERR_WATCHDOG);
+                failed = true;
+                break;
+            }
+            for (int k = tid; k < R; k += NT) {
+                long long *tot = g.energy_acc + (size_t)(q & 1) * R + k;
+                const long long now = __hip_atomic_load(tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const long long e = now - prev[(q & 1) * R + k];
+                prev[(q & 1) * R + k] = now;
+                pt_e[k] = energy_from_fixed(MODEL, J, e);
+            }
+            __syncthreads();
+            const int P = R - 1;
+            for (int left = parity + 2 * tid; left < R - 1; left += 2 * NT) {
+                const int right = left + 1;
+                const int wa = pt_t2w[left], wb = pt_t2w[right];
+                const double ea = pt_e[wa], eb = pt_e[wb];
+                const bool finite = isfinite(ea) && isfinite(eb);
+                bool accept = false;
+                if (finite) {
+                    const double lr = ((double)pt_beta[wa] - (double)pt_beta[wb]) * (ea - eb);
+                    Rng gg;
+                    gg.s0 = pt_g[2 * left];
+                    gg.s1 = pt_g[2 * left + 1];
+                    const double u = gg.uniform_f64();
+                    pt_g[2 * left] = gg.s0;
+                    pt_g[2 * left + 1] = gg.s1;
+                    accept = lr >= 0.0 || u == 0.0 || ::log(u) < lr;
+                }
+                if (accept) {
+                    pt_t2w[left] = wb;
+                    pt_t2w[right] = wa;
+                    pt_w2t[wa] = right;
+                    pt_w2t[wb] = left;
+                    const float tb = pt_beta[wa];
+                    pt_beta[wa] = pt_beta[wb];
+                    pt_beta[wb] = tb;
+                }
+                if (left % g.wg_count == lw) {
+                    g.counts[left] += 1;
+                    if (!finite) g.counts[2 * P + left] += 1;
+                    if (accept) g.counts[P + left] += 1;
+                }
+            }
+            __syncthreads();
+            for (int wk = lw + tid * g.wg_count; wk < R; wk += NT * g.wg_count) {
+                const int tt = pt_w2t[wk];
+                if (tt != 0 && tt != R - 1) continue;
+                const long long origin = g.trips[wk];
+                if (origin < 0) {
+                    g.trips[wk] = tt;
+                    g.trips[2 * R + wk] = sweep_global;
+                } else if (tt != origin) {
+                    g.trips[R + wk] = 1;
+                } else if (g.trips[R + wk]) {
+                    g.trips[3 * R + wk] += 1;
+                    g.trips[4 * R + wk] += sweep_global - g.trips[2 * R + wk];
+                    g.trips[2 * R + wk] = sweep_global;
+                    g.trips[R + wk] = 0;
+                }
+            }
+            if (R > 2) parity = 1 - parity;
+            exchanged = true;
+            q++;
+            __syncthreads();
+        }
+        if (do_sample && lw == 0 && sidx < a.max_samples)
+            for (int k = tid; k < R; k += NT) g.sample_w2t[(size_t)sidx * R + k] = pt_w2t[k];
+    }
+
+    flush_moves();
+    if (lw == 0 && use_pt) {
+        for (int k = tid; k < R; k += NT) {
+            g.betas[k] = pt_beta[k];
+            g.t2w[k] = pt_t2w[k];
+            g.w2t[k] = pt_w2t[k];
+            if (exchanged) g.energies[k] = pt_e[k];
+        }
+        for (int k = tid; k < 2 * (R - 1); k += NT) g.pt_rng[k] = pt_g[k];
+        if (tid == 0) *g.parity = parity;
+    }
+    if (blockIdx.x == 0 && tid == 0) a.clock[0] = sweep0 + a.nsweeps;
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void pack_kernel(PackArgs a) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    __shared__ long long red_e[NT / 16];
+    __shared__ double red[4][NT / 16];
+    const PackWork w = a.work[blockIdx.x];
+    const PackGroup g = a.groups[w.group];
+    if (g.model == MODEL_ISING)
+        pack_run<MODEL_ISING, NT>(a, g, w, smem, red_e, red);
+    else if (g.model == MODEL_XY_FAST)
+        pack_run<MODEL_XY_FAST, NT>(a, g, w, smem, red_e, red);
+    else
+        pack_run<MODEL_XY_GAUSS, NT>(a, g, w, smem, red_e, red);
+}
+
+// Before each launch: ghost rows from the lattice, halo words of sweep 0
+// (tag 1, slot 0; slot 1 cleared), exchange totals and arrival counters
+// cleared, the launch's base sweep, the error word.  blockIdx.y = group.
+__global__ void pack_prologue_kernel(PackArgs a) {
+    const PackGroup g = a.groups[blockIdx.y];
+    const int L = g.L, S = g.S, H = g.H, R = g.R;
+    const size_t n = (size_t)R * S * L;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        const int r = (int)(k / ((size_t)S * L));
+        const int rem = (int)(k - (size_t)r * S * L);
+        const int s = rem / L, j = rem - (rem / L) * L;
+        const size_t rep = (size_t)r * L * L;
+        const size_t gk = rep + (size_t)(((s + 1) % S) * H) * L + j;
+        reinterpret_cast<ulonglong2 *>(g.ghost_rng)[k] = reinterpret_cast<const ulonglong2 *>(g.rng)[gk];
+        const size_t k2 = rep + (size_t)(s * H + 1) * L + j, kH = rep + (size_t)(s * H + H - 1) * L + j;
+        uint32_t p2, pH;
+        if (g.model == MODEL_ISING) {
+            const int8_t *st = reinterpret_cast<const int8_t *>(g.state);
+            reinterpret_cast<int8_t *>(g.ghost_state)[k] = st[gk];
+            p2 = (uint32_t)(uint8_t)st[k2];
+            pH = (uint32_t)(uint8_t)st[kH];
+        } else {
+            const float *st = reinterpret_cast<const float *>(g.state);
+            reinterpret_cast<float *>(g.ghost_state)[k] = st[gk];
+            p2 = __float_as_uint(st[k2]);
+            pH = __float_as_uint(st[kH]);
+        }
+        uint64_t *h = g.halo + (((size_t)r * S + s) * 4) * L + j;  // [which][slot][L]
+        h[0] = ((uint64_t)1 << 32) | p2;  // row 2, slot 0
+        h[L] = 0;                          // row 2, slot 1
+        h[2 * L] = ((uint64_t)1 << 32) | pH;
+        h[3 * L] = 0;
+    }
+    if (blockIdx.x == 0) {
+        for (int k = threadIdx.x; k < 2 * R; k += blockDim.x) g.energy_acc[k] = 0;
+        if (threadIdx.x == 0) *g.arrive = 0;
+        if (blockIdx.y == 0 && threadIdx.x == 0) {
+            *a.error = 0;
+            a.clock[1] = a.clock[0];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ host side
+
+struct GroupHost {
+    PackGroup g;
+    void *halo, *ghost_rng, *ghost_state, *acc;
+};
+
+struct Pack {
+    int threads = 256;
+    std::vector<GroupHost> groups;
+    std::vector<PackWork> work;
+    PackGroup *d_groups = nullptr;
+    PackWork *d_work = nullptr;
+    long long *d_clock = nullptr;
+    int *d_error = nullptr;
+    size_t lds = 0;
+    bool dirty = true;  // the device group table needs an upload
+};
+
+static const void *pack_kernel_ptr(int threads) {
+    return threads == 512 ? reinterpret_cast<const void *>(&pack_kernel<512>)
+                          : reinterpret_cast<const void *>(&pack_kernel<256>);
+}
+
+static int pack_check_group(int threads, int model, int L, int R, int H, int K) {
+    MC_REQUIRE(model == MODEL_ISING || model == MODEL_XY_FAST || model == MODEL_XY_GAUSS,
+               "unknown batch model %d", model);
+    MC_REQUIRE(L >= 4 && L % 2 == 0 && L <= threads, "packed L must be even, >= 4 and <= %d, got %d", threads,
+               L);
+    MC_REQUIRE(H >= 4 && H % 2 == 0 && L % H == 0, "strip height %d must be even, >= 4 and divide L=%d", H, L);
+    MC_REQUIRE(K >= 1 && K <= threads / 16 && (K & (K - 1)) == 0 && R % K == 0,
+               "replicas per task K=%d must be a power of two <= %d dividing R=%d", K, threads / 16, R);
+    MC_REQUIRE((threads / (L / 2)) / K >= 1, "K=%d strips of L=%d do not fit a %d-thread pass", K, L, threads);
+    MC_REQUIRE((long long)L * L <= (1LL << 20), "at most 2^20 sites per replica");
+    return MC_OK;
+}
+
+static void pack_free(Pack *p) {
+    for (auto &gh : p->groups) {
+        (void)hipFree(gh.halo);
+        (void)hipFree(gh.ghost_rng);
+        (void)hipFree(gh.ghost_state);
+        (void)hipFree(gh.acc);
+    }
+    p->groups.clear();
+    (void)hipFree(p->d_groups);
+    (void)hipFree(p->d_work);
+    (void)hipFree(p->d_clock);
+    (void)hipFree(p->d_error);
+}
+
+static int pack_upload(Pack *p) {
+    if (!p->dirty) return MC_OK;
+    std::vector<PackGroup> tab;
+    for (auto &gh : p->groups) tab.push_back(gh.g);
+    MC_TRY_HIP(hipMemcpy(p->d_groups, tab.data(), tab.size() * sizeof(PackGroup), hipMemcpyHostToDevice));
+    p->dirty = false;
+    return MC_OK;
+}
+
+static int pack_launch(Pack *p, hipStream_t stream, int nsweeps, int sample_interval, long long sample_first,
+                       int max_samples) {
+    PackArgs a;
+    a.groups = p->d_groups;
+    a.work = p->d_work;
+    a.ngroups = (int)p->groups.size();
+    a.nsweeps = nsweeps;
+    a.clock = p->d_clock;
+    a.sample_interval = sample_interval;
+    a.sample_first = sample_first;
+    a.max_samples = max_samples;
+    a.error = p->d_error;
+    size_t most = 0;
+    for (auto &gh : p->groups) most = std::max(most, (size_t)gh.g.R * gh.g.S * gh.g.L);
+    const unsigned bx = (unsigned)std::min<size_t>((most + 255) / 256, 256);
+    hipLaunchKernelGGL(pack_prologue_kernel, dim3(bx, (unsigned)p->groups.size()), dim3(256), 0, stream, a);
+    MC_TRY_HIP(hipGetLastError());
+    if (p->threads == 512)
+        hipLaunchKernelGGL(pack_kernel<512>, dim3((unsigned)p->work.size()), dim3(512), p->lds, stream, a);
+    else
+        hipLaunchKernelGGL(pack_kernel<256>, dim3((unsigned)p->work.size()), dim3(256), p->lds, stream, a);
+    MC_TRY_HIP(hipGetLastError());
+    return MC_OK;
+}
+
+static int pack_prepare(Pack *p, int nsweeps, int sample_interval, int max_samples, double *const *sample_partials,
+                        int32_t *const *sample_w2t) {
+    MC_REQUIRE(!p->groups.empty() && !p->work.empty(), "the pack has no groups or no work table");
+    MC_REQUIRE(nsweeps >= 0 && sample_interval >= 0 && max_samples >= 0, "invalid sweep or sampling arguments");
+    int dev = 0, cus = 0, per_cu = 0;
+    MC_TRY_HIP(hipGetDevice(&dev));
+    MC_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (p->lds > 64 * 1024)
+        MC_TRY_HIP(hipFuncSetAttribute(pack_kernel_ptr(p->threads), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)p->lds));
+    MC_TRY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pack_kernel_ptr(p->threads), p->threads,
+                                                            p->lds));
+    MC_REQUIRE((long long)p->work.size() <= (long long)per_cu * cus,
+               "%zu workgroups cannot be co-resident (%d per CU x %d CUs at %zu bytes of LDS)", p->work.size(),
+               per_cu, cus, p->lds);
+    for (size_t i = 0; i < p->groups.size(); i++) {
+        PackGroup &g = p->groups[i].g;
+        double *sp = sample_partials ? sample_partials[i] : nullptr;
+        int32_t *sw = sample_w2t ? sample_w2t[i] : nullptr;
+        MC_REQUIRE(max_samples == 0 || (sp && sw), "group %zu needs sample buffers", i);
+        if (g.sample_partials != sp || g.sample_w2t != sw) {
+            g.sample_partials = sp;
+            g.sample_w2t = sw;
+            p->dirty = true;
+        }
+    }
+    return pack_upload(p);
+}
+
+}  // namespace mc
+
+using namespace mc;
+
+extern "C" {
+
+int mc_pack_lds_bytes(int model, int L, int R, int H, int K, size_t *bytes) {
+    int rc = pack_check_group(512, model, L, R, H, K);
+    if (rc) return rc;
+    *bytes = pack_lds(model, L, R, H, K).total;
+    return MC_OK;
+}
+
+int mc_pack_occupancy(int threads, size_t lds_bytes, int *max_workgroups) {
+    MC_REQUIRE(threads == 256 || threads == 512, "workgroup size must be 256 or 512, got %d", threads);
+    if (lds_bytes > 160 * 1024) {
+        *max_workgroups = 0;
+        return MC_OK;
+    }
+    if (lds_bytes > 64 * 1024)
+        MC_TRY_HIP(hipFuncSetAttribute(pack_kernel_ptr(threads), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds_bytes));
+    int per_cu = 0, dev = 0, cus = 0;
+    MC_TRY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pack_kernel_ptr(threads), threads, lds_bytes));
+    MC_TRY_HIP(hipGetDevice(&dev));
+    MC_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    *max_workgroups = per_cu * cus;
+    return MC_OK;
+}
+
+int mc_pack_create(int threads, void **pack) {
+    MC_REQUIRE(threads == 256 || threads == 512, "workgroup size must be 256 or 512, got %d", threads);
+    Pack *p = new Pack();
+    p->threads = threads;
+    if (hipMalloc(&p->d_clock, 2 * sizeof(long long)) != hipSuccess ||
+        hipMalloc(&p->d_error, sizeof(int)) != hipSuccess ||
+        hipMemset(p->d_clock, 0, 2 * sizeof(long long)) != hipSuccess ||
+        hipMemset(p->d_error, 0, sizeof(int)) != hipSuccess) {
+        pack_free(p);
+        delete p;
+        mc::set_error(MC_ERR_HIP, "mc_pack_create: device allocation failed");
+        return MC_ERR_HIP;
+    }
+    *pack = p;
+    return MC_OK;
+}
+
+int mc_pack_destroy(void *pack) {
+    if (!pack) return MC_OK;
+    Pack *p = static_cast<Pack *>(pack);
+    (void)hipDeviceSynchronize();
+    pack_free(p);
+    delete p;
+    return MC_OK;
+}
+
+int mc_pack_add_group(void *pack, int model, int L, int R, int H, int K, int exchange_interval, float J,
+                      float delta, void *state, uint64_t *rng, float *betas, const float *sigmas,
+                      unsigned long long *accepted, int *parity, double *energies, int32_t *w2t, int32_t *t2w,
+                      uint64_t *pt_rng, long long *counts, long long *trips, int *group_index) {
+    Pack *p = static_cast<Pack *>(pack);
+    MC_REQUIRE(p, "null pack");
+    MC_REQUIRE(p->work.empty(), "groups must be added before the work table is set");
+    int rc = pack_check_group(p->threads, model, L, R, H, K);
+    if (rc) return rc;
+    MC_REQUIRE(exchange_interval >= 0, "exchange_interval must be nonnegative");
+    MC_REQUIRE(exchange_interval == 0 || R >= 2, "exchanges need at least two replicas");
+    MC_REQUIRE(model != MODEL_XY_GAUSS || sigmas, "the Gaussian XY model needs per-temperature sigmas");
+    MC_REQUIRE(state && rng && betas, "state, rng and betas are required");
+    MC_REQUIRE(exchange_interval == 0 || (parity && energies && w2t && t2w && pt_rng && counts && trips),
+               "exchanges need the PT arrays");
+    GroupHost gh{};
+    PackGroup &g = gh.g;
+    g.model = model; g.L = L; g.R = R; g.H = H; g.S = L / H; g.K = K;
+    g.ntasks = (R / K) * g.S;
+    g.exchange_interval = exchange_interval; g.J = J; g.delta = delta;
+    g.state = state; g.rng = rng; g.betas = betas; g.sigmas = sigmas; g.accepted = accepted;
+    g.parity = parity; g.energies = energies; g.w2t = w2t; g.t2w = t2w; g.pt_rng = pt_rng;
+    g.counts = counts; g.trips = trips;
+    const size_t rows = (size_t)R * g.S * L;
+    const size_t sb = model == MODEL_ISING ? 1 : 4;
+    if (hipMalloc(&gh.halo, rows * 4 * sizeof(uint64_t)) != hipSuccess ||
+        hipMalloc(&gh.ghost_rng, rows * 16) != hipSuccess || hipMalloc(&gh.ghost_state, rows * sb) != hipSuccess ||
+        hipMalloc(&gh.acc, 2 * (size_t)R * sizeof(long long) + 64) != hipSuccess) {
+        (void)hipFree(gh.halo);
+        (void)hipFree(gh.ghost_rng);
+        (void)hipFree(gh.ghost_state);
+        (void)hipFree(gh.acc);
+        mc::set_error(MC_ERR_HIP, "mc_pack_add_group: device allocation failed");
+        return MC_ERR_HIP;
+    }
+    g.halo = static_cast<uint64_t *>(gh.halo);
+    g.ghost_rng = static_cast<uint64_t *>(gh.ghost_rng);
+    g.ghost_state = gh.ghost_state;
+    g.energy_acc = static_cast<long long *>(gh.acc);
+    g.arrive = reinterpret_cast<unsigned *>(static_cast<char *>(gh.acc) + 2 * (size_t)R * sizeof(long long));
+    *group_index = (int)p->groups.size();
+    p->groups.push_back(gh);
+    p->lds = std::max(p->lds, pack_lds(model, L, R, H, K).total);
+    p->dirty = true;
+    return MC_OK;
+}
+
+// work: nworkgroups rows of (group, first task, task count).  Every task of
+// every group must appear exactly once, and a group's workgroups must be
+// consecutive.
+int mc_pack_set_work(void *pack, const int *work, int nworkgroups) {
+    Pack *p = static_cast<Pack *>(pack);
+    MC_REQUIRE(p && !p->groups.empty(), "add groups first");
+    MC_REQUIRE(p->work.empty(), "the work table is already set");
+    MC_REQUIRE(nworkgroups > 0, "need at least one workgroup");
+    const int G = (int)p->groups.size();
+    std::vector<int> first(G, -1), count(G, 0), next(G, 0);
+    std::vector<PackWork> tab(nworkgroups);
+    for (int i = 0; i < nworkgroups; i++) {
+        const int gi = work[3 * i], t0 = work[3 * i + 1], n = work[3 * i + 2];
+        MC_REQUIRE(gi >= 0 && gi < G, "work row %d names group %d", i, gi);
+        MC_REQUIRE(n >= 1, "work row %d has no tasks", i);
+        if (first[gi] < 0) first[gi] = i;
+        MC_REQUIRE(first[gi] + count[gi] == i, "the workgroups of group %d are not consecutive", gi);
+        MC_REQUIRE(t0 == next[gi], "work row %d: group %d's tasks must be listed in order", i, gi);
+        next[gi] += n;
+        count[gi]++;
+        tab[i] = PackWork{gi, t0, n, 0};
+    }
+    for (int gi = 0; gi < G; gi++) {
+        PackGroup &g = p->groups[gi].g;
+        MC_REQUIRE(next[gi] == g.ntasks, "group %d: %d of %d tasks assigned", gi, next[gi], g.ntasks);
+        g.wg_first = first[gi];
+        g.wg_count = count[gi];
+    }
+    MC_TRY_HIP(hipMalloc(&p->d_work, tab.size() * sizeof(PackWork)));
+    MC_TRY_HIP(hipMemcpy(p->d_work, tab.data(), tab.size() * sizeof(PackWork), hipMemcpyHostToDevice));
+    MC_TRY_HIP(hipMalloc(&p->d_groups, p->groups.size() * sizeof(PackGroup)));
+    p->work = tab;
+    p->dirty = true;
+    return MC_OK;
+}
+
+int mc_pack_set_sweeps(void *pack, long long sweeps) {
+    Pack *p = static_cast<Pack *>(pack);
+    MC_REQUIRE(p, "null pack");
+    const long long c[2] = {sweeps, sweeps};
+    MC_TRY_HIP(hipMemcpy(p->d_clock, c, sizeof(c), hipMemcpyHostToDevice));
+    return MC_OK;
+}
+
+int mc_pack_sweeps(void *pack, long long *sweeps) {
+    Pack *p = static_cast<Pack *>(pack);
+    MC_REQUIRE(p, "null pack");
+    MC_TRY_HIP(hipMemcpy(sweeps, p->d_clock, sizeof(long long), hipMemcpyDeviceToHost));
+    return MC_OK;
+}
+
+int mc_pack_error(void *pack, int *error) {
+    Pack *p = static_cast<Pack *>(pack);
+    MC_REQUIRE(p, "null pack");
+    MC_TRY_HIP(hipMemcpy(error, p->d_error, sizeof(int), hipMemcpyDeviceToHost));
+    return MC_OK;
+}
+
+int mc_pack_run(void *pack, void *stream, int nsweeps, int sample_interval, long long sample_first, int max_samples,
+                double *const *sample_partials, int32_t *const *sample_w2t) {
+    Pack *p = static_cast<Pack *>(pack);
+    MC_REQUIRE(p, "null pack");
+    int rc = pack_prepare(p, nsweeps, sample_interval, max_samples, sample_partials, sample_w2t);
+    if (rc) return rc;
+    if (nsweeps == 0) return MC_OK;
+    return pack_launch(p, as_stream(stream), nsweeps, sample_interval, sample_first, max_samples);
+}
+
+int mc_pack_capture(void *pack, void *stream, int nsweeps, int sample_interval, long long sample_first,
+                    int max_samples, double *const *sample_partials, int32_t *const *sample_w2t, void **graph) {
+    Pack *p = static_cast<Pack *>(pack);
+    MC_REQUIRE(p, "null pack");
+    MC_REQUIRE(stream, "graph capture needs a created stream");
+    MC_REQUIRE(nsweeps > 0, "a captured launch needs sweeps");
+    int rc = pack_prepare(p, nsweeps, sample_interval, max_samples, sample_partials, sample_w2t);
+    if (rc) return rc;
+    hipStream_t s = as_stream(stream);
+    MC_TRY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    rc = pack_launch(p, s, nsweeps, sample_interval, sample_first, max_samples);
+    hipGraph_t g = nullptr;
+    const hipError_t end = hipStreamEndCapture(s, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    MC_TRY_HIP(end);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t inst = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    MC_TRY_HIP(inst);
+    *graph = exec;
+    return MC_OK;
+}
+
+int mc_graph_launch(void *graph, void *stream) {
+    MC_REQUIRE(graph, "null graph");
+    MC_TRY_HIP(hipGraphLaunch(static_cast<hipGraphExec_t>(graph), as_stream(stream)));
+    return MC_OK;
+}
+
+int mc_graph_destroy(void *graph) {
+    if (graph) MC_TRY_HIP(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph)));
+    return MC_OK;
+}
+
+}  // extern "C"

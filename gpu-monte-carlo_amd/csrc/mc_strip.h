@@ -1,0 +1,255 @@
+// Strip machinery shared by the persistent engines (mc_batch.hip keeps every
+// strip in LDS for a whole launch; mc_pack.hip streams strips of several
+// lattice sizes through LDS every sweep).
+//
+// A strip is H full rows of one replica lattice (L x L).  Its LDS image holds
+//   local row 0      bottom row of the strip above (received)
+//   rows 1..H        own rows: state, RNG streams, cos/sin (or spins)
+//   row H+1          ghost: the top row of the strip below, with a copy of its
+//                    RNG streams, updated in lock-step with its owner
+//   row H+2          second row of the strip below (received)
+// with every row stored colour-split (even columns, then odd columns), so a
+// colour step reads and writes unit-stride.
+//
+// Energies are accumulated as integers: Ising bond sums exactly, XY bond
+// terms (float32, formed as the kernels form them) rounded to multiples of
+// 2^-40 and added in int64.  The sums are therefore independent of how sites
+// are spread over threads, strips and workgroups, and exchange decisions do
+// not depend on the launch shape.  |bond term| < 4 and at most 2^20 sites per
+// replica keep every sum below 2^62.
+#pragma once
+
+#include "mc_common.h"
+#include "mc_fastmath.h"
+
+namespace mc {
+
+enum { MODEL_ISING = 0, MODEL_XY_FAST = 1, MODEL_XY_GAUSS = 2 };
+enum { BATCH_ERR_WATCHDOG = 1 };
+
+constexpr unsigned kWatchdog = 1u << 20;  // about a second of polling
+constexpr int kChannels = 5;               // XY: e, mx, my, cx, qx ; Ising: e, m
+
+// LDS position of column j in LDS row `row` (colour-split layout)
+__device__ __forceinline__ int lds_index(int row, int j, int L) {
+    return row * L + (j & 1) * (L >> 1) + (j >> 1);
+}
+
+// One group of colour rows of a pass: colour c, colour rows lo..hi (empty if
+// hi < lo).  Colour rows of colours 0/1 are local rows 1, 3, ..., H+1
+// (ri = 0..H/2, ri = H/2 being the ghost row); of colours 2/3 local rows
+// 2, 4, ..., H (ri = 0..H/2-1).
+struct Seg {
+    int c, lo, hi;
+};
+
+// XY bond term x rounded to the nearest multiple of 2^-40 (ties to even), as
+// an integer: x * 2^40 is exact in float64, and adding 1.5 * 2^52 rounds it
+// to an integer held in the low mantissa bits (|x| < 2^11).
+__device__ __forceinline__ long long bond_fixed(float x) {
+    const double magic = 0x1.8p52;
+    const double t = (double)x * 0x1p40 + magic;
+    return __double_as_longlong(t) - __double_as_longlong(magic);
+}
+
+// Energy of an integer energy sum: Ising -J * sum s (s_down + s_right);
+// XY -J * 2^-40 * sum of rounded (cos bond x + cos bond y).
+__host__ __device__ inline double energy_from_fixed(int model, float J, long long e) {
+    return model == MODEL_ISING ? -(double)J * (double)e : -(double)J * ((double)e * 0x1p-40);
+}
+
+// Cross-workgroup words (halo and energy words) are written and read with
+// agent-scope relaxed atomics, which are device-coherent on their own (gfx950
+// sets the device-scope cache bits on them), so no L2 write-back or
+// invalidate is needed.  Each word is a single-copy atomic 64-bit value whose
+// high half names the sweep or exchange it belongs to, so a reader that sees
+// the expected tag also sees the payload written with it.
+//
+// One thread: poll a word until it carries tag `seq`.  False once the
+// watchdog expires or another workgroup has recorded an error.
+__device__ inline bool poll_word(const uint64_t *p, unsigned seq, uint32_t &payload, const int *error) {
+    unsigned n = 0;
+    for (;;) {
+        const uint64_t w = __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((unsigned)(w >> 32) == seq) {
+            payload = (uint32_t)w;
+            return true;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        if ((++n & 1023u) == 0 &&
+            (n > kWatchdog ||
+             __hip_atomic_load(const_cast<int *>(error), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+            return false;
+    }
+}
+
+__device__ __forceinline__ void publish_word(uint64_t *p, unsigned seq, uint32_t payload) {
+    __hip_atomic_store(p, ((uint64_t)seq << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Ising acceptance threshold for 2 s sum_nn = v: always flip when dE <= 0,
+// else u < exp(-beta dE) in float32 through the float64 exponential.
+__device__ __forceinline__ float ising_threshold(int v, float beta, float J) {
+    const float dE = (float)v * J;
+    if (dE <= 0.0f) return __builtin_inff();
+    return rexp(-beta * dE);
+}
+
+// Block reduction of one int64 and C float64 channels over NT threads; the
+// result is valid in thread 0 (waves added in order).
+template <int C, int NT>
+__device__ void block_sum(long long &e, double (&v)[C], long long *she, double (*sh)[NT / 64]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
+    if (lane == 0) she[wave] = e;
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        double x = v[c];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if (lane == 0) sh[c][wave] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long te = she[0];
+#pragma unroll
+        for (int w = 1; w < NT / 64; w++) te += she[w];
+        e = te;
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            double t = sh[c][0];
+#pragma unroll
+            for (int w = 1; w < NT / 64; w++) t += sh[c][w];
+            v[c] = t;
+        }
+    }
+    __syncthreads();
+}
+
+// LDS bytes of one strip's site data: RNG streams and angles of rows
+// 1..H+1, cos/sin (or spins) of rows 0..H+2
+__host__ __device__ inline size_t strip_bytes(int model, int L, int H) {
+    const size_t rng = (size_t)(H + 1) * L * 16;
+    return model == MODEL_ISING ? rng + (size_t)(H + 3) * L
+                                : rng + (size_t)(H + 3) * L * 8 + (size_t)(H + 1) * L * 4;
+}
+
+// LDS arrays of one strip
+struct StripView {
+    ulonglong2 *rng;  // rows 1..H+1
+    float *c, *s;     // rows 0..H+2 (XY)
+    float *th;        // rows 1..H+1 (XY)
+    int8_t *sp;       // rows 0..H+2 (Ising)
+};
+
+__device__ __forceinline__ StripView strip_view(unsigned char *base, int L, int H) {
+    StripView v;
+    v.rng = reinterpret_cast<ulonglong2 *>(base);
+    v.c = reinterpret_cast<float *>(v.rng + (size_t)(H + 1) * L);
+    v.s = v.c + (H + 3) * L;
+    v.th = v.s + (H + 3) * L;
+    v.sp = reinterpret_cast<int8_t *>(v.rng + (size_t)(H + 1) * L);
+    return v;
+}
+
+// Update site (colour row ri, column 2 tcol + (c & 1)) of colour c of a
+// strip; true if the move was accepted.
+//  * MODEL_ISING: the reference's update (models/ising/update.py:16-29) with
+//    thresholds thr[k] for 2 s sum_nn = 4k - 8.
+//  * MODEL_XY_FAST: the reference's proposal and rule (models/xy/update.py:
+//    21-54) with dE from the cached (cos, sin) of the neighbours and the
+//    float32 sincos/exp of mc_fastmath.h.
+//  * MODEL_XY_GAUSS: proposal theta + sigma z; z and the acceptance uniform
+//    from one 64-bit draw (gauss_draw).
+template <int MODEL>
+__device__ __forceinline__ bool site_update(const StripView &v, const int c, const int ri, const int tcol,
+                                            const int L, const float beta, const float sigma, const float J,
+                                            const float delta, const float *thr) {
+    const int half = L >> 1;
+    const int p = c & 1;
+    const int lr = 1 + 2 * ri + (c >> 1);
+    const int base = lr * L;
+    const int self = base + p * half + tcol;
+    const int up = self - L, dn = self + L;
+    const int lf = p ? base + tcol : base + half + (tcol == 0 ? half - 1 : tcol - 1);
+    const int rt = p ? base + (tcol == half - 1 ? 0 : tcol + 1) : base + half + tcol;
+    const int own = self - L;  // RNG and angle arrays start at local row 1
+    const ulonglong2 st = v.rng[own];
+    Rng g;
+    g.s0 = st.x;
+    g.s1 = st.y;
+    bool acc;
+    if (MODEL == MODEL_ISING) {
+        const int sv = v.sp[self];
+        const int nb = (int)v.sp[up] + (int)v.sp[dn] + (int)v.sp[lf] + (int)v.sp[rt];
+        const float u = g.uniform_f32();
+        acc = u < thr[(2 * sv * nb + 8) >> 2];
+        if (acc) v.sp[self] = (int8_t)(-sv);
+    } else {
+        const float one = 1.0f, two = 2.0f;
+        const float pi = 3.14159274101257324f, two_pi = 6.28318548202514648f;
+        const float th = v.th[own];
+        float pr;
+        float w;
+        if (MODEL == MODEL_XY_GAUSS) {
+            const GaussDraw d = gauss_draw(g.next());
+            pr = wrap_turns(th + sigma * d.z);
+            w = d.v;
+        } else {
+            const float u = g.uniform_f32();
+            pr = th + delta * (two * u - one);
+            if (pr >= pi)
+                pr -= two_pi;
+            else if (pr < -pi)
+                pr += two_pi;
+        }
+        float sp, cp;
+        fast_sincosf(pr, sp, cp);
+        const float Cs = ((v.c[up] + v.c[dn]) + v.c[lf]) + v.c[rt];
+        const float Ss = ((v.s[up] + v.s[dn]) + v.s[lf]) + v.s[rt];
+        float t = (cp - v.c[self]) * Cs;
+        t = fmaf(sp - v.s[self], Ss, t);
+        const float dE = -J * t;
+        if (MODEL != MODEL_XY_GAUSS) w = g.uniform_f32();
+        acc = w < accept_weight(-beta * dE);  // dE <= 0 or w < exp(-beta dE)
+        if (acc) {
+            v.th[own] = pr;
+            v.c[self] = cp;
+            v.s[self] = sp;
+        }
+    }
+    ulonglong2 out;
+    out.x = g.s0;
+    out.y = g.s1;
+    v.rng[own] = out;
+    return acc;
+}
+
+// Observables of the site in LDS row lr (1..H), column j: right and down bonds
+// (the row below row H is the ghost row H+1).  Adds the integer energy term
+// to e and, when `all`, magnetisation (and XY twist sums) to m.
+template <int MODEL>
+__device__ __forceinline__ void site_observables(const StripView &v, const int lr, const int j, const int L,
+                                                 const bool all, long long &e, double (&m)[4]) {
+    const int self = lds_index(lr, j, L), dn = lds_index(lr + 1, j, L);
+    const int rt = lds_index(lr, j == L - 1 ? 0 : j + 1, L);
+    if (MODEL == MODEL_ISING) {
+        const int sv = v.sp[self];
+        e += sv * ((int)v.sp[dn] + (int)v.sp[rt]);
+        if (all) m[0] += sv;
+    } else {
+        const float cs = v.c[self], ss = v.s[self];
+        const float bxc = fmaf(ss, v.s[dn], cs * v.c[dn]);
+        const float byc = fmaf(ss, v.s[rt], cs * v.c[rt]);
+        e += bond_fixed(bxc + byc);
+        if (all) {
+            m[0] += cs;
+            m[1] += ss;
+            m[2] += bxc;
+            m[3] += fmaf(ss, v.c[dn], -(cs * v.s[dn]));
+        }
+    }
+}
+
+}  // namespace mc

@@ -76,6 +76,20 @@ SIGNATURES = {
     "mc_batch_max_workgroups": [INT, INT, INT, INT, INT, INT, ctypes.POINTER(INT)],
     "mc_batch_run": [P, INT, INT, INT, INT, INT, P, P, P, F32, F32, P, P, INT, I64, INT, P, P, P, P, P, P, P,
                      INT, I64, INT, P, P, P, P, P],
+    "mc_pack_lds_bytes": [INT, INT, INT, INT, INT, ctypes.POINTER(SZ)],
+    "mc_pack_occupancy": [INT, SZ, ctypes.POINTER(INT)],
+    "mc_pack_create": [INT, ctypes.POINTER(P)],
+    "mc_pack_destroy": [P],
+    "mc_pack_add_group": [P, INT, INT, INT, INT, INT, INT, F32, F32, P, P, P, P, P, P, P, P, P, P, P, P,
+                          ctypes.POINTER(INT)],
+    "mc_pack_set_work": [P, P, INT],
+    "mc_pack_set_sweeps": [P, I64],
+    "mc_pack_sweeps": [P, ctypes.POINTER(I64)],
+    "mc_pack_error": [P, ctypes.POINTER(INT)],
+    "mc_pack_run": [P, P, INT, INT, I64, INT, P, P],
+    "mc_pack_capture": [P, P, INT, INT, I64, INT, P, P, ctypes.POINTER(P)],
+    "mc_graph_launch": [P, P],
+    "mc_graph_destroy": [P],
 }
 
 _lock = threading.RLock()

@@ -36,6 +36,7 @@ HIP_FLAGS = [
 # pointer increments each (VGPRs 136 -> 82 without it, fewer instructions).
 FILE_FLAGS = {
     "mc_batch.hip": ["-mllvm", "-disable-lsr"],
+    "mc_pack.hip": ["-mllvm", "-disable-lsr"],
 }
 
 

@@ -57,13 +57,14 @@ def _edge_summary(T, pt):
 
 
 class PTAnalysis:
-    """Diagnostics of every run in ``{L: run}`` that carries PT metadata."""
+    """Diagnostics of every run in ``{L: run}`` (or ``{(L, i): run}``) that carries PT metadata."""
 
     def __init__(self, runs):
         self.results = {}
         self._histories = {}
         self._thermalization = {}
-        for L in sorted(runs):
+        # sizes in order; runs of one size under several models carry (L, i) keys
+        for L in sorted(runs, key=lambda k: k if isinstance(k, tuple) else (k,)):
             metadata = runs[L].get("metadata", {})
             pt = metadata.get("parallel_tempering")
             if pt is None:

@@ -220,6 +220,61 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, int threads, void
                  long long *trips, int sample_interval, long long sample_first, int max_samples,
                  double *sample_partials, int32_t *sample_w2t, uint64_t *halo, uint64_t *energy,
                  int *error);
+/* Energies of the batched and packed paths are integer sums: Ising
+ * sum s (s_down + s_right) exactly, XY the per-site (cos bond x + cos bond y)
+ * in float32 rounded to multiples of 2^-40 and added in int64; the energy is
+ * -J times that sum (times 2^-40 for XY).  sample_partials channel 0 holds
+ * each strip's energy. */
+
+/* ---------------------------------------------- packed persistent path
+ * No single reference function: the loop of core/simulation.py:36-72 (as in
+ * mc_batch_run) for several ensembles -- lattice sizes, models, ladders --
+ * in one launch, e.g. a finite-size-scaling study.  A pack owns its
+ * workspace; the ensembles' arrays stay the caller's.  Each ensemble is cut
+ * into strips of H rows; a task is one strip position of K replicas and is
+ * streamed through LDS every sweep.  Sequence: mc_pack_create, one
+ * mc_pack_add_group per ensemble, mc_pack_set_work, then mc_pack_run (or
+ * mc_pack_capture + mc_graph_launch) any number of times, mc_pack_destroy.
+ * Trajectories, exchanges and samples equal mc_batch_run's for the same
+ * ensemble. */
+/* Dynamic LDS of a workgroup running tasks of shape (H, K) of an ensemble */
+int mc_pack_lds_bytes(int model, int L, int R, int H, int K, size_t *bytes);
+/* Co-resident packed workgroups of `threads` (256 or 512) threads with
+ * lds_bytes of dynamic LDS */
+int mc_pack_occupancy(int threads, size_t lds_bytes, int *max_workgroups);
+int mc_pack_create(int threads, void **pack);
+int mc_pack_destroy(void *pack);
+/* Arguments as mc_batch_run's; K replicas per task (a power of two dividing
+ * R, at most threads / 16), L <= threads.  group_index receives the
+ * ensemble's index in the pack. */
+int mc_pack_add_group(void *pack, int model, int L, int R, int H, int K, int exchange_interval, float J,
+                      float delta, void *state, uint64_t *rng, float *betas, const float *sigmas,
+                      unsigned long long *accepted, int *parity, double *energies,
+                      int32_t *walker_to_temperature, int32_t *temperature_to_walker, uint64_t *pt_rng,
+                      long long *counts, long long *trips, int *group_index);
+/* work: nworkgroups rows of (group, first task, task count); a group's rows
+ * are consecutive and list its tasks (R / K * L / H of them) in order.  The
+ * launch must fit co-resident (mc_pack_occupancy). */
+int mc_pack_set_work(void *pack, const int *work, int nworkgroups);
+/* The sweep count the next launch starts from (default 0); it lives on the
+ * device and every launch or replay advances it. */
+int mc_pack_set_sweeps(void *pack, long long sweeps);
+int mc_pack_sweeps(void *pack, long long *sweeps);
+/* Nonzero after a launch stopped by its watchdog (synchronous) */
+int mc_pack_error(void *pack, int *error);
+/* nsweeps sweeps of every ensemble.  sample_partials[g] ([max_samples][R][L/H][5]
+ * float64) and sample_w2t[g] ([max_samples][R]) are device buffers per
+ * ensemble, in a host array; samples at sweeps sample_first + k * interval. */
+int mc_pack_run(void *pack, void *stream, int nsweeps, int sample_interval, long long sample_first,
+                int max_samples, double *const *sample_partials, int32_t *const *sample_w2t);
+/* The same launch captured into an executable HIP graph (prologue kernel +
+ * packed kernel, no memset nodes); each mc_graph_launch continues the chains
+ * by nsweeps.  Samples of all replays go to the given buffers. */
+int mc_pack_capture(void *pack, void *stream, int nsweeps, int sample_interval, long long sample_first,
+                    int max_samples, double *const *sample_partials, int32_t *const *sample_w2t,
+                    void **graph);
+int mc_graph_launch(void *graph, void *stream);
+int mc_graph_destroy(void *graph);
 
 #ifdef __cplusplus
 }

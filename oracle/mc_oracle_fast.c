@@ -276,10 +276,14 @@ void orc_xyfast_sweeps(float *th, float *c, float *s, uint64_t *rng, const float
 }
 
 /* energy, mx, my, twist_cos_x, twist_sin_x from the cache; per-site terms in
- * float32 exactly as the kernel forms them, sums in float64 */
+ * float32 exactly as the kernel forms them.  The energy is the batched
+ * engine's integer sum: each site's (cos bond x + cos bond y) rounded to the
+ * nearest multiple of 2^-40 (ties to even), added exactly in int64, then
+ * -J * 2^-40 * sum in float64.  The other sums are float64. */
 void orc_xyfast_observables(const float *c, const float *s, int R, int L, float J, double *out) {
     for (int r = 0; r < R; r++) {
-        double e = 0, mx = 0, my = 0, cx = 0, qx = 0;
+        int64_t ef = 0;
+        double mx = 0, my = 0, cx = 0, qx = 0;
         for (int i = 0; i < L; i++)
             for (int j = 0; j < L; j++) {
                 const size_t k = FSITE(L, r, i, j);
@@ -287,13 +291,13 @@ void orc_xyfast_observables(const float *c, const float *s, int R, int L, float 
                 const float bxc = fmaf(s[k], s[kd], c[k] * c[kd]);
                 const float byc = fmaf(s[k], s[kr], c[k] * c[kr]);
                 const float bxs = fmaf(s[k], c[kd], -(c[k] * s[kd]));
-                e += (double)(-J * (bxc + byc));
+                ef += (int64_t)llrint((double)(bxc + byc) * 0x1p40);
                 mx += c[k];
                 my += s[k];
                 cx += bxc;
                 qx += bxs;
             }
-        out[0 * R + r] = e;
+        out[0 * R + r] = -(double)J * ((double)ef * 0x1p-40);
         out[1 * R + r] = mx;
         out[2 * R + r] = my;
         out[3 * R + r] = cx;

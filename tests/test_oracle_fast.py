@@ -82,6 +82,30 @@ def test_fast_observables_equal_reference_measurement():
     assert (e, mx, my, cx, qx) == (-2 * float(np.float32(J)) * L * L, L * L, 0.0, L * L, 0.0)
 
 
+def test_fast_energy_is_an_exact_integer_sum():
+    """The batched engines' energy: per-site (cos bond x + cos bond y) rounded to
+    multiples of 2^-40 and added as integers, so any summation order -- here the
+    lattice translated, which reorders every sum -- gives the same bits."""
+    L, J = 32, 1.0
+    theta = np.random.default_rng(11).uniform(-np.pi, np.pi, (1, L, L)).astype(np.float32)
+    c, s = oracle.xyfast_cache(theta)
+    e0 = oracle.xyfast_observables(c, s, J)[0, 0]
+    for shift in [(3, 0), (0, 5), (7, 11)]:
+        rolled = np.ascontiguousarray(np.roll(theta, shift, axis=(1, 2)))
+        cr, sr = oracle.xyfast_cache(rolled)
+        assert oracle.xyfast_observables(cr, sr, J)[0, 0] == e0
+    # quantisation: within 2^-41 per site of the float64 sum of the float32 terms
+    cd, sd = c.astype(np.float64), s.astype(np.float64)
+    down = lambda a: np.roll(a, -1, axis=1)
+    right = lambda a: np.roll(a, -1, axis=2)
+    # fmaf(s, s_n, c * c_n) in float32, as the kernels form the bond terms
+    bx = (sd * down(sd) + (c * down(c)).astype(np.float64)).astype(np.float32)
+    by = (sd * right(sd) + (c * right(c)).astype(np.float64)).astype(np.float32)
+    approx = -J * np.sum((bx + by).astype(np.float64))
+    assert abs(e0 - approx) <= L * L * 2.0 ** -41 + 1e-9
+    assert abs(e0 - (-J * np.sum(cd * down(cd) + sd * down(sd) + cd * right(cd) + sd * right(sd)))) < L * L * 1e-6
+
+
 def _binned(x, nbins):
     b = np.asarray(x)[: len(x) // nbins * nbins].reshape(nbins, -1).mean(axis=1)
     return b.mean(), b.std(ddof=1) / np.sqrt(nbins)

@@ -122,3 +122,8 @@ def test_invalid_histories_are_rejected(positions):
         "walker_ids": [0], "sweeps": [0, 10], "temperature_indices": positions}
     with pytest.raises(ValueError):
         PTAnalysis({16: run})
+
+
+def test_runs_of_one_size_under_several_models_keep_their_keys():
+    analysis = PTAnalysis({16: saved_run(), (16, 1): saved_run(), 8: saved_run()})
+    assert list(analysis.summary()) == [8, 16, (16, 1)]
