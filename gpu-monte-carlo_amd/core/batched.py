@@ -12,8 +12,9 @@ Models:
 * ``IsingModel``: the reference update; trajectories, exchanges and samples
   are bit-identical to the drop-in path with the same seeds.
 * ``XYModel(dtype=float32)``: the float32 fast update (dE from cached cos/sin
-  of every site, float32 sincos/exp); energies are compensated float32 sums
-  per thread, float64 across threads and strips.
+  of every site, float32 sincos/exp); energies are exact integer sums of the
+  float32 bond terms, each rounded to a multiple of 2^-40, so they do not
+  depend on how sites are spread over threads, strips and workgroups.
 * ``XYModel(dtype=float32, proposal="gaussian")``: the same with Gaussian
   proposals of per-temperature width; ``adapt_sigmas`` tunes the widths
   toward a target acceptance during thermalisation.
@@ -66,16 +67,23 @@ def max_workgroups(code, L, R, H, exchange, threads=256):
     return int(n.value)
 
 
+def sites_per_pass(L, H, threads):
+    """Sites a thread updates in the largest pass of a sweep (the kernel keeps
+    their streams in registers and supports at most two)."""
+    rows = threads // (L // 2)
+    return -(-max(H // 2, 4) // rows)
+
+
 def candidate_configs(code, L, R, exchange):
     """(strip height H, workgroup size) pairs whose grid (R * L / H workgroups)
-    can be co-resident: H an even divisor of L, at most 8 colour rows of work
-    per thread and pass, and at least one colour row per wave for 512 threads."""
+    can be co-resident: H an even divisor of L, at most two sites per thread
+    and pass, and at least one colour row per wave for 512 threads."""
     out = []
     for threads in WORKGROUP_SIZES:
         if L // 2 > threads:
             continue
         for H in range(4, L + 1, 2):
-            if L % H or (H // 2) * (L // 2) > 8 * threads:
+            if L % H or sites_per_pass(L, H, threads) > 2:
                 continue
             if threads > 256 and (H // 2) * (L // 2) < threads:
                 continue

@@ -38,8 +38,9 @@ DEFAULT_CONFIG = (256, 2048)
 
 def group_shapes(L, R, threads):
     """Every valid (H, K) of one ensemble: H an even divisor of L (>= 4), K a
-    power of two dividing R with K strips fitting one pass and NT / K >= 16
-    threads per replica for the observables."""
+    power of two dividing R with K strips fitting one pass, K rows of L sites
+    fitting one row map (K * L <= NT) and NT / K >= 16 threads per replica
+    for the observables."""
     if L > threads or L % 2 or L < 4:
         return []
     rows = threads // (L // 2)
@@ -48,14 +49,14 @@ def group_shapes(L, R, threads):
         if L % H:
             continue
         K = 1
-        while K <= threads // 16 and K <= rows:
+        while K <= threads // 16 and K <= rows and K * L <= threads:
             if R % K == 0:
                 out.append((H, K))
             K *= 2
     return out
 
 
-def choose_shape(L, R, threads, task_sites, lds_of=None, lds_limit=160 * 1024):
+def choose_shape(L, R, threads, task_sites, lds_of=None, lds_limit=159 * 1024):
     """(H, K) whose task (K * H * L sites) is closest to `task_sites` in ratio;
     ties go to taller strips (less halo and ghost traffic), then fewer replicas."""
     best, key = None, None

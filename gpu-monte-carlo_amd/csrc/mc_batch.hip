@@ -85,7 +85,31 @@ struct BatchArgs {
     int *error;
 };
 
-template <int MODEL, int NT>
+// Item q of pass p (the five-pass schedule below): colour c and colour row
+// ri; false if the pass has fewer items.
+__device__ __forceinline__ bool pass_item(const int p, const int q, const int HH, int &c, int &ri) {
+    switch (p) {
+    case 0:  // c0 of colour rows 1..HH-1 (local rows 3..H-1)
+    case 1:  // c1 of the same rows
+        c = p;
+        ri = 1 + q;
+        return q < HH - 1;
+    case 2:  // c0 of colour rows 0 and HH (ghost); c2 of colour rows 1..HH-2
+        c = q < 2 ? 0 : 2;
+        ri = q == 0 ? 0 : (q == 1 ? HH : q - 1);
+        return q < HH;
+    case 3:  // c1 of colour rows 0 and HH; c2 of colour rows 0 and HH-1
+        c = q < 2 ? 1 : 2;
+        ri = (q == 0 || q == 2) ? 0 : (q == 1 ? HH : HH - 1);
+        return q < 4;
+    default:  // c3 of colour rows 0..HH-1
+        c = 3;
+        ri = q;
+        return q < HH;
+    }
+}
+
+template <int MODEL, int NT, int MAXP>
 __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const int L = a.L, H = a.H, S = a.S, R = a.R;
@@ -99,14 +123,13 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     const int r = lin / S, s = lin % S;
     const int s_up = (s + S - 1) % S, s_dn = (s + 1) % S;
 
-    // ---- LDS carve-up
-    const StripView view = strip_view(smem, L, H);
-    ulonglong2 *lrng = view.rng;  // rows 1..H+1
-    float *lc = view.c;           // rows 0..H+2
-    float *ls = view.s;           // rows 0..H+2
-    float *lth = view.th;         // rows 1..H+1
-    int8_t *lsp = view.sp;        // rows 0..H+2 (Ising)
-    unsigned char *pt_base = smem + ((strip_bytes(MODEL, L, H) + 15) & ~(size_t)15);
+    // ---- LDS carve-up (the RNG streams live in registers)
+    const StripView view = strip_view(smem, L, H, false);
+    float *lc = view.c;     // rows 0..H+2
+    float *ls = view.s;     // rows 0..H+2
+    float *lth = view.th;   // rows 1..H+1
+    int8_t *lsp = view.sp;  // rows 0..H+2 (Ising)
+    unsigned char *pt_base = smem + ((strip_bytes(MODEL, L, H, false) + 15) & ~(size_t)15);
     float *pt_beta = reinterpret_cast<float *>(pt_base);                 // R
     int32_t *pt_t2w = reinterpret_cast<int32_t *>(pt_beta + R);          // R
     int32_t *pt_w2t = pt_t2w + R;                                         // R
@@ -122,13 +145,11 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     const int row0 = s * H;  // first global row of the strip
     const bool use_pt = a.exchange_interval > 0;
 
-    // ---- load the strip: rows 0..H+2 (global rows row0-1 .. row0+H+1)
+    // ---- load the strip's state: rows 0..H+2 (global rows row0-1 .. row0+H+1)
     for (int idx = tid; idx < (H + 3) * L; idx += NT) {
         const int lr = idx / L, j = idx - lr * L;
         const int gi = (row0 + lr - 1 + L) % L;
         const size_t gk = rep + (size_t)gi * L + j;
-        const bool upd_row = lr >= 1 && lr <= H + 1;  // own rows and the ghost
-        if (upd_row) lrng[lds_index(lr - 1, j, L)] = reinterpret_cast<const ulonglong2 *>(a.rng)[gk];
         if (MODEL == MODEL_ISING) {
             lsp[lds_index(lr, j, L)] = reinterpret_cast<const int8_t *>(a.state)[gk];
         } else {
@@ -137,7 +158,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             fast_sincosf(th, sv, cv);
             lc[lds_index(lr, j, L)] = cv;
             ls[lds_index(lr, j, L)] = sv;
-            if (upd_row) lth[lds_index(lr - 1, j, L)] = th;
+            if (lr >= 1 && lr <= H + 1) lth[lds_index(lr - 1, j, L)] = th;
         }
     }
     if (use_pt) {
@@ -156,46 +177,54 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     if (MODEL == MODEL_ISING && tid < 5) thr[tid] = ising_threshold(4 * tid - 8, my_beta, a.J);
     int parity = use_pt ? *a.parity : 0;
     bool exchanged = false;  // pt_e holds this launch's last exchange energies
-    __syncthreads();
 
     const float J = a.J, delta = a.delta;
     int n_acc = 0;  // accepted moves of this thread since the last flush
-    // fixed thread -> (column, colour-row offset) map of every pass
+    // fixed thread -> (column, colour-row offset) map of every pass: a thread
+    // takes items trow, trow + rows_per_pass, ... (at most MAXP) of each pass
     const int tcol = tid % half, trow = tid / half, rows_per_pass = NT / half;
 
-    // Update site (colour row ri, column 2 tcol + (c & 1)) of colour c.  The
-    // ghost row's moves belong to the strip below and are not counted.
-    auto site = [&](const int c, const int ri, const float beta) {
-        const int counted = !((c < 2) & (ri == HH));
-        if (site_update<MODEL>(view, c, ri, tcol, L, beta, sigma, J, delta, thr)) n_acc += counted;
+    // ---- the thread's sites: LDS positions and RNG streams in registers for
+    // the whole launch (the ghost row's streams are a copy of its owner's)
+    constexpr int NP = 5;
+    int slot_self[NP][MAXP], slot_lf[NP][MAXP], slot_rt[NP][MAXP];
+    bool slot_on[NP][MAXP], slot_own[NP][MAXP];
+    uint64_t slot_s0[NP][MAXP], slot_s1[NP][MAXP];
+    SiteDraw slot_draw[NP][MAXP];
+    auto slot_global = [&](int c, int ri) {
+        const int lr = 1 + 2 * ri + (c >> 1);
+        return rep + (size_t)((row0 + lr - 1) % L) * L + 2 * tcol + (c & 1);
     };
-    // One pass: up to four groups of colour rows, spread over the threads in
-    // the fixed map (a thread takes colour rows trow, trow + rows_per_pass, ...
-    // of the concatenated groups).
-    auto pass = [&](const Seg g0, const Seg g1, const Seg g2, const Seg g3, const float beta) {
-        if (trow >= rows_per_pass) return;
-        const int n0 = max(g0.hi - g0.lo + 1, 0), n1 = max(g1.hi - g1.lo + 1, 0);
-        const int n2 = max(g2.hi - g2.lo + 1, 0), n3 = max(g3.hi - g3.lo + 1, 0);
-        const int total = n0 + n1 + n2 + n3;
-        for (int q = trow; q < total; q += rows_per_pass) {
-            int k = q;
-            Seg g = g0;
-            if (k >= n0) {
-                k -= n0;
-                g = g1;
-                if (k >= n1) {
-                    k -= n1;
-                    g = g2;
-                    if (k >= n2) {
-                        k -= n2;
-                        g = g3;
-                    }
-                }
+#pragma unroll
+    for (int p = 0; p < NP; p++)
+#pragma unroll
+        for (int m = 0; m < MAXP; m++) {
+            int c = 0, ri = 0;
+            const bool on = trow < rows_per_pass && pass_item(p, trow + m * rows_per_pass, HH, c, ri);
+            const int pc = c & 1, base = (1 + 2 * ri + (c >> 1)) * L;
+            slot_on[p][m] = on;
+            slot_own[p][m] = on && !((c < 2) & (ri == HH));  // the ghost row's moves are its owner's
+            slot_self[p][m] = base + pc * half + tcol;
+            slot_lf[p][m] = pc ? base + tcol : base + half + (tcol == 0 ? half - 1 : tcol - 1);
+            slot_rt[p][m] = pc ? base + (tcol == half - 1 ? 0 : tcol + 1) : base + half + tcol;
+            slot_s0[p][m] = slot_s1[p][m] = 0;
+            if (on) {
+                const size_t gk = slot_global(c, ri);
+                slot_s0[p][m] = a.rng[2 * gk];
+                slot_s1[p][m] = a.rng[2 * gk + 1];
             }
-            site(g.c, g.lo + k, beta);
+            slot_draw[p][m] = SiteDraw{0.0f, 0.0f};
         }
+    __syncthreads();
+
+    // The sites of pass p with this sweep's draws
+    auto apply = [&](const int p, const float beta) {
+#pragma unroll
+        for (int m = 0; m < MAXP; m++)
+            if (slot_on[p][m] && site_apply<MODEL>(view, slot_self[p][m], slot_lf[p][m], slot_rt[p][m], L, beta,
+                                                   sigma, J, delta, thr, slot_draw[p][m]))
+                n_acc += slot_own[p][m];
     };
-    const Seg none{0, 0, -1};
     auto halo_ptr = [&](int ss, int which, int par) {
         return a.halo + ((((size_t)r * S + ss) * 2 + which) * 2 + par) * L;
     };
@@ -269,14 +298,12 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
                 const uint32_t payload = MODEL == MODEL_ISING
                                              ? (uint32_t)(uint8_t)lsp[lds_index(lr, j, L)]
                                              : __float_as_uint(lth[lds_index(lr - 1, j, L)]);
-                __hip_atomic_store(dst + j, ((uint64_t)(unsigned)(n + 1) << 32) | payload, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+                publish_word(dst + j, (unsigned)(n + 1), payload);
             }
         }
     };
 
     // Add this workgroup's accepted moves to the counter of its temperature.
-    // The ghost row's moves belong to the strip below and are not counted.
     // Called by every thread (contains barriers).
     auto flush_accepted = [&]() {
         if (!a.accepted) return;
@@ -301,6 +328,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     //   c3(row) follows c2(row) and c1(rows +-1),
     //   and each site's update precedes its neighbours' next update
     // gives identical results.  Per sweep n:
+    //   D   every site's draws for the sweep, from its stream (registers only)
     //   P1  c0 of interior 0/1 rows (local 3..H-1)
     //   P2  c1 of the same rows; receive rows 0 and H+2 (sweep n-1)
     //   P3  c0 of rows 1 and H+1 (ghost); c2 of local rows 4..H-2
@@ -311,15 +339,27 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     for (int n = 0; n < a.nsweeps; n++) {
         const long long sweep_global = a.sweep0 + n + 1;
         const float beta = my_beta;
-        pass(Seg{0, 1, HH - 1}, none, none, none, beta);
+#pragma unroll
+        for (int p = 0; p < NP; p++)
+#pragma unroll
+            for (int m = 0; m < MAXP; m++)
+                if (slot_on[p][m]) {
+                    Rng g;
+                    g.s0 = slot_s0[p][m];
+                    g.s1 = slot_s1[p][m];
+                    slot_draw[p][m] = site_draw<MODEL>(g);
+                    slot_s0[p][m] = g.s0;
+                    slot_s1[p][m] = g.s1;
+                }
+        apply(0, beta);
         __syncthreads();
-        pass(Seg{1, 1, HH - 1}, none, none, none, beta);
+        apply(1, beta);
         if (!receive(n)) goto done;
-        pass(Seg{0, 0, 0}, Seg{0, HH, HH}, Seg{2, 1, HH - 2}, none, beta);
+        apply(2, beta);
         __syncthreads();
-        pass(Seg{1, 0, 0}, Seg{1, HH, HH}, Seg{2, 0, 0}, Seg{2, HH - 1, HH - 1}, beta);
+        apply(3, beta);
         __syncthreads();
-        pass(Seg{3, 0, HH - 1}, none, none, none, beta);
+        apply(4, beta);
         publish(n);
 
         const bool do_exchange = use_pt && (sweep_global % a.exchange_interval == 0);
@@ -477,17 +517,27 @@ done:
         }
         for (int k = tid; k < 2 * (R - 1); k += NT) a.pt_rng[k] = pt_g[k];
     }
-    // ---- write back own rows: state and RNG streams
+    // ---- write back own rows' state and the streams of owned sites
     __syncthreads();
     for (int idx = tid; idx < H * L; idx += NT) {
         const int lr = 1 + idx / L, j = idx - (lr - 1) * L;
         const size_t gk = rep + (size_t)(row0 + lr - 1) * L + j;
-        reinterpret_cast<ulonglong2 *>(a.rng)[gk] = lrng[lds_index(lr - 1, j, L)];
         if (MODEL == MODEL_ISING)
             reinterpret_cast<int8_t *>(a.state)[gk] = lsp[lds_index(lr, j, L)];
         else
             reinterpret_cast<float *>(a.state)[gk] = lth[lds_index(lr - 1, j, L)];
     }
+#pragma unroll
+    for (int p = 0; p < NP; p++)
+#pragma unroll
+        for (int m = 0; m < MAXP; m++)
+            if (slot_own[p][m]) {
+                int c = 0, ri = 0;
+                pass_item(p, trow + m * rows_per_pass, HH, c, ri);
+                const size_t gk = slot_global(c, ri);
+                a.rng[2 * gk] = slot_s0[p][m];
+                a.rng[2 * gk + 1] = slot_s1[p][m];
+            }
     if (blockIdx.x == 0 && tid == 0 && use_pt) *a.parity = parity;
 }
 
@@ -507,28 +557,46 @@ static size_t energy_words(int L, int R, int H) { return (size_t)2 * R * (L / H)
 static size_t batch_lds(int model, int L, int R, int H, int exchange) {
     const size_t pt_bytes =
         exchange ? (size_t)3 * R * 4 + 8 + (size_t)R * 8 + (size_t)2 * R * 8 + (size_t)R * (L / H) * 2 * 4 : 0;
-    return ((strip_bytes(model, L, H) + 15) & ~(size_t)15) + pt_bytes + 64;
+    return ((strip_bytes(model, L, H, false) + 15) & ~(size_t)15) + pt_bytes + 64;
 }
 
-template <int NT>
-static const void *kernel_ptr_nt(int model) {
-    if (model == MODEL_ISING) return reinterpret_cast<const void *>(&batch_kernel<MODEL_ISING, NT>);
-    if (model == MODEL_XY_FAST) return reinterpret_cast<const void *>(&batch_kernel<MODEL_XY_FAST, NT>);
-    return reinterpret_cast<const void *>(&batch_kernel<MODEL_XY_GAUSS, NT>);
+// Items per pass and thread: the largest pass has max(H/2, 4) colour rows,
+// spread over threads / (L/2) colour-row slots
+static int batch_items(int L, int H, int threads) {
+    const int rpp = threads / (L / 2);
+    const int most = std::max(H / 2, 4);
+    return (most + rpp - 1) / rpp;
 }
 
-static const void *kernel_ptr(int model, int threads) {
-    return threads == 512 ? kernel_ptr_nt<512>(model) : kernel_ptr_nt<256>(model);
+template <int NT, int MAXP>
+static const void *kernel_ptr_nm(int model) {
+    if (model == MODEL_ISING) return reinterpret_cast<const void *>(&batch_kernel<MODEL_ISING, NT, MAXP>);
+    if (model == MODEL_XY_FAST) return reinterpret_cast<const void *>(&batch_kernel<MODEL_XY_FAST, NT, MAXP>);
+    return reinterpret_cast<const void *>(&batch_kernel<MODEL_XY_GAUSS, NT, MAXP>);
 }
 
-static int check_shape(int model, int L, int R, int H, int threads) {
-    MC_REQUIRE(threads == 256 || threads == 512, "workgroup size must be 256 or 512, got %d", threads);
+static const void *kernel_ptr(int model, int threads, int items) {
+    if (threads == 512) return items <= 1 ? kernel_ptr_nm<512, 1>(model) : kernel_ptr_nm<512, 2>(model);
+    return items <= 1 ? kernel_ptr_nm<256, 1>(model) : kernel_ptr_nm<256, 2>(model);
+}
+
+static int check_strips(int model, int L, int R, int H) {
     MC_REQUIRE(model == MODEL_ISING || model == MODEL_XY_FAST || model == MODEL_XY_GAUSS,
                "unknown batch model %d", model);
     MC_REQUIRE(L >= 4 && L % 2 == 0, "L must be even and >= 4, got %d", L);
     MC_REQUIRE(H >= 4 && H % 2 == 0 && L % H == 0, "strip height %d must be even, >= 4 and divide L=%d",
                H, L);
+    MC_REQUIRE(R >= 1, "need at least one replica");
+    return MC_OK;
+}
+
+static int check_shape(int model, int L, int R, int H, int threads) {
+    MC_REQUIRE(threads == 256 || threads == 512, "workgroup size must be 256 or 512, got %d", threads);
+    int rc = check_strips(model, L, R, H);
+    if (rc) return rc;
     MC_REQUIRE(L / 2 <= threads, "L=%d is too wide for a %d-thread workgroup (L/2 <= threads)", L, threads);
+    MC_REQUIRE(batch_items(L, H, threads) <= 2,
+               "strips of %d rows of L=%d need more than two sites per pass and thread of %d", H, L, threads);
     MC_REQUIRE(R >= 1, "need at least one replica");
     return MC_OK;
 }
@@ -540,7 +608,7 @@ using namespace mc;
 extern "C" {
 
 int mc_batch_workspace_size(int model, int L, int R, int H, size_t *halo_words_out, size_t *energy_words_out) {
-    int rc = check_shape(model, L, R, H, 256);
+    int rc = check_strips(model, L, R, H);
     if (rc) return rc;
     *halo_words_out = halo_words(L, R, H);
     *energy_words_out = energy_words(L, R, H);
@@ -556,10 +624,11 @@ int mc_batch_max_workgroups(int model, int L, int R, int H, int threads, int exc
         return MC_OK;
     }
     if (lds > 64 * 1024)
-        MC_TRY_HIP(hipFuncSetAttribute(kernel_ptr(model, threads), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds));
+        MC_TRY_HIP(hipFuncSetAttribute(kernel_ptr(model, threads, batch_items(L, H, threads)),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0, dev = 0, cus = 0;
-    MC_TRY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(model, threads), threads, lds));
+    MC_TRY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(model, threads, batch_items(L, H, threads)),
+                                                            threads, lds));
     MC_TRY_HIP(hipGetDevice(&dev));
     MC_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     *max_workgroups = per_cu * cus;
@@ -596,17 +665,17 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, int threads, void
     a.sample_partials = sample_partials; a.sample_w2t = sample_w2t;
     a.halo = halo; a.energy = energy; a.error = error;
 
+    const void *kern = kernel_ptr(model, threads, batch_items(L, H, threads));
     if (lds > 64 * 1024)
-        MC_TRY_HIP(hipFuncSetAttribute(kernel_ptr(model, threads), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds));
+        MC_TRY_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const size_t nh = halo_words(L, R, H);
     const unsigned reset_blocks = (unsigned)std::min<size_t>((nh + 255) / 256, 1024);
     hipLaunchKernelGGL(batch_reset_kernel, dim3(reset_blocks), dim3(256), 0, as_stream(stream), halo, nh, energy,
                        energy_words(L, R, H), error);
     MC_TRY_HIP(hipGetLastError());
     void *kargs[] = {&a};
-    MC_TRY_HIP(hipLaunchCooperativeKernel(kernel_ptr(model, threads), dim3((unsigned)(R * S)), dim3(threads),
-                                          kargs, lds, as_stream(stream)));
+    MC_TRY_HIP(hipLaunchCooperativeKernel(kern, dim3((unsigned)(R * S)), dim3(threads), kargs, lds,
+                                          as_stream(stream)));
     return MC_OK;
 }
 

@@ -10,10 +10,10 @@
 // whole study do not fit on chip together (the RNG streams alone are 16 bytes
 // per site), so a task is streamed through LDS each sweep -- rows 1..H from
 // the lattice arrays, the ghost row from its own buffer, rows 0 and H+2 from
-// the neighbours' halo words -- then swept with the five-pass schedule of the
-// resident engine and written back.  One sweep of a task reads and writes
-// each of its sites' RNG stream and state once, plus one ghost row and two
-// halo rows per strip.
+// the neighbours' halo words, loaded before the strip so that their latency
+// overlaps it -- then swept colour by colour in the reference order and
+// written back.  One sweep of a task reads and writes each of its sites' RNG
+// stream and state once, plus one ghost row and two halo rows per strip.
 //
 // Ordering.  A task at sweep n needs only its neighbours' rows as of the end
 // of sweep n-1, published as tagged halo words (tag n+1 = "at the start of
@@ -43,6 +43,34 @@
 #include "mc_strip.h"
 
 namespace mc {
+
+// Pointers read from the group table are generic; these name them as global
+// memory so that loads and stores compile to global (not flat) instructions.
+template <class T>
+using gmem = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ gmem<T> as_global(void *p) {
+    return (gmem<T>)(T *)p;
+}
+template <class T>
+__device__ __forceinline__ gmem<const T> as_global(const void *p) {
+    return (gmem<const T>)(const T *)p;
+}
+// one RNG record in global memory (member-wise access, merged into one
+// 16-byte load or store)
+struct alignas(16) RngRec {
+    unsigned long long s0, s1;
+};
+__device__ __forceinline__ ulonglong2 load_rec(gmem<RngRec> p) {
+    ulonglong2 r;
+    r.x = p->s0;
+    r.y = p->s1;
+    return r;
+}
+__device__ __forceinline__ void store_rec(gmem<RngRec> p, const ulonglong2 &r) {
+    p->s0 = r.x;
+    p->s1 = r.y;
+}
 
 struct PackGroup {
     int model, L, R, H, S, K;
@@ -89,39 +117,42 @@ struct PackArgs {
 
 // Dynamic LDS of a workgroup of a group: K strips, then the group's PT state
 struct PackLds {
-    size_t stride, beta, w2t, t2w, thr, sig, moves, e, g, prev, total;
+    unsigned stride, beta, w2t, t2w, thr, sig, moves, e, g, prev, total;
 };
 
 __host__ __device__ inline PackLds pack_lds(int model, int L, int R, int H, int K) {
     PackLds o;
-    o.stride = (strip_bytes(model, L, H) + 15) & ~(size_t)15;
-    size_t p = (size_t)K * o.stride;
+    o.stride = (unsigned)((strip_bytes(model, L, H) + 15) & ~(size_t)15);
+    unsigned p = (unsigned)K * o.stride;
     o.beta = p;
-    p += 4 * (size_t)R;
+    p += 4u * R;
     o.w2t = p;
-    p += 4 * (size_t)R;
+    p += 4u * R;
     o.t2w = p;
-    p += 4 * (size_t)R;
+    p += 4u * R;
     o.thr = p;
-    p += 4 * 5 * (size_t)K;
+    p += 20u * K;
     o.sig = p;
-    p += 4 * (size_t)K;
+    p += 4u * K;
     o.moves = p;
-    p += 4 * (size_t)R;
-    p = (p + 7) & ~(size_t)7;
+    p += 4u * R;
+    p = (p + 7) & ~7u;
     o.e = p;
-    p += 8 * (size_t)R;
+    p += 8u * R;
     o.g = p;
-    p += 16 * (size_t)(R > 1 ? R - 1 : 1);
+    p += 16u * (R > 1 ? R - 1 : 1);
     o.prev = p;
-    p += 16 * (size_t)R;
+    p += 16u * R;
     o.total = p;
     return o;
 }
 
+// The group descriptor is read through a pointer where each field is used
+// (scalar loads), rather than held in registers for the whole launch.
 template <int MODEL, int NT>
-__device__ void pack_run(const PackArgs &a, const PackGroup &g, const PackWork &w, unsigned char *smem,
+__device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork &w, unsigned char *smem,
                          long long *red_e, double (*red)[NT / 16]) {
+    const PackGroup &g = *gp;
     const int L = g.L, H = g.H, S = g.S, R = g.R, K = g.K;
     const int HH = H >> 1, half = L >> 1;
     const int tid = threadIdx.x;
@@ -163,11 +194,12 @@ __device__ void pack_run(const PackArgs &a, const PackGroup &g, const PackWork &
     const bool pass_on = pk < K;
     // row map of loads, stores, halo rows: whole rows of L sites
     const int lj = tid % L, lrow = tid / L, lrows = NT / L;
+    const int ljc = (lj & 1) * half + (lj >> 1);  // column lj in a colour-split row
     // observables map: NT / K threads per replica
     const int TPR = NT / K, ok = tid / TPR, ot = tid - ok * TPR;
     const int W = TPR < 64 ? TPR : 64;
 
-    auto view_of = [&](int k) { return strip_view(smem + (size_t)k * lay.stride, L, H); };
+    auto view_of = [&](int k) { return strip_view(smem + (unsigned)k * lay.stride, L, H); };
     auto halo_ptr = [&](int r, int ss, int which, int slot) {
         return g.halo + ((((size_t)r * S + ss) * 2 + which) * 2 + slot) * L;
     };
@@ -196,126 +228,121 @@ __device__ void pack_run(const PackArgs &a, const PackGroup &g, const PackWork &
             const int r0 = b * K;
             const int row0 = s * H;
             const int s_up = (s + S - 1) % S, s_dn = (s + 1) % S;
+            // ---- prefetch the halo words of rows 0 and H+2 (as of the start of
+            // sweep n): their latency overlaps the strip load.  K * L <= NT,
+            // so a thread holds at most two of the 2K halo rows.
+            const int hq0 = lrow, hq1 = lrow + lrows;
+            const bool h0 = S > 1 && lrow < lrows && hq0 < 2 * K;
+            const bool h1 = S > 1 && lrow < lrows && hq1 < 2 * K;
+            auto halo_src = [&](int qr) {
+                const int k = qr >> 1;
+                return (qr & 1) ? halo_ptr(r0 + k, s_dn, 0, n & 1) + lj : halo_ptr(r0 + k, s_up, 1, n & 1) + lj;
+            };
+            uint64_t hw0 = 0, hw1 = 0;
+            if (h0) hw0 = __hip_atomic_load(halo_src(hq0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (h1) hw1 = __hip_atomic_load(halo_src(hq1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (tid < K) {
                 const int r = r0 + tid;
                 if (MODEL == MODEL_ISING)
                     for (int m = 0; m < 5; m++) thr[tid * 5 + m] = ising_threshold(4 * m - 8, pt_beta[r], J);
                 if (MODEL == MODEL_XY_GAUSS) sig[tid] = g.sigmas[pt_w2t[r]];
             }
-            // ---- load rows 1..H (lattice) and H+1 (ghost buffer)
+            // ---- load rows 1..H (lattice) and H+1 (ghost buffer); thread
+            // (lrow, lj) takes column lj of rows 1 + lrow, 1 + lrow + lrows, ...
             if (lrow < lrows)
-                for (int qr = lrow; qr < K * (H + 1); qr += lrows) {
-                    const int k = qr / (H + 1), lr = 1 + qr - k * (H + 1);
-                    const int r = r0 + k;
+                for (int k = 0; k < K; k++) {
                     const StripView v = view_of(k);
-                    size_t gk;
-                    const uint64_t *rsrc;
-                    const void *ssrc;
-                    if (lr <= H) {
-                        gk = (size_t)r * L * L + (size_t)(row0 + lr - 1) * L + lj;
-                        rsrc = g.rng;
-                        ssrc = g.state;
-                    } else {
-                        gk = ((size_t)r * S + s) * L + lj;
-                        rsrc = g.ghost_rng;
-                        ssrc = g.ghost_state;
-                    }
-                    v.rng[lds_index(lr - 1, lj, L)] = reinterpret_cast<const ulonglong2 *>(rsrc)[gk];
-                    if (MODEL == MODEL_ISING) {
-                        v.sp[lds_index(lr, lj, L)] = reinterpret_cast<const int8_t *>(ssrc)[gk];
-                    } else {
-                        const float th = reinterpret_cast<const float *>(ssrc)[gk];
-                        float sv, cv;
-                        fast_sincosf(th, sv, cv);
-                        v.c[lds_index(lr, lj, L)] = cv;
-                        v.s[lds_index(lr, lj, L)] = sv;
-                        v.th[lds_index(lr - 1, lj, L)] = th;
-                    }
-                }
-            __syncthreads();
-
-            // ---- the five passes (the schedule of the resident engine)
-            const StripView pv = view_of(pass_on ? pk : 0);
-            const float pbeta = pass_on ? pt_beta[r0 + pk] : 0.0f;
-            const float psig = MODEL == MODEL_XY_GAUSS && pass_on ? sig[pk] : 0.0f;
-            const float *pthr = thr + 5 * (pass_on ? pk : 0);
-            int n_acc = 0;
-            auto pass = [&](const Seg g0, const Seg g1, const Seg g2, const Seg g3) {
-                if (!pass_on) return;
-                const int n0 = max(g0.hi - g0.lo + 1, 0), n1 = max(g1.hi - g1.lo + 1, 0);
-                const int n2 = max(g2.hi - g2.lo + 1, 0), n3 = max(g3.hi - g3.lo + 1, 0);
-                const int total = n0 + n1 + n2 + n3;
-                for (int qq = prow; qq < total; qq += rpp) {
-                    int k = qq;
-                    Seg sg = g0;
-                    if (k >= n0) {
-                        k -= n0;
-                        sg = g1;
-                        if (k >= n1) {
-                            k -= n1;
-                            sg = g2;
-                            if (k >= n2) {
-                                k -= n2;
-                                sg = g3;
-                            }
-                        }
-                    }
-                    const int ri = sg.lo + k;
-                    const int counted = !((sg.c < 2) & (ri == HH));  // ghost moves are the owner's
-                    if (site_update<MODEL>(pv, sg.c, ri, tcol, L, pbeta, psig, J, delta, pthr)) n_acc += counted;
-                }
-            };
-            const Seg none{0, 0, -1};
-            pass(Seg{0, 1, HH - 1}, none, none, none);
-            __syncthreads();
-            pass(Seg{1, 1, HH - 1}, none, none, none);
-            // receive rows 0 and H+2 as of the start of sweep n (rows H and 2
-            // of this strip itself when S == 1; they are not updated yet)
-            {
-                int bad = 0;
-                if (lrow < lrows)
-                    for (int qr = lrow; qr < 2 * K; qr += lrows) {
-                        const int k = qr >> 1, top = !(qr & 1);
-                        const StripView v = view_of(k);
-                        const int dst_row = top ? 0 : H + 2;
-                        if (S == 1) {
-                            const int src_row = top ? H : 2;
-                            if (MODEL == MODEL_ISING) {
-                                v.sp[lds_index(dst_row, lj, L)] = v.sp[lds_index(src_row, lj, L)];
-                            } else {
-                                v.c[lds_index(dst_row, lj, L)] = v.c[lds_index(src_row, lj, L)];
-                                v.s[lds_index(dst_row, lj, L)] = v.s[lds_index(src_row, lj, L)];
-                            }
-                            continue;
-                        }
-                        const uint64_t *src = top ? halo_ptr(r0 + k, s_up, 1, n & 1) : halo_ptr(r0 + k, s_dn, 0, n & 1);
-                        uint32_t payload;
-                        if (!poll_word(src + lj, (unsigned)n + 1u, payload, a.error)) {
-                            bad = 1;
-                            break;
-                        }
+                    const size_t rep = (size_t)(r0 + k) * L * L + (size_t)row0 * L + lj;
+                    const auto rng_src = as_global<RngRec>(g.rng) + rep;
+                    const size_t gh = ((size_t)(r0 + k) * S + s) * L + lj;
+#pragma unroll 2
+                    for (int lr = 1 + lrow; lr <= H + 1; lr += lrows) {
+                        const int off = (lr - 1) * L;
+                        const bool ghost = lr > H;
+                        const ulonglong2 rec =
+                            load_rec(ghost ? as_global<RngRec>(g.ghost_rng) + gh : rng_src + off);
+                        v.rng[off + ljc] = rec;
                         if (MODEL == MODEL_ISING) {
-                            v.sp[lds_index(dst_row, lj, L)] = (int8_t)(uint8_t)payload;
+                            const int8_t sp = ghost ? as_global<int8_t>(g.ghost_state)[gh]
+                                                    : as_global<int8_t>(g.state)[rep + off];
+                            v.sp[off + L + ljc] = sp;
                         } else {
+                            const float th = ghost ? as_global<float>(g.ghost_state)[gh]
+                                                   : as_global<float>(g.state)[rep + off];
                             float sv, cv;
-                            fast_sincosf(__uint_as_float(payload), sv, cv);
-                            v.c[lds_index(dst_row, lj, L)] = cv;
-                            v.s[lds_index(dst_row, lj, L)] = sv;
+                            fast_sincosf(th, sv, cv);
+                            v.c[off + L + ljc] = cv;
+                            v.s[off + L + ljc] = sv;
+                            v.th[off + ljc] = th;
                         }
                     }
+                }
+            // ---- rows 0 and H+2: the prefetched words (polled again only if
+            // a neighbour was late), or rows H and 2 of this strip itself
+            // when it is a whole replica (S == 1)
+            {
+                auto put = [&](int qr, uint32_t payload) {
+                    const StripView v = view_of(qr >> 1);
+                    const int row = (qr & 1) ? H + 2 : 0;
+                    if (MODEL == MODEL_ISING) {
+                        v.sp[lds_index(row, lj, L)] = (int8_t)(uint8_t)payload;
+                    } else {
+                        float sv, cv;
+                        fast_sincosf(__uint_as_float(payload), sv, cv);
+                        v.c[lds_index(row, lj, L)] = cv;
+                        v.s[lds_index(row, lj, L)] = sv;
+                    }
+                };
+                int bad = 0;
+                const unsigned tag = (unsigned)n + 1u;
+                if (h0) {
+                    uint32_t payload = (uint32_t)hw0;
+                    if ((unsigned)(hw0 >> 32) != tag && !poll_word(halo_src(hq0), tag, payload, a.error)) bad = 1;
+                    put(hq0, payload);
+                }
+                if (h1) {
+                    uint32_t payload = (uint32_t)hw1;
+                    if ((unsigned)(hw1 >> 32) != tag && !poll_word(halo_src(hq1), tag, payload, a.error)) bad = 1;
+                    put(hq1, payload);
+                }
                 if (__syncthreads_or(bad)) {
                     if (tid == 0) atomicOr(a.error, BATCH_ERR_WATCHDOG);
                     failed = true;
                     break;
                 }
+                if (S == 1 && lrow < lrows)
+                    for (int qr = lrow; qr < 2 * K; qr += lrows) {
+                        const StripView v = view_of(qr >> 1);
+                        const int dst = (qr & 1) ? H + 2 : 0, src = (qr & 1) ? 2 : H;
+                        if (MODEL == MODEL_ISING) {
+                            v.sp[lds_index(dst, lj, L)] = v.sp[lds_index(src, lj, L)];
+                        } else {
+                            v.c[lds_index(dst, lj, L)] = v.c[lds_index(src, lj, L)];
+                            v.s[lds_index(dst, lj, L)] = v.s[lds_index(src, lj, L)];
+                        }
+                    }
+                __syncthreads();
             }
-            pass(Seg{0, 0, 0}, Seg{0, HH, HH}, Seg{2, 1, HH - 2}, none);
-            __syncthreads();
-            pass(Seg{1, 0, 0}, Seg{1, HH, HH}, Seg{2, 0, 0}, Seg{2, HH - 1, HH - 1});
-            __syncthreads();
-            pass(Seg{3, 0, HH - 1}, none, none, none);
+
+            // ---- colours 0, 1, 2, 3 in the reference order; colours 0/1
+            // include the ghost row (colour row HH), whose moves are its
+            // owner's and are not counted
+            const StripView pv = view_of(pass_on ? pk : 0);
+            const float pbeta = pass_on ? pt_beta[r0 + pk] : 0.0f;
+            const float psig = MODEL == MODEL_XY_GAUSS && pass_on ? sig[pk] : 0.0f;
+            const float *pthr = thr + 5 * (pass_on ? pk : 0);
+            int n_acc = 0;
+            for (int c = 0; c < 4; c++) {
+                if (pass_on) {
+                    const int nrows = c < 2 ? HH + 1 : HH;
+                    for (int ri = prow; ri < nrows; ri += rpp) {
+                        const int counted = !((c < 2) & (ri == HH));
+                        if (site_update<MODEL>(pv, c, ri, tcol, L, pbeta, psig, J, delta, pthr)) n_acc += counted;
+                    }
+                }
+                __syncthreads();
+            }
             if (pass_on && n_acc) atomicAdd(&moves[r0 + pk], (unsigned)n_acc);
-            __syncthreads();
 
             // ---- publish rows 2 and H (tag n + 2, slot (n + 1) & 1); write back
             if (lrow < lrows) {
@@ -329,27 +356,29 @@ __device__ void pack_run(const PackArgs &a, const PackGroup &g, const PackWork &
                                                      : __float_as_uint(v.th[lds_index(lr - 1, lj, L)]);
                         publish_word(halo_ptr(r0 + k, s, which, (n + 1) & 1) + lj, (unsigned)n + 2u, payload);
                     }
-                for (int qr = lrow; qr < K * (H + 1); qr += lrows) {
-                    const int k = qr / (H + 1), lr = 1 + qr - k * (H + 1);
-                    const int r = r0 + k;
+                for (int k = 0; k < K; k++) {
                     const StripView v = view_of(k);
-                    size_t gk;
-                    uint64_t *rdst;
-                    void *sdst;
-                    if (lr <= H) {
-                        gk = (size_t)r * L * L + (size_t)(row0 + lr - 1) * L + lj;
-                        rdst = g.rng;
-                        sdst = g.state;
-                    } else {
-                        gk = ((size_t)r * S + s) * L + lj;
-                        rdst = g.ghost_rng;
-                        sdst = g.ghost_state;
+                    const size_t rep = (size_t)(r0 + k) * L * L + (size_t)row0 * L + lj;
+                    const auto rng_dst = as_global<RngRec>(g.rng) + rep;
+                    const size_t gh = ((size_t)(r0 + k) * S + s) * L + lj;
+#pragma unroll 2
+                    for (int lr = 1 + lrow; lr <= H + 1; lr += lrows) {
+                        const int off = (lr - 1) * L;
+                        const ulonglong2 rec = v.rng[off + ljc];
+                        if (lr > H) {
+                            store_rec(as_global<RngRec>(g.ghost_rng) + gh, rec);
+                            if (MODEL == MODEL_ISING)
+                                as_global<int8_t>(g.ghost_state)[gh] = v.sp[off + L + ljc];
+                            else
+                                as_global<float>(g.ghost_state)[gh] = v.th[off + ljc];
+                        } else {
+                            store_rec(rng_dst + off, rec);
+                            if (MODEL == MODEL_ISING)
+                                as_global<int8_t>(g.state)[rep + off] = v.sp[off + L + ljc];
+                            else
+                                as_global<float>(g.state)[rep + off] = v.th[off + ljc];
+                        }
                     }
-                    reinterpret_cast<ulonglong2 *>(rdst)[gk] = v.rng[lds_index(lr - 1, lj, L)];
-                    if (MODEL == MODEL_ISING)
-                        reinterpret_cast<int8_t *>(sdst)[gk] = v.sp[lds_index(lr, lj, L)];
-                    else
-                        reinterpret_cast<float *>(sdst)[gk] = v.th[lds_index(lr - 1, lj, L)];
                 }
             }
 
@@ -513,10 +542,11 @@ __global__ __launch_bounds__(NT) void pack_kernel(PackArgs a) {
     __shared__ long long red_e[NT / 16];
     __shared__ double red[4][NT / 16];
     const PackWork w = a.work[blockIdx.x];
-    const PackGroup g = a.groups[w.group];
-    if (g.model == MODEL_ISING)
+    const PackGroup *g = a.groups + w.group;
+    const int model = g->model;
+    if (model == MODEL_ISING)
         pack_run<MODEL_ISING, NT>(a, g, w, smem, red_e, red);
-    else if (g.model == MODEL_XY_FAST)
+    else if (model == MODEL_XY_FAST)
         pack_run<MODEL_XY_FAST, NT>(a, g, w, smem, red_e, red);
     else
         pack_run<MODEL_XY_GAUSS, NT>(a, g, w, smem, red_e, red);
@@ -599,6 +629,7 @@ static int pack_check_group(int threads, int model, int L, int R, int H, int K) 
     MC_REQUIRE(K >= 1 && K <= threads / 16 && (K & (K - 1)) == 0 && R % K == 0,
                "replicas per task K=%d must be a power of two <= %d dividing R=%d", K, threads / 16, R);
     MC_REQUIRE((threads / (L / 2)) / K >= 1, "K=%d strips of L=%d do not fit a %d-thread pass", K, L, threads);
+    MC_REQUIRE(K * L <= threads, "K=%d replicas of L=%d exceed one %d-thread row map", K, L, threads);
     MC_REQUIRE((long long)L * L <= (1LL << 20), "at most 2^20 sites per replica");
     return MC_OK;
 }
@@ -695,7 +726,9 @@ int mc_pack_lds_bytes(int model, int L, int R, int H, int K, size_t *bytes) {
 
 int mc_pack_occupancy(int threads, size_t lds_bytes, int *max_workgroups) {
     MC_REQUIRE(threads == 256 || threads == 512, "workgroup size must be 256 or 512, got %d", threads);
-    if (lds_bytes > 160 * 1024) {
+    hipFuncAttributes attr;
+    MC_TRY_HIP(hipFuncGetAttributes(&attr, pack_kernel_ptr(threads)));
+    if (lds_bytes + attr.sharedSizeBytes > 160 * 1024) {  // dynamic plus static LDS of a CU
         *max_workgroups = 0;
         return MC_OK;
     }
@@ -778,7 +811,7 @@ int mc_pack_add_group(void *pack, int model, int L, int R, int H, int K, int exc
     g.arrive = reinterpret_cast<unsigned *>(static_cast<char *>(gh.acc) + 2 * (size_t)R * sizeof(long long));
     *group_index = (int)p->groups.size();
     p->groups.push_back(gh);
-    p->lds = std::max(p->lds, pack_lds(model, L, R, H, K).total);
+    p->lds = std::max(p->lds, (size_t)pack_lds(model, L, R, H, K).total);
     p->dirty = true;
     return MC_OK;
 }

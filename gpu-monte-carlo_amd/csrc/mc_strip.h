@@ -127,30 +127,99 @@ __device__ void block_sum(long long &e, double (&v)[C], long long *she, double (
     __syncthreads();
 }
 
-// LDS bytes of one strip's site data: RNG streams and angles of rows
-// 1..H+1, cos/sin (or spins) of rows 0..H+2
-__host__ __device__ inline size_t strip_bytes(int model, int L, int H) {
-    const size_t rng = (size_t)(H + 1) * L * 16;
+// LDS bytes of one strip's site data: RNG streams (when staged in LDS) and
+// angles of rows 1..H+1, cos/sin (or spins) of rows 0..H+2
+__host__ __device__ inline size_t strip_bytes(int model, int L, int H, bool rng_in_lds = true) {
+    const size_t rng = rng_in_lds ? (size_t)(H + 1) * L * 16 : 0;
     return model == MODEL_ISING ? rng + (size_t)(H + 3) * L
                                 : rng + (size_t)(H + 3) * L * 8 + (size_t)(H + 1) * L * 4;
 }
 
 // LDS arrays of one strip
 struct StripView {
-    ulonglong2 *rng;  // rows 1..H+1
+    ulonglong2 *rng;  // rows 1..H+1 (when staged in LDS)
     float *c, *s;     // rows 0..H+2 (XY)
     float *th;        // rows 1..H+1 (XY)
     int8_t *sp;       // rows 0..H+2 (Ising)
 };
 
-__device__ __forceinline__ StripView strip_view(unsigned char *base, int L, int H) {
+__device__ __forceinline__ StripView strip_view(unsigned char *base, int L, int H, bool rng_in_lds = true) {
     StripView v;
     v.rng = reinterpret_cast<ulonglong2 *>(base);
-    v.c = reinterpret_cast<float *>(v.rng + (size_t)(H + 1) * L);
+    unsigned char *site = base + (rng_in_lds ? (size_t)(H + 1) * L * 16 : 0);
+    v.c = reinterpret_cast<float *>(site);
     v.s = v.c + (H + 3) * L;
     v.th = v.s + (H + 3) * L;
-    v.sp = reinterpret_cast<int8_t *>(v.rng + (size_t)(H + 1) * L);
+    v.sp = reinterpret_cast<int8_t *>(site);
     return v;
+}
+
+// A site's draws for one sweep, taken from its stream before the sweep's
+// colour passes (they do not depend on the lattice): Ising a = u; XY a = u1
+// (proposal), b = u2 (acceptance); Gaussian XY a = z, b = v (gauss_draw).
+struct SiteDraw {
+    float a, b;
+};
+
+template <int MODEL>
+__device__ __forceinline__ SiteDraw site_draw(Rng &g) {
+    SiteDraw d;
+    if (MODEL == MODEL_XY_GAUSS) {
+        const GaussDraw gd = gauss_draw(g.next());
+        d.a = gd.z;
+        d.b = gd.v;
+    } else if (MODEL == MODEL_XY_FAST) {
+        d.a = g.uniform_f32();
+        d.b = g.uniform_f32();
+    } else {
+        d.a = g.uniform_f32();
+        d.b = 0.0f;
+    }
+    return d;
+}
+
+// The update of site_update (below) with the site's draws given and its LDS
+// positions precomputed: self (cos/sin or spin arrays), lf and rt (left and
+// right neighbours); up and down are self -/+ L, the angle is at self - L.
+template <int MODEL>
+__device__ __forceinline__ bool site_apply(const StripView &v, const int self, const int lf, const int rt,
+                                           const int L, const float beta, const float sigma, const float J,
+                                           const float delta, const float *thr, const SiteDraw d) {
+    const int up = self - L, dn = self + L, own = self - L;
+    if (MODEL == MODEL_ISING) {
+        const int sv = v.sp[self];
+        const int nb = (int)v.sp[up] + (int)v.sp[dn] + (int)v.sp[lf] + (int)v.sp[rt];
+        const bool acc = d.a < thr[(2 * sv * nb + 8) >> 2];
+        if (acc) v.sp[self] = (int8_t)(-sv);
+        return acc;
+    } else {
+        const float th = v.th[own];
+        float pr;
+        if (MODEL == MODEL_XY_GAUSS) {
+            pr = wrap_turns(th + sigma * d.a);
+        } else {
+            const float pi = 3.14159274101257324f, two_pi = 6.28318548202514648f;
+            pr = th + delta * (2.0f * d.a - 1.0f);
+            if (pr >= pi)
+                pr -= two_pi;
+            else if (pr < -pi)
+                pr += two_pi;
+        }
+        float sp, cp;
+        fast_sincosf(pr, sp, cp);
+        const float Cs = ((v.c[up] + v.c[dn]) + v.c[lf]) + v.c[rt];
+        const float Ss = ((v.s[up] + v.s[dn]) + v.s[lf]) + v.s[rt];
+        float t = (cp - v.c[self]) * Cs;
+        t = fmaf(sp - v.s[self], Ss, t);
+        const float dE = -J * t;
+        const bool acc = d.b < accept_weight(-beta * dE);  // dE <= 0 or b < exp(-beta dE)
+        if (acc) {
+            v.th[own] = pr;
+            v.c[self] = cp;
+            v.s[self] = sp;
+        }
+        return acc;
+    }
 }
 
 // Update site (colour row ri, column 2 tcol + (c & 1)) of colour c of a

@@ -1,8 +1,9 @@
 """FP32 batched sweeps vs FP64 plain sweeps on the XY benchmark configuration.
 
 Two independent chains of the same physics:
-* float32: the batched engine (float32 angles and sweep arithmetic, per-thread
-  compensated float32 energy sums, float64 totals across threads and strips);
+* float32: the batched engine (float32 angles and sweep arithmetic; energies
+  are exact integer sums of the float32 bond terms, each rounded to a
+  multiple of 2^-40);
 * float64: the drop-in path (Runtime + ParallelTempering + Simulation) with
   float64 angles, betas, uniforms and sums.
 Both use Gaussian-MH proposals with the same per-temperature widths (adapted

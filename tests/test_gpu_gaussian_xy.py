@@ -101,7 +101,7 @@ def _fast_reference(theta, c, s, rng, betas, sig_walker, nsweeps, acc_walker):
 
 
 @pytest.mark.parametrize("L,R,H,threads", [(16, 3, 16, 256), (32, 4, 8, 256), (64, 2, 4, 256),
-                                           (128, 2, 16, 512)])
+                                           (128, 2, 16, 512), (128, 2, 32, 512)])
 def test_batched_gaussian_trajectory_matches_oracle(gpu, L, R, H, threads):
     from core.batched import BatchedRun
     from models.xy.model import XYModel

@@ -22,7 +22,7 @@ def test_shapes_respect_the_kernel_constraints():
             for H, K in group_shapes(L, 48, threads):
                 assert L % H == 0 and H % 2 == 0 and H >= 4
                 assert 48 % K == 0 and K & (K - 1) == 0 and K <= threads // 16
-                assert (threads // (L // 2)) // K >= 1
+                assert (threads // (L // 2)) // K >= 1 and K * L <= threads
     assert group_shapes(1024, 48, 512) == []  # L > threads
 
 
