@@ -136,6 +136,11 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     double *pt_e = reinterpret_cast<double *>(pt_base + ((size_t)(3 * R * 4 + 7) & ~(size_t)7));  // R
     uint64_t *pt_g = reinterpret_cast<uint64_t *>(pt_e + R);             // 2 (R-1)
     uint32_t *pt_part = reinterpret_cast<uint32_t *>(pt_g + 2 * (R - 1));  // R S (lo, hi)
+    // round-trip records and swap counters of the walkers and pairs this
+    // workgroup keeps (index mod grid size), held in LDS for the launch
+    long long *pt_trips = reinterpret_cast<long long *>(
+        reinterpret_cast<unsigned char *>(pt_part) + (((size_t)R * S * 2 * 4 + 7) & ~(size_t)7));  // 5 R
+    unsigned *pt_cnt = reinterpret_cast<unsigned *>(pt_trips + 5 * R);                            // 3 (R-1)
     __shared__ double red[kChannels][NT / 64];
     __shared__ long long red_e[NT / 64];
     __shared__ float thr[5];
@@ -168,6 +173,8 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             pt_w2t[k] = a.w2t[k];
         }
         for (int k = tid; k < 2 * (R - 1); k += NT) pt_g[k] = a.pt_rng[k];
+        for (int k = tid; k < 5 * R; k += NT) pt_trips[k] = a.trips[k];
+        for (int k = tid; k < 3 * (R - 1); k += NT) pt_cnt[k] = 0;
     }
     // this walker's beta, ladder temperature and proposal width (the same in
     // every thread of the workgroup)
@@ -459,9 +466,9 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
                     pt_beta[wb] = tb;
                 }
                 if ((unsigned)left % nwg == blockIdx.x) {
-                    a.counts[left] += 1;
-                    if (!finite) a.counts[2 * P + left] += 1;
-                    if (accept) a.counts[P + left] += 1;
+                    pt_cnt[left] += 1;
+                    if (!finite) pt_cnt[2 * P + left] += 1;
+                    if (accept) pt_cnt[P + left] += 1;
                 }
             }
             __syncthreads();
@@ -470,17 +477,17 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             for (int w = blockIdx.x + tid * (int)nwg; w < R; w += NT * (int)nwg) {
                 const int t = pt_w2t[w];
                 if (t != 0 && t != R - 1) continue;
-                const long long origin = a.trips[w];
+                const long long origin = pt_trips[w];
                 if (origin < 0) {
-                    a.trips[w] = t;
-                    a.trips[2 * R + w] = sweep_global;
+                    pt_trips[w] = t;
+                    pt_trips[2 * R + w] = sweep_global;
                 } else if (t != origin) {
-                    a.trips[R + w] = 1;
-                } else if (a.trips[R + w]) {
-                    a.trips[3 * R + w] += 1;
-                    a.trips[4 * R + w] += sweep_global - a.trips[2 * R + w];
-                    a.trips[2 * R + w] = sweep_global;
-                    a.trips[R + w] = 0;
+                    pt_trips[R + w] = 1;
+                } else if (pt_trips[R + w]) {
+                    pt_trips[3 * R + w] += 1;
+                    pt_trips[4 * R + w] += sweep_global - pt_trips[2 * R + w];
+                    pt_trips[2 * R + w] = sweep_global;
+                    pt_trips[R + w] = 0;
                 }
             }
             if (R > 2) parity = 1 - parity;
@@ -516,6 +523,13 @@ done:
             if (exchanged) a.energies[k] = pt_e[k];
         }
         for (int k = tid; k < 2 * (R - 1); k += NT) a.pt_rng[k] = pt_g[k];
+    }
+    if (use_pt) {  // the kept walkers' records and pairs' counters
+        const int P = R - 1;
+        for (int w = blockIdx.x + tid * (int)nwg; w < R; w += NT * (int)nwg)
+            for (int row = 0; row < 5; row++) a.trips[row * R + w] = pt_trips[row * R + w];
+        for (int left = blockIdx.x + tid * (int)nwg; left < P; left += NT * (int)nwg)
+            for (int row = 0; row < 3; row++) a.counts[row * P + left] += pt_cnt[row * P + left];
     }
     // ---- write back own rows' state and the streams of owned sites
     __syncthreads();
@@ -555,8 +569,9 @@ static size_t halo_words(int L, int R, int H) { return (size_t)R * (L / H) * 2 *
 static size_t energy_words(int L, int R, int H) { return (size_t)2 * R * (L / H) * 2; }
 
 static size_t batch_lds(int model, int L, int R, int H, int exchange) {
-    const size_t pt_bytes =
-        exchange ? (size_t)3 * R * 4 + 8 + (size_t)R * 8 + (size_t)2 * R * 8 + (size_t)R * (L / H) * 2 * 4 : 0;
+    const size_t pt_bytes = exchange ? (size_t)3 * R * 4 + 8 + (size_t)R * 8 + (size_t)2 * R * 8 +
+                                           (size_t)R * (L / H) * 2 * 4 + 8 + (size_t)5 * R * 8 + (size_t)3 * R * 4
+                                     : 0;
     return ((strip_bytes(model, L, H, false) + 15) & ~(size_t)15) + pt_bytes + 64;
 }
 

@@ -117,7 +117,7 @@ struct PackArgs {
 
 // Dynamic LDS of a workgroup of a group: K strips, then the group's PT state
 struct PackLds {
-    unsigned stride, beta, w2t, t2w, thr, sig, moves, e, g, prev, total;
+    unsigned stride, beta, w2t, t2w, thr, sig, moves, cnt, e, g, prev, trips, total;
 };
 
 __host__ __device__ inline PackLds pack_lds(int model, int L, int R, int H, int K) {
@@ -136,6 +136,8 @@ __host__ __device__ inline PackLds pack_lds(int model, int L, int R, int H, int 
     p += 4u * K;
     o.moves = p;
     p += 4u * R;
+    o.cnt = p;
+    p += 12u * (R > 1 ? R - 1 : 1);
     p = (p + 7) & ~7u;
     o.e = p;
     p += 8u * R;
@@ -143,6 +145,8 @@ __host__ __device__ inline PackLds pack_lds(int model, int L, int R, int H, int 
     p += 16u * (R > 1 ? R - 1 : 1);
     o.prev = p;
     p += 16u * R;
+    o.trips = p;
+    p += 40u * R;
     o.total = p;
     return o;
 }
@@ -170,6 +174,10 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
     double *pt_e = reinterpret_cast<double *>(smem + lay.e);
     uint64_t *pt_g = reinterpret_cast<uint64_t *>(smem + lay.g);
     long long *prev = reinterpret_cast<long long *>(smem + lay.prev);  // [2][R]
+    // round-trip records and swap counters of the walkers and pairs this
+    // workgroup keeps (index mod the group's workgroups), in LDS for the launch
+    long long *pt_trips = reinterpret_cast<long long *>(smem + lay.trips);  // [5][R]
+    unsigned *pt_cnt = reinterpret_cast<unsigned *>(smem + lay.cnt);       // [3][R-1]
 
     for (int k = tid; k < R; k += NT) {
         pt_beta[k] = g.betas[k];
@@ -179,8 +187,11 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
         prev[k] = 0;
         prev[R + k] = 0;
     }
-    if (use_pt)
+    if (use_pt) {
         for (int k = tid; k < 2 * (R - 1); k += NT) pt_g[k] = g.pt_rng[k];
+        for (int k = tid; k < 5 * R; k += NT) pt_trips[k] = g.trips[k];
+        for (int k = tid; k < 3 * (R - 1); k += NT) pt_cnt[k] = 0;
+    }
     int parity = use_pt ? *g.parity : 0;
     const long long sweep0 = a.clock[1];
     bool exchanged = false;
@@ -491,26 +502,26 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
                     pt_beta[wb] = tb;
                 }
                 if (left % g.wg_count == lw) {
-                    g.counts[left] += 1;
-                    if (!finite) g.counts[2 * P + left] += 1;
-                    if (accept) g.counts[P + left] += 1;
+                    pt_cnt[left] += 1;
+                    if (!finite) pt_cnt[2 * P + left] += 1;
+                    if (accept) pt_cnt[P + left] += 1;
                 }
             }
             __syncthreads();
             for (int wk = lw + tid * g.wg_count; wk < R; wk += NT * g.wg_count) {
                 const int tt = pt_w2t[wk];
                 if (tt != 0 && tt != R - 1) continue;
-                const long long origin = g.trips[wk];
+                const long long origin = pt_trips[wk];
                 if (origin < 0) {
-                    g.trips[wk] = tt;
-                    g.trips[2 * R + wk] = sweep_global;
+                    pt_trips[wk] = tt;
+                    pt_trips[2 * R + wk] = sweep_global;
                 } else if (tt != origin) {
-                    g.trips[R + wk] = 1;
-                } else if (g.trips[R + wk]) {
-                    g.trips[3 * R + wk] += 1;
-                    g.trips[4 * R + wk] += sweep_global - g.trips[2 * R + wk];
-                    g.trips[2 * R + wk] = sweep_global;
-                    g.trips[R + wk] = 0;
+                    pt_trips[R + wk] = 1;
+                } else if (pt_trips[R + wk]) {
+                    pt_trips[3 * R + wk] += 1;
+                    pt_trips[4 * R + wk] += sweep_global - pt_trips[2 * R + wk];
+                    pt_trips[2 * R + wk] = sweep_global;
+                    pt_trips[R + wk] = 0;
                 }
             }
             if (R > 2) parity = 1 - parity;
@@ -532,6 +543,13 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
         }
         for (int k = tid; k < 2 * (R - 1); k += NT) g.pt_rng[k] = pt_g[k];
         if (tid == 0) *g.parity = parity;
+    }
+    if (use_pt) {  // the kept walkers' records and pairs' counters
+        const int P = R - 1;
+        for (int wk = lw + tid * g.wg_count; wk < R; wk += NT * g.wg_count)
+            for (int row = 0; row < 5; row++) g.trips[row * R + wk] = pt_trips[row * R + wk];
+        for (int left = lw + tid * g.wg_count; left < P; left += NT * g.wg_count)
+            for (int row = 0; row < 3; row++) g.counts[row * P + left] += pt_cnt[row * P + left];
     }
     if (blockIdx.x == 0 && tid == 0) a.clock[0] = sweep0 + a.nsweeps;
 }
