@@ -24,6 +24,12 @@ every 10 sweeps.  One step = 10 sweeps + 1 exchange.
 * cpu_baseline: the CPU oracle's restatement of the same Gaussian update
   (float32, OpenMP over replicas) on the same lattice, a bounded number of
   sweeps on the host cores.
+* fss_packed: the finite-size-scaling set in one packed launch
+  (core.packed.PackedRun): Ising L in {16, 24, 32, 48, 64, 96, 128, 256} with
+  48 temperatures 2.1-2.45 and XY (uniform window) L in {16, 32, 64, 128, 256}
+  with 48 temperatures 0.8-1.0, exchanges every 10 sweeps, 8.96 M
+  site-updates per sweep; its own roofline (34 B per Ising and 40 B per XY
+  site-update).
 
 Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build measures one GPU.
 """
@@ -48,6 +54,10 @@ ADAPT_SWEEPS, ADAPT_INTERVAL = 2000, 100
 BYTES_PER_SITE_UPDATE = 40          # XY float32: 2 * 4 B angle + 2 * 16 B RNG state
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = REPO / "profiles" / "batch_kernel_xy_gauss_L128_R32.traffic.json"
+FSS_ISING_L = (16, 24, 32, 48, 64, 96, 128, 256)
+FSS_XY_L = (16, 32, 64, 128, 256)
+FSS_R = 48
+FSS_CONFIG = (512, 2048)            # workgroup size, task sites (autotuned on MI355X)
 
 
 def temperatures():
@@ -133,6 +143,56 @@ def bench_plain(dtype, sweeps, sigmas, warmup=20):
     return {"wall_s": wall, "site_updates": sweeps * L * L * R, "sweeps": sweeps}
 
 
+def fss_ensembles():
+    from core.batched import Ensemble
+    from models.ising.model import IsingModel
+    from models.xy.model import XYModel
+    ising_T = np.linspace(2.1, 2.45, FSS_R)
+    xy_T = np.linspace(0.8, 1.0, FSS_R)
+    ens = [Ensemble(IsingModel(start="random"), n, ising_T, seed=42 + n, exchange_interval=EXCHANGE, pt_seed=43 + n)
+           for n in FSS_ISING_L]
+    ens += [Ensemble(XYModel(start="random"), n, xy_T, seed=42 + n, exchange_interval=EXCHANGE, pt_seed=43 + n)
+            for n in FSS_XY_L]
+    return ens
+
+
+def bench_fss(sweeps, warmup=50, config=FSS_CONFIG):
+    """The finite-size-scaling set in one packed launch (HIP-event and wall time)."""
+    from core.packed import PackedRun
+    from mcgpu.timing import EventTimer
+    ens = fss_ensembles()
+    pr = PackedRun(ens, threads=config[0], task_sites=config[1])
+    sites = sum(e.L * e.L * e.R for e in ens)
+    ising_sites = sum(e.L * e.L * e.R for e in ens if e.code == 0)
+    log(f"packed finite-size set: {len(ens)} ensembles, {sites} sites, {pr.workgroups} workgroups of "
+        f"{pr.threads} threads, {pr.plan['tasks_per_workgroup']} tasks each")
+    pr.run(warmup)
+    pr.synchronize()
+    timer = EventTimer(pr._stream)
+    t0 = time.perf_counter()
+    timer.start()
+    handle = pr.launch(sweeps)
+    timer.stop()
+    pr.collect(handle)
+    wall = time.perf_counter() - t0
+    kernel_ms = timer.elapsed_ms()
+    algo = sweeps * (34 * ising_sites + 40 * (sites - ising_sites))
+    achieved = algo / (kernel_ms * 1e-3) / 1e9
+    rate = sites * sweeps / (wall * 1e9)
+    log(f"packed finite-size set: {rate:.2f} site-updates/ns wall, {kernel_ms / sweeps * 1e3:.2f} us/sweep in-kernel")
+    return {
+        "value": round(rate, 4), "unit": "site-updates/ns",
+        "kernel_site_updates_per_ns": round(sites * sweeps / (kernel_ms * 1e6), 3),
+        "us_per_sweep": round(kernel_ms * 1e3 / sweeps, 3),
+        "sweeps": sweeps, "site_updates_per_sweep": sites,
+        "workgroups": pr.workgroups, "workgroup_threads": pr.threads, "task_sites": pr.task_sites,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "algorithmic_bytes": algo},
+        "workload": (f"Ising L={list(FSS_ISING_L)} x {FSS_R} T (2.1-2.45) and XY L={list(FSS_XY_L)} x {FSS_R} "
+                     f"T (0.8-1.0), random starts, exchange every {EXCHANGE} sweeps, one packed launch"),
+    }
+
+
 def copy_probe(mib):
     """One device-to-device copy of `mib` MiB: reads and writes exactly mib MiB."""
     from mcgpu import _lib, as_pointer, device_zeros, synchronize
@@ -202,6 +262,8 @@ def main():
     ap.add_argument("--strip-rows", type=int, default=None)
     ap.add_argument("--threads", type=int, default=None, help="workgroup size with --strip-rows (256 or 512)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--fss-sweeps", type=int, default=1000, help="timed sweeps of the packed finite-size set")
+    ap.add_argument("--no-fss", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device once, so a counter run holds a "
                          "plain copy kernel of known size to scale its byte counters by")
@@ -283,6 +345,8 @@ def main():
         out["plain_fp32"] = {"value": round(v32, 4), "unit": "site-updates/ns",
                              "ms_per_sweep": round(p32["wall_s"] * 1e3 / p32["sweeps"], 4)}
         out["speedup_vs_plain_fp64"] = round(value / v64, 2)
+    if not args.no_fss:
+        out["fss_packed"] = bench_fss(args.fss_sweeps)
     if not args.no_cpu:
         log("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, b["sigmas"])
