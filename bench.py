@@ -30,6 +30,11 @@ every 10 sweeps.  One step = 10 sweeps + 1 exchange.
   with 48 temperatures 0.8-1.0, exchanges every 10 sweeps, 8.96 M
   site-updates per sweep; its own roofline (34 B per Ising and 40 B per XY
   site-update).
+* films: the drop-in path of the Kawamura model (L = 192) and of the He3
+  Gaussian sampler (128 x 128), 64 replicas each with on-device exchanges
+  every 10 sweeps, in float32 and float64 (main/precision_models.py holds
+  their ladders and parameters); algorithmic bytes 2 x state + 32 per
+  site-update (Kawamura 72 / 112 B, He3 128 / 224 B).
 
 Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build measures one GPU.
 """
@@ -193,6 +198,33 @@ def bench_fss(sweeps, warmup=50, config=FSS_CONFIG):
     }
 
 
+def bench_film(kind, precise, sweeps, warmup=20):
+    """Drop-in path throughput of a film model with on-device PT every 10 sweeps."""
+    from core.parallel_tempering import ParallelTempering
+    from main.precision_models import runtime, temperatures
+    from mcgpu import synchronize
+    from mcgpu.timing import EventTimer
+    R = 64
+    L = 192 if kind == "kawamura" else 128
+    rt = runtime(kind, L, temperatures(kind, R), precise, 42 + L)
+    pt = ParallelTempering(rt, exchange_interval=EXCHANGE, seed=43 + L)
+    for _ in range(warmup):
+        pt.sweep()
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(sweeps):
+        pt.sweep()
+    synchronize()
+    wall = time.perf_counter() - t0
+    state_bytes = {("kawamura", False): 20, ("kawamura", True): 40, ("he3", False): 48, ("he3", True): 96}
+    b = 2 * state_bytes[(kind, precise)] + 32
+    rate = L * L * R * sweeps / (wall * 1e9)
+    return {"value": round(rate, 4), "unit": "site-updates/ns", "L": L, "replicas": R, "sweeps": sweeps,
+            "dtype": ("f64" if precise else "f32") if kind == "kawamura" else ("c128" if precise else "c64"),
+            "ms_per_sweep": round(wall * 1e3 / sweeps, 4), "bytes_per_site_update": b,
+            "roofline_frac": round(rate * b / HBM_PEAK_GBS, 4)}
+
+
 def copy_probe(mib):
     """One device-to-device copy of `mib` MiB: reads and writes exactly mib MiB."""
     from mcgpu import _lib, as_pointer, device_zeros, synchronize
@@ -264,6 +296,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--fss-sweeps", type=int, default=1000, help="timed sweeps of the packed finite-size set")
     ap.add_argument("--no-fss", action="store_true")
+    ap.add_argument("--film-sweeps", type=int, default=200, help="timed sweeps of each film-model path")
+    ap.add_argument("--no-films", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device once, so a counter run holds a "
                          "plain copy kernel of known size to scale its byte counters by")
@@ -347,6 +381,14 @@ def main():
         out["speedup_vs_plain_fp64"] = round(value / v64, 2)
     if not args.no_fss:
         out["fss_packed"] = bench_fss(args.fss_sweeps)
+    if not args.no_films:
+        films = {}
+        for kind in ("kawamura", "he3"):
+            for precise in (False, True):
+                r = bench_film(kind, precise, args.film_sweeps)
+                log(f"{kind} {r['dtype']}: {r['value']:.2f} site-updates/ns, {r['ms_per_sweep']} ms per sweep")
+                films[f"{kind}_{r['dtype']}"] = r
+        out["films"] = films
     if not args.no_cpu:
         log("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, b["sigmas"])
