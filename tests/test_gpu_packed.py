@@ -190,3 +190,17 @@ def test_finite_size_set_in_one_launch(gpu):
     np.testing.assert_array_equal(xy.snapshot(), theta)
     np.testing.assert_array_equal(xy.betas.copy_to_host(), betas)
     np.testing.assert_array_equal(xy.diagnostics()["accepted"], counts[1])
+
+
+def test_fss_study_small(gpu):
+    """The finite-size-scaling comparison on small sizes: Ising samples of the
+    packed launch equal the plain path's; XY helicity and Binder curves and
+    their crossings agree within errors."""
+    from main.fss_study import run
+    rep = run(ising_sizes=(8, 12), xy_sizes=(8, 16), R=8, thermalization=2000, measurement=10000, interval=10)
+    assert rep["ising_samples_identical"]
+    for key, v in rep["xy"]["z"].items():
+        if "mean_z2" in v:
+            assert v["mean_z2"] < 3.0 and v["max_abs_z"] < 4.5, (key, v)
+        else:
+            assert abs(v["z"]) < 4.5, (key, v)

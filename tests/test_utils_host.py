@@ -119,3 +119,17 @@ def test_weber_minnhagen_fit_recovers_generated_parameters():
     assert fit["temperature"] == pytest.approx(Tc, abs=1e-4)
     assert fit["C"] == pytest.approx(C, abs=1e-2)
     assert fit["chi2"] < 1e-6
+
+
+def test_crossing_jackknife_recovers_the_crossing_of_noisy_lines():
+    from main.fss_study import _crossing_jackknife
+    T = np.linspace(0.0, 1.0, 11)
+    rng = np.random.default_rng(3)
+    B = 40
+    a = {"y": 2 * T[None, :] - 0.7 + 0.01 * rng.normal(size=(B, 11))}
+    b = {"y": 0.3 + 0.0 * T[None, :] + 0.01 * rng.normal(size=(B, 11))}
+    c, err = _crossing_jackknife({"a": a, "b": b}, lambda m: (m["a"]["y"], m["b"]["y"]), T)
+    assert abs(c - 0.5) < 5 * err and 0.0005 < err < 0.01   # exact crossing 0.5; error ~ 0.01 / 2 / sqrt(40)
+    exact = {"y": np.repeat(2 * T[None, :] - 0.7, B, axis=0)}
+    c, err = _crossing_jackknife({"a": exact}, lambda m: (m["a"]["y"], 0.3), T)
+    assert c == pytest.approx(0.5) and err == pytest.approx(0.0, abs=1e-12)
