@@ -27,6 +27,7 @@ concurrently, one stream each, and returns the reference's run dictionaries
 """
 
 import ctypes
+import sys
 import operator
 import time
 
@@ -430,7 +431,7 @@ def autotune(model, L, temperatures, seed=42, exchange_interval=10, sweeps=300, 
         dt = time.perf_counter() - t0
         if verbose:
             print(f"  H={H:4d} threads={threads:4d} strips={L // H:4d} {dt / sweeps * 1e6:9.2f} us/sweep",
-                  flush=True)
+                  file=sys.stderr, flush=True)
         if best_t is None or dt < best_t:
             best, best_t = (H, threads), dt
     _tuned[key] = best

@@ -20,6 +20,7 @@ ensembles.
 """
 
 import ctypes
+import sys
 import math
 import time
 import weakref
@@ -336,7 +337,7 @@ def autotune_pack(specs, sweeps=60, candidates=None, verbose=False, seed=1):
         dt = (time.perf_counter() - t0) / sweeps
         if verbose:
             print(f"  threads={threads:4d} task_sites={sites:6d} workgroups={pr.workgroups:5d} "
-                  f"{dt * 1e6:9.2f} us/sweep", flush=True)
+                  f"{dt * 1e6:9.2f} us/sweep", file=sys.stderr, flush=True)
         if best_t is None or dt < best_t:
             best, best_t = (threads, sites), dt
         del pr  # frees the pack; the scratch ensembles are free to join the next one

@@ -28,11 +28,19 @@
 // trip per hand-off, no flags.
 //
 // Strips of one replica are placed on one XCD.  Every `exchange_interval`
-// sweeps each strip publishes its energy partial as two tagged words (float
-// hi + lo, about 48 bits); every workgroup polls all R*S partials into LDS,
-// sums each replica's strips in strip order and evaluates the same exchange
-// matching on identical inputs (identical decisions, no grid barrier); only
-// workgroup 0 writes the PT state back to global memory.
+// sweeps each strip publishes its integer energy partial as two tagged words
+// (low and high halves) under its walker's ladder temperature.  A pair's
+// decision depends only on the two walkers' energies, the two ladder betas and
+// the pair's own PT stream, so each workgroup gathers just its own walker's
+// and its partner's partials and decides that pair; the strips of both walkers
+// reach the same decision, and nothing waits for the other pairs (no grid
+// barrier).  Every workgroup advances every attempted pair's stream by one
+// draw (energies are integer sums here, always finite), so the streams stay
+// identical everywhere.  The energy words form a ring of four exchanges: a
+// walker can be at most two exchanges past a partner that has not read its
+// words yet, because the same two ladder slots pair up again two exchanges
+// later.  Each walker's first strip keeps its round-trip record and counts
+// the pairs it is the left member of; the PT streams go back from workgroup 0.
 //
 // Every wait is bounded: past a watchdog count a workgroup records an error
 // and leaves, so a broken invariant ends the launch instead of hanging it.
@@ -80,7 +88,7 @@ struct BatchArgs {
     // synchronisation
     uint64_t *halo;             // [R][S][2 (row 2, row H)][2 (sweep parity)][L] words
                                 //   (sweep << 32 | payload)
-    uint64_t *energy;           // [2 (exchange parity)][R][S][2 (lo, hi)] words
+    uint64_t *energy;           // [4 (exchange mod 4)][R (ladder temperature)][S][2 (lo, hi)] words
                                 //   (exchange tag << 32 | half of the int64 energy partial)
     int *error;
 };
@@ -109,6 +117,13 @@ __device__ __forceinline__ bool pass_item(const int p, const int q, const int HH
     }
 }
 
+// Sum of S strip partials stored as (lo, hi) halves
+__device__ __forceinline__ long long strip_total(const uint32_t *w, int S) {
+    long long t = 0;
+    for (int q = 0; q < S; q++) t += (long long)(((unsigned long long)w[2 * q + 1] << 32) | w[2 * q]);
+    return t;
+}
+
 template <int MODEL, int NT, int MAXP>
 __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -130,17 +145,12 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     float *lth = view.th;   // rows 1..H+1
     int8_t *lsp = view.sp;  // rows 0..H+2 (Ising)
     unsigned char *pt_base = smem + ((strip_bytes(MODEL, L, H, false) + 15) & ~(size_t)15);
-    float *pt_beta = reinterpret_cast<float *>(pt_base);                 // R
-    int32_t *pt_t2w = reinterpret_cast<int32_t *>(pt_beta + R);          // R
-    int32_t *pt_w2t = pt_t2w + R;                                         // R
-    double *pt_e = reinterpret_cast<double *>(pt_base + ((size_t)(3 * R * 4 + 7) & ~(size_t)7));  // R
-    uint64_t *pt_g = reinterpret_cast<uint64_t *>(pt_e + R);             // 2 (R-1)
-    uint32_t *pt_part = reinterpret_cast<uint32_t *>(pt_g + 2 * (R - 1));  // R S (lo, hi)
-    // round-trip records and swap counters of the walkers and pairs this
-    // workgroup keeps (index mod grid size), held in LDS for the launch
-    long long *pt_trips = reinterpret_cast<long long *>(
-        reinterpret_cast<unsigned char *>(pt_part) + (((size_t)R * S * 2 * 4 + 7) & ~(size_t)7));  // 5 R
-    unsigned *pt_cnt = reinterpret_cast<unsigned *>(pt_trips + 5 * R);                            // 3 (R-1)
+    uint64_t *pt_g = reinterpret_cast<uint64_t *>(pt_base);        // 2 (R-1): PT streams
+    float *ladder = reinterpret_cast<float *>(pt_g + 2 * (R - 1));  // R: beta of each ladder temperature
+    float *sig_tab = ladder + R;                                    // R: proposal width of each temperature
+    uint32_t *pt_part = reinterpret_cast<uint32_t *>(sig_tab + R);  // 2 (walker, partner) x S x (lo, hi)
+    __shared__ long long my_trips[5];  // this walker's round-trip record (strip 0)
+    __shared__ int x_acc;              // the exchange decision of this walker's pair
     __shared__ double red[kChannels][NT / 64];
     __shared__ long long red_e[NT / 64];
     __shared__ float thr[5];
@@ -167,23 +177,21 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
         }
     }
     if (use_pt) {
-        for (int k = tid; k < R; k += NT) {
-            pt_beta[k] = a.betas[k];
-            pt_t2w[k] = a.t2w[k];
-            pt_w2t[k] = a.w2t[k];
-        }
+        for (int k = tid; k < R; k += NT) ladder[k] = a.betas[a.t2w[k]];
         for (int k = tid; k < 2 * (R - 1); k += NT) pt_g[k] = a.pt_rng[k];
-        for (int k = tid; k < 5 * R; k += NT) pt_trips[k] = a.trips[k];
-        for (int k = tid; k < 3 * (R - 1); k += NT) pt_cnt[k] = 0;
+        if (s == 0 && tid < 5) my_trips[tid] = a.trips[tid * R + r];
     }
     // this walker's beta, ladder temperature and proposal width (the same in
     // every thread of the workgroup)
     float my_beta = a.betas[r];
     int my_t = use_pt ? a.w2t[r] : r;
+    if (MODEL == MODEL_XY_GAUSS)
+        for (int k = tid; k < R; k += NT) sig_tab[k] = a.sigmas[k];
     float sigma = MODEL == MODEL_XY_GAUSS ? a.sigmas[my_t] : 0.0f;
     if (MODEL == MODEL_ISING && tid < 5) thr[tid] = ising_threshold(4 * tid - 8, my_beta, a.J);
     int parity = use_pt ? *a.parity : 0;
-    bool exchanged = false;  // pt_e holds this launch's last exchange energies
+    bool exchanged = false;  // my_e holds this walker's energy at this launch's last exchange
+    double my_e = 0.0;       // (thread 0)
 
     const float J = a.J, delta = a.delta;
     int n_acc = 0;  // accepted moves of this thread since the last flush
@@ -194,10 +202,21 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     // ---- the thread's sites: LDS positions and RNG streams in registers for
     // the whole launch (the ghost row's streams are a copy of its owner's)
     constexpr int NP = 5;
+    // The halo words of sweep n are loaded once, early in the draw phase
+    // (after the draws of pass 0's sites), and consumed after two colour
+    // passes; only a word that had not arrived yet is polled again.
+    // Measured on MI355X (XY L=128 R=32, H=16, 512 threads), the position in
+    // the draw phase: none 108, after pass 0's draws 121, after all draws 119
+    // site-updates/ns.
+    constexpr int kPrefetchPass = 1;
     int slot_self[NP][MAXP], slot_lf[NP][MAXP], slot_rt[NP][MAXP];
     bool slot_on[NP][MAXP], slot_own[NP][MAXP];
     uint64_t slot_s0[NP][MAXP], slot_s1[NP][MAXP];
     SiteDraw slot_draw[NP][MAXP];
+    // XY: the site's angle and (cos, sin), which only this thread changes, and
+    // the sweep's proposal (angle, cos, sin) formed in the draw phase
+    float slot_th[NP][MAXP], slot_c[NP][MAXP], slot_s[NP][MAXP];
+    float slot_pr[NP][MAXP], slot_pc[NP][MAXP], slot_ps[NP][MAXP], slot_dc[NP][MAXP], slot_ds[NP][MAXP];
     auto slot_global = [&](int c, int ri) {
         const int lr = 1 + 2 * ri + (c >> 1);
         return rep + (size_t)((row0 + lr - 1) % L) * L + 2 * tcol + (c & 1);
@@ -221,16 +240,50 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
                 slot_s1[p][m] = a.rng[2 * gk + 1];
             }
             slot_draw[p][m] = SiteDraw{0.0f, 0.0f};
+            slot_pr[p][m] = slot_pc[p][m] = slot_ps[p][m] = slot_dc[p][m] = slot_ds[p][m] = 0.0f;
         }
     __syncthreads();
+    if (MODEL != MODEL_ISING) {
+#pragma unroll
+        for (int p = 0; p < NP; p++)
+#pragma unroll
+            for (int m = 0; m < MAXP; m++) {
+                const int k = slot_on[p][m] ? slot_self[p][m] : L;
+                slot_th[p][m] = lth[k - L];
+                slot_c[p][m] = lc[k];
+                slot_s[p][m] = ls[k];
+            }
+    }
 
     // The sites of pass p with this sweep's draws
     auto apply = [&](const int p, const float beta) {
 #pragma unroll
-        for (int m = 0; m < MAXP; m++)
-            if (slot_on[p][m] && site_apply<MODEL>(view, slot_self[p][m], slot_lf[p][m], slot_rt[p][m], L, beta,
-                                                   sigma, J, delta, thr, slot_draw[p][m]))
-                n_acc += slot_own[p][m];
+        for (int m = 0; m < MAXP; m++) {
+            if (!slot_on[p][m]) continue;
+            if (MODEL == MODEL_ISING) {
+                if (site_apply<MODEL>(view, slot_self[p][m], slot_lf[p][m], slot_rt[p][m], L, beta, sigma, J, delta,
+                                      thr, slot_draw[p][m]))
+                    n_acc += slot_own[p][m];
+            } else {
+                const int self = slot_self[p][m], lf = slot_lf[p][m], rt = slot_rt[p][m];
+                const int up = self - L, dn = self + L;
+                const float Cs = ((lc[up] + lc[dn]) + lc[lf]) + lc[rt];
+                const float Ss = ((ls[up] + ls[dn]) + ls[lf]) + ls[rt];
+                float t = slot_dc[p][m] * Cs;
+                t = fmaf(slot_ds[p][m], Ss, t);
+                const float dE = -J * t;
+                if (slot_draw[p][m].b < accept_weight(-beta * dE)) {
+                    const float pr = slot_pr[p][m];
+                    lth[up] = pr;
+                    lc[self] = slot_pc[p][m];
+                    ls[self] = slot_ps[p][m];
+                    slot_th[p][m] = pr;
+                    slot_c[p][m] = slot_pc[p][m];
+                    slot_s[p][m] = slot_ps[p][m];
+                    n_acc += slot_own[p][m];
+                }
+            }
+        }
     };
     auto halo_ptr = [&](int ss, int which, int par) {
         return a.halo + ((((size_t)r * S + ss) * 2 + which) * 2 + par) * L;
@@ -249,6 +302,14 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     // strip s+1 as of the end of sweep n-1 (tag n).  With one strip they are
     // this strip's rows H and 2, which still hold sweep n-1 when this runs.
     // Ends with a workgroup barrier; false if the watchdog expired.
+    uint64_t pre = 0;  // this thread's halo word (k = tid), loaded ahead of the receive
+    auto prefetch = [&](int n) {
+        if (S == 1 || n == 0 || tid >= 2 * L) return;
+        const bool top = tid < L;
+        const int j = top ? tid : tid - L;
+        const uint64_t *src = top ? halo_ptr(s_up, 1, (n - 1) & 1) : halo_ptr(s_dn, 0, (n - 1) & 1);
+        pre = __hip_atomic_load(const_cast<uint64_t *>(src + j), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
     auto receive = [&](int n) -> bool {
         if (S == 1) {
             for (int k = tid; k < 2 * L; k += NT) {
@@ -276,7 +337,9 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             const int j = top ? k : k - L;
             const uint64_t *src = top ? halo_ptr(s_up, 1, par) : halo_ptr(s_dn, 0, par);
             uint32_t payload;
-            if (!poll_word(src + j, (unsigned)n, payload, a.error)) {
+            if (k == tid && (unsigned)(pre >> 32) == (unsigned)n) {
+                payload = (uint32_t)pre;
+            } else if (!poll_word(src + j, (unsigned)n, payload, a.error)) {
                 bad = 1;
                 break;
             }
@@ -347,17 +410,41 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
         const long long sweep_global = a.sweep0 + n + 1;
         const float beta = my_beta;
 #pragma unroll
-        for (int p = 0; p < NP; p++)
+        for (int p = 0; p < NP; p++) {
+            if (p == kPrefetchPass) prefetch(n);
 #pragma unroll
             for (int m = 0; m < MAXP; m++)
                 if (slot_on[p][m]) {
                     Rng g;
                     g.s0 = slot_s0[p][m];
                     g.s1 = slot_s1[p][m];
-                    slot_draw[p][m] = site_draw<MODEL>(g);
+                    const SiteDraw d = site_draw<MODEL>(g);
+                    slot_draw[p][m] = d;
                     slot_s0[p][m] = g.s0;
                     slot_s1[p][m] = g.s1;
+                    if (MODEL != MODEL_ISING) {
+                        const float th = slot_th[p][m];
+                        float pr;
+                        if (MODEL == MODEL_XY_GAUSS) {
+                            pr = wrap_turns(th + sigma * d.a);
+                        } else {
+                            const float pi = 3.14159274101257324f, two_pi = 6.28318548202514648f;
+                            pr = th + delta * (2.0f * d.a - 1.0f);
+                            if (pr >= pi)
+                                pr -= two_pi;
+                            else if (pr < -pi)
+                                pr += two_pi;
+                        }
+                        float sp, cp;
+                        fast_sincosf(pr, sp, cp);
+                        slot_pr[p][m] = pr;
+                        slot_pc[p][m] = cp;
+                        slot_ps[p][m] = sp;
+                        slot_dc[p][m] = cp - slot_c[p][m];
+                        slot_ds[p][m] = sp - slot_s[p][m];
+                    }
                 }
+        }
         apply(0, beta);
         __syncthreads();
         apply(1, beta);
@@ -406,18 +493,27 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             exchanged = true;
             const long long xi = sweep_global / a.exchange_interval;
             const unsigned tag = (unsigned)xi + 1u;
-            uint64_t *ew = a.energy + (size_t)(xi & 1) * R * S * 2;
+            // energy words by ladder temperature, a ring of four exchanges
+            uint64_t *ring = a.energy + (size_t)(xi & 3) * R * S * 2;
             if (tid == 0) {
-                uint64_t *dst = ew + ((size_t)r * S + s) * 2;
+                uint64_t *dst = ring + ((size_t)my_t * S + s) * 2;
                 publish_word(dst, tag, (uint32_t)(unsigned long long)ef);
                 publish_word(dst + 1, tag, (uint32_t)((unsigned long long)ef >> 32));
             }
-            // every workgroup gathers every strip's partial (the same values
-            // everywhere) and adds each replica's strips as integers
+            // this walker's pair in the matching of parity `parity`: left
+            // temperatures parity, parity + 2, ... below R - 1
+            // (core/parallel_tempering.py:44-69)
+            const int rel = my_t - parity;
+            const int partner = (rel >= 0 && !(rel & 1) && my_t < R - 1) ? my_t + 1
+                                : (rel > 0 && (rel & 1)) ? my_t - 1 : -1;
+            const int my_left = partner < 0 ? -1 : (partner < my_t ? partner : my_t);
+            // gather the strips of this walker and of its partner only
             int bad = 0;
-            for (int k = tid; k < R * S * 2; k += NT) {
+            const int nw = partner >= 0 ? 4 * S : 2 * S;
+            for (int k = tid; k < nw; k += NT) {
+                const int t = k < 2 * S ? my_t : partner;
                 uint32_t payload;
-                if (!poll_word(ew + k, tag, payload, a.error)) {
+                if (!poll_word(ring + (size_t)t * S * 2 + (k % (2 * S)), tag, payload, a.error)) {
                     bad = 1;
                     break;
                 }
@@ -427,73 +523,52 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
                 if (tid == 0) atomicOr(a.error, BATCH_ERR_WATCHDOG);
                 goto done;
             }
-            for (int k = tid; k < R; k += NT) {
-                long long tot = 0;
-                for (int q = 0; q < S; q++) {
-                    const uint32_t *pp = pt_part + ((size_t)k * S + q) * 2;
-                    tot += (long long)(((unsigned long long)pp[1] << 32) | pp[0]);
-                }
-                pt_e[k] = energy_from_fixed(MODEL, J, tot);
-            }
-            __syncthreads();
-            // one matching of parity `parity` (core/parallel_tempering.py:44-69);
-            // every workgroup evaluates all pairs on identical inputs, and the
-            // counters of pair `left` are kept by workgroup left mod grid
-            const int P = R - 1;
+            // every attempted pair's stream takes one draw (energies are
+            // integer sums here, always finite); this walker's pair decides
             for (int left = parity + 2 * tid; left < R - 1; left += 2 * NT) {
-                const int right = left + 1;
-                const int wa = pt_t2w[left], wb = pt_t2w[right];
-                const double ea = pt_e[wa], eb = pt_e[wb];
-                const bool finite = isfinite(ea) && isfinite(eb);
-                bool accept = false;
-                if (finite) {
-                    const double lr = ((double)pt_beta[wa] - (double)pt_beta[wb]) * (ea - eb);
-                    Rng g;
-                    g.s0 = pt_g[2 * left];
-                    g.s1 = pt_g[2 * left + 1];
-                    const double u = g.uniform_f64();
-                    pt_g[2 * left] = g.s0;
-                    pt_g[2 * left + 1] = g.s1;
-                    accept = lr >= 0.0 || u == 0.0 || ::log(u) < lr;
-                }
-                if (accept) {
-                    pt_t2w[left] = wb;
-                    pt_t2w[right] = wa;
-                    pt_w2t[wa] = right;
-                    pt_w2t[wb] = left;
-                    const float tb = pt_beta[wa];
-                    pt_beta[wa] = pt_beta[wb];
-                    pt_beta[wb] = tb;
-                }
-                if ((unsigned)left % nwg == blockIdx.x) {
-                    pt_cnt[left] += 1;
-                    if (!finite) pt_cnt[2 * P + left] += 1;
-                    if (accept) pt_cnt[P + left] += 1;
+                Rng g;
+                g.s0 = pt_g[2 * left];
+                g.s1 = pt_g[2 * left + 1];
+                const double u = g.uniform_f64();
+                pt_g[2 * left] = g.s0;
+                pt_g[2 * left + 1] = g.s1;
+                if (left == my_left) {
+                    const double e0 = energy_from_fixed(MODEL, J, strip_total(pt_part, S));
+                    const double e1 = energy_from_fixed(MODEL, J, strip_total(pt_part + 2 * S, S));
+                    const double ea = my_t == left ? e0 : e1, eb = my_t == left ? e1 : e0;
+                    const double lr = ((double)ladder[left] - (double)ladder[left + 1]) * (ea - eb);
+                    x_acc = (lr >= 0.0 || u == 0.0 || ::log(u) < lr) ? 1 : 0;
                 }
             }
+            if (tid == 0) {
+                if (my_left < 0) x_acc = 0;
+                my_e = energy_from_fixed(MODEL, J, strip_total(pt_part, S));
+            }
             __syncthreads();
-            // round-trip bookkeeping (core/parallel_tempering.py:71-91), walker w
-            // kept by workgroup w mod grid
-            for (int w = blockIdx.x + tid * (int)nwg; w < R; w += NT * (int)nwg) {
-                const int t = pt_w2t[w];
-                if (t != 0 && t != R - 1) continue;
-                const long long origin = pt_trips[w];
+            const bool acc = x_acc != 0;
+            if (s == 0 && tid == 0 && my_left == my_t) {  // the left walker's strip 0 counts the pair
+                atomicAdd(reinterpret_cast<unsigned long long *>(a.counts) + my_left, 1ull);
+                if (acc) atomicAdd(reinterpret_cast<unsigned long long *>(a.counts) + (R - 1) + my_left, 1ull);
+            }
+            if (acc) my_t = partner;
+            my_beta = ladder[my_t];
+            if (MODEL == MODEL_XY_GAUSS) sigma = sig_tab[my_t];
+            // this walker's round trips (core/parallel_tempering.py:71-91)
+            if (s == 0 && tid == 0 && (my_t == 0 || my_t == R - 1)) {
+                const long long origin = my_trips[0];
                 if (origin < 0) {
-                    pt_trips[w] = t;
-                    pt_trips[2 * R + w] = sweep_global;
-                } else if (t != origin) {
-                    pt_trips[R + w] = 1;
-                } else if (pt_trips[R + w]) {
-                    pt_trips[3 * R + w] += 1;
-                    pt_trips[4 * R + w] += sweep_global - pt_trips[2 * R + w];
-                    pt_trips[2 * R + w] = sweep_global;
-                    pt_trips[R + w] = 0;
+                    my_trips[0] = my_t;
+                    my_trips[2] = sweep_global;
+                } else if (my_t != origin) {
+                    my_trips[1] = 1;
+                } else if (my_trips[1]) {
+                    my_trips[3] += 1;
+                    my_trips[4] += sweep_global - my_trips[2];
+                    my_trips[2] = sweep_global;
+                    my_trips[1] = 0;
                 }
             }
             if (R > 2) parity = 1 - parity;
-            my_beta = pt_beta[r];
-            my_t = pt_w2t[r];
-            if (MODEL == MODEL_XY_GAUSS) sigma = a.sigmas[my_t];
             if (MODEL == MODEL_ISING) {
                 if (tid < 5) thr[tid] = ising_threshold(4 * tid - 8, my_beta, a.J);
                 __syncthreads();
@@ -515,21 +590,16 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
 
 done:
     flush_accepted();
-    if (use_pt && blockIdx.x == 0) {  // the PT state after the last exchange
-        for (int k = tid; k < R; k += NT) {
-            a.betas[k] = pt_beta[k];
-            a.t2w[k] = pt_t2w[k];
-            a.w2t[k] = pt_w2t[k];
-            if (exchanged) a.energies[k] = pt_e[k];
+    if (use_pt) {  // the PT state after the last exchange
+        if (s == 0 && tid == 0) {  // this walker's temperature, beta, energy and record
+            a.betas[r] = my_beta;
+            a.w2t[r] = my_t;
+            a.t2w[my_t] = r;
+            if (exchanged) a.energies[r] = my_e;
+            for (int row = 0; row < 5; row++) a.trips[row * R + r] = my_trips[row];
         }
-        for (int k = tid; k < 2 * (R - 1); k += NT) a.pt_rng[k] = pt_g[k];
-    }
-    if (use_pt) {  // the kept walkers' records and pairs' counters
-        const int P = R - 1;
-        for (int w = blockIdx.x + tid * (int)nwg; w < R; w += NT * (int)nwg)
-            for (int row = 0; row < 5; row++) a.trips[row * R + w] = pt_trips[row * R + w];
-        for (int left = blockIdx.x + tid * (int)nwg; left < P; left += NT * (int)nwg)
-            for (int row = 0; row < 3; row++) a.counts[row * P + left] += pt_cnt[row * P + left];
+        if (blockIdx.x == 0)
+            for (int k = tid; k < 2 * (R - 1); k += NT) a.pt_rng[k] = pt_g[k];
     }
     // ---- write back own rows' state and the streams of owned sites
     __syncthreads();
@@ -566,12 +636,11 @@ __global__ void batch_reset_kernel(uint64_t *halo, size_t nhalo, uint64_t *energ
 }
 
 static size_t halo_words(int L, int R, int H) { return (size_t)R * (L / H) * 2 * 2 * L; }
-static size_t energy_words(int L, int R, int H) { return (size_t)2 * R * (L / H) * 2; }
+static size_t energy_words(int L, int R, int H) { return (size_t)4 * R * (L / H) * 2; }
 
 static size_t batch_lds(int model, int L, int R, int H, int exchange) {
-    const size_t pt_bytes = exchange ? (size_t)3 * R * 4 + 8 + (size_t)R * 8 + (size_t)2 * R * 8 +
-                                           (size_t)R * (L / H) * 2 * 4 + 8 + (size_t)5 * R * 8 + (size_t)3 * R * 4
-                                     : 0;
+    (void)exchange;
+    const size_t pt_bytes = (size_t)16 * (R - 1) + (size_t)8 * R + (size_t)16 * (L / H);
     return ((strip_bytes(model, L, H, false) + 15) & ~(size_t)15) + pt_bytes + 64;
 }
 

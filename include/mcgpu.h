@@ -205,7 +205,9 @@ int mc_kawamura_twist_grad(void *stream, int dtype, const void *state, int R, in
  * accepted (R entries per ladder temperature, may be NULL) counts accepted
  * moves.  The state is split into L/H strips of H rows per replica; each
  * workgroup (threads = 256 or 512) keeps its strip on chip.  Workspace sizes
- * come from mc_batch_workspace_size. */
+ * come from mc_batch_workspace_size.  With exchanges, betas must hold the
+ * ladder betas in walker order (betas[w] = ladder[walker_to_temperature[w]]),
+ * as every exchange leaves them; counts are added to in place. */
 int mc_batch_workspace_size(int model, int L, int R, int H, size_t *halo_words,
                             size_t *energy_words);
 /* Largest grid (R * L / H workgroups) that can be co-resident for this
