@@ -6,23 +6,24 @@
 
 namespace mc {
 
-// Four bonds of site (i, j) with a proposed local state; neighbours in the
-// order up, down, left, right.
+// Local energy of site (i, j) from its four bonds (neighbours up, down,
+// left, right) given per bond the neighbour's tau, the spin dot product and
+// cos(theta - theta_n): -sum ((J (1 + tau tau_n)) dot) cos, accumulated in
+// bond order as models/kawamura/update.py:7-23 does.
 template <class R>
-__device__ __forceinline__ R kaw_local(const R *__restrict__ b, int L, int i, int j, R theta, R tau,
-                                       R dx, R dy, R dz, R J) {
-    const int ni[4] = {wrap_up(i, L), wrap_dn(i, L), i, i};
-    const int nj[4] = {j, j, wrap_up(j, L), wrap_dn(j, L)};
+__device__ __forceinline__ R kaw_bonds(const R (&ntau)[4], const R (&dot)[4], const R (&cs)[4], R tau, R J) {
     R energy = R(0);
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const R *n = b + 5 * (ni[k] * L + nj[k]);
-        const R dot = (dx * n[2] + dy * n[3]) + dz * n[4];
-        energy -= ((J * (R(1) + tau * n[1])) * dot) * rcos(theta - n[0]);
-    }
+    for (int k = 0; k < 4; k++) energy -= ((J * (R(1) + tau * ntau[k])) * dot[k]) * cs[k];
     return energy;
 }
 
+// One site visit (models/kawamura/update.py:26-65): theta move, tau flip,
+// director move, each against the energy left by the previous sub-move.  The
+// neighbours are read once; cos(theta - theta_n) and the dot products are
+// computed once per value of theta and of the director and reused by the
+// later sub-moves (the same operations on the same operands as evaluating
+// every bond afresh, so the results are the same).
 template <class R>
 __global__ __launch_bounds__(256) void kawamura_color_kernel(R *__restrict__ st, uint64_t *__restrict__ rng,
                                                              const R *__restrict__ betas, int L,
@@ -40,24 +41,43 @@ __global__ __launch_bounds__(256) void kawamura_color_kernel(R *__restrict__ st,
     const R beta = betas[r];
     R *site = b + 5 * (i * L + j);
     R theta = site[0], tau = site[1], dx = site[2], dy = site[3], dz = site[4];
+    const int ni[4] = {wrap_up(i, L), wrap_dn(i, L), i, i};
+    const int nj[4] = {j, j, wrap_up(j, L), wrap_dn(j, L)};
+    R nth[4], ntau[4], nx[4], ny[4], nz[4], dot[4], cs[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const R *n = b + 5 * (ni[q] * L + nj[q]);
+        nth[q] = n[0];
+        ntau[q] = n[1];
+        nx[q] = n[2];
+        ny[q] = n[3];
+        nz[q] = n[4];
+        dot[q] = (dx * nx[q] + dy * ny[q]) + dz * nz[q];
+        cs[q] = rcos(theta - nth[q]);
+    }
     Rng g = load_rng(rng, k);
-    R energy = kaw_local(b, L, i, j, theta, tau, dx, dy, dz, J);
+    R energy = kaw_bonds(ntau, dot, cs, tau, J);
 
     R pt = theta + delta * (two * g.template uniform<R>() - one);
     if (pt >= pi)
         pt -= two_pi;
     else if (pt < -pi)
         pt += two_pi;
-    R pe = kaw_local(b, L, i, j, pt, tau, dx, dy, dz, J);
+    R pcs[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) pcs[q] = rcos(pt - nth[q]);
+    R pe = kaw_bonds(ntau, dot, pcs, tau, J);
     R dE = pe - energy;
     R u = g.template uniform<R>();
     if (dE <= zero || u < rexp(-beta * dE)) {
         theta = pt;
         energy = pe;
+#pragma unroll
+        for (int q = 0; q < 4; q++) cs[q] = pcs[q];
     }
 
     const R ptau = -tau;
-    pe = kaw_local(b, L, i, j, theta, ptau, dx, dy, dz, J);
+    pe = kaw_bonds(ntau, dot, cs, ptau, J);
     dE = pe - energy;
     u = g.template uniform<R>();
     if (dE <= zero || u < rexp(-beta * dE)) {
@@ -69,7 +89,10 @@ __global__ __launch_bounds__(256) void kawamura_color_kernel(R *__restrict__ st,
     const R phi = two_pi * g.template uniform<R>();
     const R radius = rsqrt_(one - z * z);
     const R x = radius * rcos(phi), y = radius * rsin(phi);
-    pe = kaw_local(b, L, i, j, theta, tau, x, y, z, J);
+    R pdot[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) pdot[q] = (x * nx[q] + y * ny[q]) + z * nz[q];
+    pe = kaw_bonds(ntau, pdot, cs, tau, J);
     dE = pe - energy;
     u = g.template uniform<R>();
     if (dE <= zero || u < rexp(-beta * dE)) {
