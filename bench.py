@@ -30,6 +30,13 @@ every 10 sweeps.  One step = 10 sweeps + 1 exchange.
   with 48 temperatures 0.8-1.0, exchanges every 10 sweeps, 8.96 M
   site-updates per sweep; its own roofline (34 B per Ising and 40 B per XY
   site-update).
+* ising_config1: the Ising configuration (L = 32, 8 temperatures, exchange
+  every sweep, 2,000 + 10,000 sweeps) as a whole run on the drop-in path, the
+  batched engine and the CPU oracle (one thread), with their end states
+  compared.
+* gradients: the differentiable energy (XY and Kawamura, L = 256, 64
+  replicas, float32 and float64), forward + backward per call, against 3 x
+  the state bytes per call.
 * films: the drop-in path of the Kawamura model (L = 192) and of the He3
   Gaussian sampler (128 x 128), 64 replicas each with on-device exchanges
   every 10 sweeps, in float32 and float64 (main/precision_models.py holds
@@ -225,6 +232,120 @@ def bench_film(kind, precise, sweeps, warmup=20):
             "roofline_frac": round(rate * b / HBM_PEAK_GBS, 4)}
 
 
+def bench_ising_config1(therm=2000, meas=10000, interval=10):
+    """The Ising configuration (L = 32, 8 temperatures 2.0-2.6, random start, exchange every sweep,
+    2,000 + 10,000 sweeps, a sample every 10) as a whole run on three paths: the drop-in
+    Simulation over ParallelTempering, the batched engine (one launch), and the CPU oracle
+    (single thread, the same sweeps and exchanges).  The three runs end in identical spins."""
+    import oracle
+    from core.batched import BatchedRun, BatchedSimulation
+    from core.parallel_tempering import ParallelTempering
+    from core.runtime import Runtime
+    from core.simulation import Simulation
+    from mcgpu import synchronize
+    from models.ising.model import IsingModel
+    Li, Ri = 32, 8
+    temps = np.linspace(2.0, 2.6, Ri)
+    seed, pt_seed = 42 + Li, 43
+    sweeps = therm + meas
+    out = {"workload": (f"Ising J=1 L={Li}, {Ri} temperatures 2.0-2.6, random start seed {seed}, exchange every "
+                        f"sweep (PT seed {pt_seed}), {therm} + {meas} sweeps, a sample every {interval}"),
+           "unit": "site-updates/ns", "site_updates": Li * Li * Ri * sweeps}
+    model = IsingModel(start="random")
+    rt = Runtime(model, Li, temps, seed=seed)
+    pt = ParallelTempering(rt, exchange_interval=1, seed=pt_seed)
+    synchronize()
+    t0 = time.perf_counter()
+    Simulation(pt, thermalization_sweeps=therm, measurement_sweeps=meas, sample_interval=interval).run()
+    synchronize()
+    wall = time.perf_counter() - t0
+    out["plain"] = {"value": round(out["site_updates"] / (wall * 1e9), 4), "wall_s": round(wall, 3),
+                    "path": "Simulation + ParallelTempering + Runtime, 4 colour launches per sweep"}
+    br = BatchedRun(IsingModel(start="random"), Li, temps, seed=seed, exchange_interval=1, pt_seed=pt_seed,
+                    strip_rows=8)
+    synchronize()
+    t0 = time.perf_counter()
+    BatchedSimulation([br], thermalization_sweeps=therm, measurement_sweeps=meas, sample_interval=interval).run()
+    wall = time.perf_counter() - t0
+    out["batched"] = {"value": round(out["site_updates"] / (wall * 1e9), 4), "wall_s": round(wall, 3),
+                      "path": f"BatchedSimulation, one launch of {Li // 8 * Ri} workgroups (strips of 8 rows)"}
+    same = np.array_equal(br.snapshot(), rt.snapshot())
+    oracle.set_threads(1)
+    spins = model.initialize_state(Li, Ri, seed).copy()
+    rng = oracle.rng_init(Ri * Li * Li, seed)
+    betas = (1.0 / temps).astype(np.float32)
+    w2t = np.arange(Ri, dtype=np.int32)
+    t2w = np.arange(Ri, dtype=np.int32)
+    counts = np.zeros((3, Ri - 1), np.int64)
+    trips = np.zeros((5, Ri), np.int64)
+    trips[0] = trips[2] = -1
+    trips[0, 0], trips[0, -1] = 0, Ri - 1
+    trips[2, [0, -1]] = 0
+    pt_rng = oracle.rng_init(Ri - 1, pt_seed)
+    parity = 0
+    t0 = time.perf_counter()
+    for sweep in range(1, sweeps + 1):
+        oracle.ising_sweeps(spins, rng, betas, model.J, 1)
+        e = oracle.ising_energy(spins, model.J)
+        oracle.pt_exchange(e, betas, w2t, t2w, pt_rng, counts, trips, parity, sweep)
+        parity = 1 - parity
+        if sweep > therm and (sweep - therm) % interval == 0:
+            oracle.ising_measure(spins, model.J)
+    wall = time.perf_counter() - t0
+    out["cpu_oracle"] = {"value": round(out["site_updates"] / (wall * 1e9), 5), "wall_s": round(wall, 3),
+                         "cores": 1, "kind": "port"}
+    out["end_states_identical"] = bool(same and np.array_equal(spins, rt.snapshot()))
+    log(f"Ising config 1: plain {out['plain']['value']}, batched {out['batched']['value']}, "
+        f"CPU {out['cpu_oracle']['value']} site-updates/ns; identical end states: {out['end_states_identical']}")
+    return out
+
+
+def bench_gradients(calls=20):
+    """Differentiable energy at L = 256, 64 replicas: H(theta; J, twist) forward and dH/dstate,
+    dH/dtwist, dH/dJ backward through the HIP kernels, XY and Kawamura, float32 and float64.
+    Algorithmic bytes per call: the state read by the terms kernel, read again and the gradient
+    written by the gradient kernel (3 x state bytes)."""
+    import torch
+    from autograd import lattice_energy
+    from models.kawamura.model import KawamuraModel
+    from models.xy.model import XYModel
+    Lg, Rg = 256, 64
+    out = {}
+    for name in ("xy", "kawamura"):
+        for dt in (torch.float32, torch.float64):
+            npdt = np.float32 if dt == torch.float32 else np.float64
+            m = XYModel(start="random", dtype=npdt) if name == "xy" else KawamuraModel(dtype=npdt, start="random")
+            host = np.ascontiguousarray(m.initialize_state(Lg, Rg, 7))
+            state = torch.tensor(host, device="cuda")
+            J = torch.tensor(1.0, dtype=torch.float64, device="cuda", requires_grad=True)
+            twist = torch.zeros(Rg, dtype=torch.float64, device="cuda", requires_grad=True)
+
+            def once():
+                x = state.detach().requires_grad_(True)
+                lattice_energy(x, J, twist, name).sum().backward()
+                return x.grad
+
+            for _ in range(3):
+                once()
+            torch.cuda.synchronize()
+            start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            start.record()
+            for _ in range(calls):
+                once()
+            stop.record()
+            torch.cuda.synchronize()
+            ms = start.elapsed_time(stop) / calls
+            nbytes = 3 * state.numel() * state.element_size()
+            key = f"{name}_{'f32' if dt == torch.float32 else 'f64'}"
+            out[key] = {"ms_per_call": round(ms, 4), "GB_per_s": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                        "roofline_frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "algorithmic_bytes": nbytes}
+            log(f"gradients {key}: {ms:.3f} ms per forward+backward, {out[key]['GB_per_s']} GB/s")
+    out["workload"] = (f"lattice_energy(state, J, twist) forward + backward (dH/dstate, dH/dtwist, dH/dJ), L={Lg}, "
+                       f"{Rg} replicas, random angles")
+    return out
+
+
 def copy_probe(mib):
     """One device-to-device copy of `mib` MiB: reads and writes exactly mib MiB."""
     from mcgpu import _lib, as_pointer, device_zeros, synchronize
@@ -298,6 +419,8 @@ def main():
     ap.add_argument("--no-fss", action="store_true")
     ap.add_argument("--film-sweeps", type=int, default=200, help="timed sweeps of each film-model path")
     ap.add_argument("--no-films", action="store_true")
+    ap.add_argument("--no-ising", action="store_true", help="skip the Ising L=32 R=8 whole-run comparison")
+    ap.add_argument("--no-gradients", action="store_true", help="skip the differentiable-energy line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device once, so a counter run holds a "
                          "plain copy kernel of known size to scale its byte counters by")
@@ -389,6 +512,10 @@ def main():
                 log(f"{kind} {r['dtype']}: {r['value']:.2f} site-updates/ns, {r['ms_per_sweep']} ms per sweep")
                 films[f"{kind}_{r['dtype']}"] = r
         out["films"] = films
+    if not args.no_ising:
+        out["ising_config1"] = bench_ising_config1()
+    if not args.no_gradients:
+        out["gradients"] = bench_gradients()
     if not args.no_cpu:
         log("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, b["sigmas"])

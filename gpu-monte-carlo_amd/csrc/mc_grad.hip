@@ -23,15 +23,14 @@ namespace mc {
 
 template <class R>
 __device__ __forceinline__ void sincos_r(R x, R &s, R &c);
+// sine and cosine from one shared argument reduction
 template <>
 __device__ __forceinline__ void sincos_r<float>(float x, float &s, float &c) {
-    s = ::sinf(x);
-    c = ::cosf(x);
+    ::sincosf(x, &s, &c);
 }
 template <>
 __device__ __forceinline__ void sincos_r<double>(double x, double &s, double &c) {
-    s = ::sin(x);
-    c = ::cos(x);
+    ::sincos(x, &s, &c);
 }
 
 __device__ __forceinline__ double twist_of(const double *twist, int r) { return twist ? twist[r] : 0.0; }

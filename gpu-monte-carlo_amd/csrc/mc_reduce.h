@@ -46,10 +46,21 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
     double v[CA];
 #pragma unroll
     for (int c = 0; c < CA; c++) v[c] = 0.0;
-    const int n = L * L;
-    for (int k = b * kReduceThreads + tid; k < n; k += nblk * kReduceThreads) {
-        const int i = k / L, j = k - (k / L) * L;
+    // sites k = b * 256 + tid + m * stride in order, with (i, j) = divmod(k, L)
+    // advanced by the constant step divmod(stride, L) instead of a division
+    // per site
+    const int n = L * L, stride = nblk * kReduceThreads;
+    const int di = stride / L, dj = stride - di * L;
+    int k = b * kReduceThreads + tid;
+    int i = k / L, j = k - i * L;
+    for (; k < n; k += stride) {
         acc.template site<ENERGY_ONLY>(r, i, j, v, use_cache);
+        i += di;
+        j += dj;
+        if (j >= L) {
+            j -= L;
+            i += 1;
+        }
     }
     __shared__ double sh[C][kReduceThreads / 64];
     const int lane = tid & 63, wave = tid >> 6;
