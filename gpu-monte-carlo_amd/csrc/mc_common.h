@@ -38,8 +38,7 @@ int fail_hip(hipError_t err, const char *what);
     } while (0)
 
 // Library-owned device scratch for reductions; grows on demand, never shrinks.
-// Returns MC_OK and sets *ptr, or an error code.  The first `counter_bytes`
-// of the returned region are zero and kernels leave them zero.
+// Returns MC_OK and sets *ptr, or an error code.
 int scratch(size_t bytes, void **ptr);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }

@@ -15,8 +15,14 @@
 //   * terms:    per-replica (Cx, Cy, Sx), deterministic multi-block sums;
 //   * gradient: per-site state gradient of alpha Cx + beta Cy + gamma Sx;
 //   * hvp:      per-site Hessian-vector product of the same combination (XY).
-// One thread per site; its four neighbours come through L1/L2.  Per-site terms
-// are evaluated in the state's precision, per-replica sums in float64.
+//
+// Each workgroup stages a 16 x 32 tile of one replica plus a one-site halo in
+// LDS with coalesced loads, evaluating (cos theta, sin theta) once per staged
+// site; every bond's cos a and sin a then follow from the angle-difference
+// identities (and the twist's rotation by -Delta, its sincos taken once per
+// workgroup), so a site costs about 1.2 sincos instead of one per bond end.
+// Per-site terms are evaluated in the state's precision, per-replica sums in
+// float64.
 #include "mc_reduce.h"
 
 namespace mc {
@@ -35,41 +41,8 @@ __device__ __forceinline__ void sincos_r<double>(double x, double &s, double &c)
 
 __device__ __forceinline__ double twist_of(const double *twist, int r) { return twist ? twist[r] : 0.0; }
 
-// ------------------------------------------------------------------ terms
-
-template <class R, int COMP>
-struct TwistAcc {
-    const R *st;
-    const double *twist;
-    int L;
-    template <bool ENERGY_ONLY, int CA>
-    __device__ __forceinline__ void site(int r, int i, int j, double (&v)[CA], bool) const {
-        const R *b = st + (size_t)r * L * L * COMP;
-        const R *me = b + ((size_t)i * L + j) * COMP;
-        const R *nx = b + ((size_t)wrap_dn(i, L) * L + j) * COMP;
-        const R *ny = b + ((size_t)i * L + wrap_dn(j, L)) * COMP;
-        const R dl = (R)twist_of(twist, r);
-        R wx = 1, wy = 1;
-        if (COMP == 5) {
-            wx = ((R)1 + me[1] * nx[1]) * ((me[2] * nx[2] + me[3] * nx[3]) + me[4] * nx[4]);
-            wy = ((R)1 + me[1] * ny[1]) * ((me[2] * ny[2] + me[3] * ny[3]) + me[4] * ny[4]);
-        }
-        R sx, cx, sy, cy;
-        sincos_r<R>((me[0] - nx[0]) - dl, sx, cx);
-        sincos_r<R>(me[0] - ny[0], sy, cy);
-        v[0] += (double)(wx * cx);
-        v[1] += (double)(wy * cy);
-        v[2] += (double)(wx * sx);
-    }
-    template <int CA>
-    __device__ __forceinline__ void finalize(const double (&t)[CA], double, double *out, int R_,
-                                             int dest) const {
-#pragma unroll
-        for (int c = 0; c < CA; c++) out[(size_t)c * R_ + dest] = t[c];
-    }
-};
-
-// ------------------------------------------------------------- gradient
+constexpr int kTileRows = 16, kTileCols = 32, kTileThreads = 256;
+constexpr int kHaloRows = kTileRows + 2, kHaloCols = kTileCols + 2;
 
 // Per-replica coefficients; NULL means 0.
 struct Coefs {
@@ -81,59 +54,180 @@ struct Coefs {
     }
 };
 
+// LDS image of one tile and its halo: local (li, lj) = global
+// ((i0 + li - 1) mod L, (j0 + lj - 1) mod L); per site (cos theta, sin theta)
+// and, for Kawamura, (tau, dx, dy, dz).
+template <class R, int COMP>
+struct Tile {
+    R c[kHaloRows][kHaloCols], s[kHaloRows][kHaloCols];
+    R x[COMP == 5 ? 4 : 1][kHaloRows][kHaloCols];
+};
+
+template <class R, int COMP>
+__device__ __forceinline__ void stage_tile(Tile<R, COMP> &t, const R *__restrict__ b, int L, int i0, int j0) {
+    for (int q = threadIdx.x; q < kHaloRows * kHaloCols; q += kTileThreads) {
+        const int li = q / kHaloCols, lj = q - (q / kHaloCols) * kHaloCols;
+        int gi = i0 + li - 1, gj = j0 + lj - 1;
+        gi = gi < 0 ? gi + L : gi % L;
+        gj = gj < 0 ? gj + L : gj % L;
+        const R *p = b + ((size_t)gi * L + gj) * COMP;
+        R sv, cv;
+        sincos_r<R>(p[0], sv, cv);
+        t.c[li][lj] = cv;
+        t.s[li][lj] = sv;
+        if (COMP == 5) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) t.x[m][li][lj] = p[1 + m];
+        }
+    }
+}
+
+// Bond from site p to site q (a = theta_p - theta_q), rotated by -dl:
+// (cos(a - dl), sin(a - dl)) from the staged cos/sin and (cd, sd) = (cos dl, sin dl).
 template <class R>
-__global__ __launch_bounds__(256) void xy_grad_kernel(const R *__restrict__ th, int L, const double *twist,
-                                                      Coefs k, R *__restrict__ grad) {
-    const int r = blockIdx.y;
-    const int site = blockIdx.x * 256 + threadIdx.x;
-    if (site >= L * L) return;
-    const int i = site / L, j = site - (site / L) * L;
-    const R *b = th + (size_t)r * L * L;
-    const R t = b[site];
-    const R tdn = b[wrap_dn(i, L) * L + j], tup = b[wrap_up(i, L) * L + j];
-    const R trt = b[i * L + wrap_dn(j, L)], tlf = b[i * L + wrap_up(j, L)];
-    double al, be, ga;
-    k.at(r, al, be, ga);
-    const R dl = (R)twist_of(twist, r);
-    R s1, c1, s2, c2, s3, c3, s4, c4;
-    sincos_r<R>((t - tdn) - dl, s1, c1);  // bond (i,j)-(i+1,j), site first
-    sincos_r<R>((tup - t) - dl, s2, c2);  // bond (i-1,j)-(i,j), site second
-    sincos_r<R>(t - trt, s3, c3);         // bond (i,j)-(i,j+1)
-    sincos_r<R>(tlf - t, s4, c4);         // bond (i,j-1)-(i,j)
-    // dCx/dt = -s1 + s2, dCy/dt = -s3 + s4, dSx/dt = c1 - c2
-    const R g = (R)al * (s2 - s1) + (R)be * (s4 - s3) + (R)ga * (c1 - c2);
-    grad[(size_t)r * L * L + site] = g;
+__device__ __forceinline__ void bond_cs(R cp, R sp, R cq, R sq, R cd, R sd, R &c, R &s) {
+    const R ca = cp * cq + sp * sq, sa = sp * cq - cp * sq;
+    c = ca * cd + sa * sd;
+    s = sa * cd - ca * sd;
 }
 
 template <class R>
-__global__ __launch_bounds__(256) void xy_hvp_kernel(const R *__restrict__ th, int L, const double *twist,
-                                                     Coefs k, const R *__restrict__ u, R *__restrict__ out) {
+__device__ __forceinline__ void twist_cs(const double *twist, int r, R &cd, R &sd) {
+    sincos_r<R>((R)twist_of(twist, r), sd, cd);
+}
+
+// Tiles of a replica: row-major over ceil(L / 16) x ceil(L / 32)
+__device__ __forceinline__ void tile_origin(int L, int &i0, int &j0) {
+    const int tc = (L + kTileCols - 1) / kTileCols;
+    i0 = (blockIdx.x / tc) * kTileRows;
+    j0 = (blockIdx.x - (blockIdx.x / tc) * tc) * kTileCols;
+}
+
+// ------------------------------------------------------------------ terms
+
+// Per-replica (Cx, Cy, Sx): each workgroup sums the down (x) and right (y)
+// bonds of kTermsTiles consecutive tiles (row-major tile order) into a
+// float64 partial per channel; reduce_finish_kernel adds the partials in a
+// fixed order (the same result for the same state every time).
+constexpr int kTermsTiles = 4;
+
+template <class R, int COMP>
+__global__ __launch_bounds__(kTileThreads) void terms_tile_kernel(const R *__restrict__ st, int L,
+                                                                  const double *twist, int ntiles, int nblk,
+                                                                  double *__restrict__ partials) {
+    __shared__ Tile<R, COMP> t;
+    const int r = blockIdx.y, tid = threadIdx.x;
+    const R *b = st + (size_t)r * L * L * COMP;
+    const int tc = (L + kTileCols - 1) / kTileCols;
+    R cd, sd;
+    twist_cs<R>(twist, r, cd, sd);
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int tile = blockIdx.x * kTermsTiles; tile < ntiles && tile < (blockIdx.x + 1) * kTermsTiles; tile++) {
+        const int i0 = (tile / tc) * kTileRows, j0 = (tile - (tile / tc) * tc) * kTileCols;
+        __syncthreads();  // the previous tile's reads are done
+        stage_tile<R, COMP>(t, b, L, i0, j0);
+        __syncthreads();
+        for (int q = tid; q < kTileRows * kTileCols; q += kTileThreads) {
+            const int li = 1 + q / kTileCols, lj = 1 + (q - (q / kTileCols) * kTileCols);
+            if (i0 + li - 1 >= L || j0 + lj - 1 >= L) continue;
+            R wx = 1, wy = 1;
+            if (COMP == 5) {
+                wx = ((R)1 + t.x[0][li][lj] * t.x[0][li + 1][lj]) *
+                     ((t.x[1][li][lj] * t.x[1][li + 1][lj] + t.x[2][li][lj] * t.x[2][li + 1][lj]) +
+                      t.x[3][li][lj] * t.x[3][li + 1][lj]);
+                wy = ((R)1 + t.x[0][li][lj] * t.x[0][li][lj + 1]) *
+                     ((t.x[1][li][lj] * t.x[1][li][lj + 1] + t.x[2][li][lj] * t.x[2][li][lj + 1]) +
+                      t.x[3][li][lj] * t.x[3][li][lj + 1]);
+            }
+            R cx, sx, cy, sy;
+            bond_cs<R>(t.c[li][lj], t.s[li][lj], t.c[li + 1][lj], t.s[li + 1][lj], cd, sd, cx, sx);
+            bond_cs<R>(t.c[li][lj], t.s[li][lj], t.c[li][lj + 1], t.s[li][lj + 1], (R)1, (R)0, cy, sy);
+            v[0] += (double)(wx * cx);
+            v[1] += (double)(wy * cy);
+            v[2] += (double)(wx * sx);
+        }
+    }
+    block_partial<3, kTileThreads>(v, partials + ((size_t)r * nblk + blockIdx.x) * 3);
+}
+
+// The per-replica channels as the outputs (3, R)
+struct TermsOut {
+    template <int CA>
+    __device__ __forceinline__ void finalize(const double (&t)[CA], double, double *out, int R_, int dest) const {
+#pragma unroll
+        for (int c = 0; c < CA; c++) out[(size_t)c * R_ + dest] = t[c];
+    }
+};
+
+// ------------------------------------------------------------- gradient
+
+template <class R>
+__global__ __launch_bounds__(kTileThreads) void xy_grad_kernel(const R *__restrict__ th, int L, const double *twist,
+                                                               Coefs k, R *__restrict__ grad) {
+    __shared__ Tile<R, 1> t;
     const int r = blockIdx.y;
-    const int site = blockIdx.x * 256 + threadIdx.x;
-    if (site >= L * L) return;
-    const int i = site / L, j = site - (site / L) * L;
-    const size_t base = (size_t)r * L * L;
-    const R *b = th + base;
-    const R *v = u + base;
-    const int kdn = wrap_dn(i, L) * L + j, kup = wrap_up(i, L) * L + j;
-    const int krt = i * L + wrap_dn(j, L), klf = i * L + wrap_up(j, L);
-    const R t = b[site];
+    int i0, j0;
+    tile_origin(L, i0, j0);
+    stage_tile<R, 1>(t, th + (size_t)r * L * L, L, i0, j0);
+    R cd, sd;
+    twist_cs<R>(twist, r, cd, sd);
     double al, be, ga;
     k.at(r, al, be, ga);
-    const R dl = (R)twist_of(twist, r);
-    R s1, c1, s2, c2, s3, c3, s4, c4;
-    sincos_r<R>((t - b[kdn]) - dl, s1, c1);
-    sincos_r<R>((b[kup] - t) - dl, s2, c2);
-    sincos_r<R>(t - b[krt], s3, c3);
-    sincos_r<R>(b[klf] - t, s4, c4);
-    const R du_dn = v[site] - v[kdn], du_up = v[site] - v[kup];
-    const R du_rt = v[site] - v[krt], du_lf = v[site] - v[klf];
-    // Hess(Cx) u = -c1 du_dn - c2 du_up ; Hess(Cy) u = -c3 du_rt - c4 du_lf ;
-    // Hess(Sx) u = -s1 du_dn - s2 du_up
-    const R hx = -(c1 * du_dn) - c2 * du_up;
-    const R hy = -(c3 * du_rt) - c4 * du_lf;
-    const R hs = -(s1 * du_dn) - s2 * du_up;
-    out[base + site] = (R)al * hx + (R)be * hy + (R)ga * hs;
+    __syncthreads();
+    for (int q = threadIdx.x; q < kTileRows * kTileCols; q += kTileThreads) {
+        const int li = 1 + q / kTileCols, lj = 1 + (q - (q / kTileCols) * kTileCols);
+        const int i = i0 + li - 1, j = j0 + lj - 1;
+        if (i >= L || j >= L) continue;
+        const R c0 = t.c[li][lj], s0 = t.s[li][lj];
+        R c1, s1, c2, s2, c3, s3, c4, s4;
+        bond_cs<R>(c0, s0, t.c[li + 1][lj], t.s[li + 1][lj], cd, sd, c1, s1);  // (i,j)-(i+1,j), site first
+        bond_cs<R>(t.c[li - 1][lj], t.s[li - 1][lj], c0, s0, cd, sd, c2, s2);  // (i-1,j)-(i,j), site second
+        bond_cs<R>(c0, s0, t.c[li][lj + 1], t.s[li][lj + 1], (R)1, (R)0, c3, s3);  // (i,j)-(i,j+1)
+        bond_cs<R>(t.c[li][lj - 1], t.s[li][lj - 1], c0, s0, (R)1, (R)0, c4, s4);  // (i,j-1)-(i,j)
+        // dCx/dt = -s1 + s2, dCy/dt = -s3 + s4, dSx/dt = c1 - c2
+        const R g = (R)al * (s2 - s1) + (R)be * (s4 - s3) + (R)ga * (c1 - c2);
+        grad[(size_t)r * L * L + (size_t)i * L + j] = g;
+    }
+}
+
+template <class R>
+__global__ __launch_bounds__(kTileThreads) void xy_hvp_kernel(const R *__restrict__ th, int L, const double *twist,
+                                                              Coefs k, const R *__restrict__ u, R *__restrict__ out) {
+    __shared__ Tile<R, 1> t;
+    const int r = blockIdx.y;
+    int i0, j0;
+    tile_origin(L, i0, j0);
+    const size_t base = (size_t)r * L * L;
+    stage_tile<R, 1>(t, th + base, L, i0, j0);
+    R cd, sd;
+    twist_cs<R>(twist, r, cd, sd);
+    double al, be, ga;
+    k.at(r, al, be, ga);
+    const R *v = u + base;
+    __syncthreads();
+    for (int q = threadIdx.x; q < kTileRows * kTileCols; q += kTileThreads) {
+        const int li = 1 + q / kTileCols, lj = 1 + (q - (q / kTileCols) * kTileCols);
+        const int i = i0 + li - 1, j = j0 + lj - 1;
+        if (i >= L || j >= L) continue;
+        const int site = i * L + j;
+        const int kdn = wrap_dn(i, L) * L + j, kup = wrap_up(i, L) * L + j;
+        const int krt = i * L + wrap_dn(j, L), klf = i * L + wrap_up(j, L);
+        const R c0 = t.c[li][lj], s0 = t.s[li][lj];
+        R c1, s1, c2, s2, c3, s3, c4, s4;
+        bond_cs<R>(c0, s0, t.c[li + 1][lj], t.s[li + 1][lj], cd, sd, c1, s1);
+        bond_cs<R>(t.c[li - 1][lj], t.s[li - 1][lj], c0, s0, cd, sd, c2, s2);
+        bond_cs<R>(c0, s0, t.c[li][lj + 1], t.s[li][lj + 1], (R)1, (R)0, c3, s3);
+        bond_cs<R>(t.c[li][lj - 1], t.s[li][lj - 1], c0, s0, (R)1, (R)0, c4, s4);
+        const R vs = v[site];
+        const R du_dn = vs - v[kdn], du_up = vs - v[kup];
+        const R du_rt = vs - v[krt], du_lf = vs - v[klf];
+        // Hess(Cx) u = -c1 du_dn - c2 du_up ; Hess(Cy) u = -c3 du_rt - c4 du_lf ;
+        // Hess(Sx) u = -s1 du_dn - s2 du_up
+        const R hx = -(c1 * du_dn) - c2 * du_up;
+        const R hy = -(c3 * du_rt) - c4 * du_lf;
+        const R hs = -(s1 * du_dn) - s2 * du_up;
+        out[base + site] = (R)al * hx + (R)be * hy + (R)ga * hs;
+    }
 }
 
 // Kawamura: state (theta, tau, dx, dy, dz) per site.  For a bond term
@@ -141,57 +235,83 @@ __global__ __launch_bounds__(256) void xy_hvp_kernel(const R *__restrict__ th, i
 //   d/dtheta_p = w f'(a), d/dtheta_q = -w f'(a),
 //   d/dtau_p = tau_q (d_p . d_q) f(a), d/dd_p = (1 + tau_p tau_q) f(a) d_q.
 template <class R>
-__global__ __launch_bounds__(256) void kawamura_grad_kernel(const R *__restrict__ st, int L, const double *twist,
-                                                            Coefs k, R *__restrict__ grad) {
+__global__ __launch_bounds__(kTileThreads) void kawamura_grad_kernel(const R *__restrict__ st, int L,
+                                                                     const double *twist, Coefs k,
+                                                                     R *__restrict__ grad) {
+    __shared__ Tile<R, 5> t;
     const int r = blockIdx.y;
-    const int site = blockIdx.x * 256 + threadIdx.x;
-    if (site >= L * L) return;
-    const int i = site / L, j = site - (site / L) * L;
-    const R *b = st + (size_t)r * L * L * 5;
-    const R *me = b + (size_t)site * 5;
+    int i0, j0;
+    tile_origin(L, i0, j0);
+    stage_tile<R, 5>(t, st + (size_t)r * L * L * 5, L, i0, j0);
+    R cd, sd;
+    twist_cs<R>(twist, r, cd, sd);
     double al, be, ga;
     k.at(r, al, be, ga);
-    const R dl = (R)twist_of(twist, r);
-    const R t = me[0], tau = me[1], dx = me[2], dy = me[3], dz = me[4];
-    R g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0;
-    // (neighbour, is_x_bond, site_is_first)
-    const int nbr[4] = {wrap_dn(i, L) * L + j, wrap_up(i, L) * L + j, i * L + wrap_dn(j, L), i * L + wrap_up(j, L)};
+    __syncthreads();
+    for (int q = threadIdx.x; q < kTileRows * kTileCols; q += kTileThreads) {
+        const int li = 1 + q / kTileCols, lj = 1 + (q - (q / kTileCols) * kTileCols);
+        const int i = i0 + li - 1, j = j0 + lj - 1;
+        if (i >= L || j >= L) continue;
+        const R c0 = t.c[li][lj], s0 = t.s[li][lj];
+        const R tau = t.x[0][li][lj], dx = t.x[1][li][lj], dy = t.x[2][li][lj], dz = t.x[3][li][lj];
+        R g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0;
+        // neighbours down, up (x bonds), right, left (y bonds); the site is
+        // the bond's first end for down and right
+        const int oi[4] = {li + 1, li - 1, li, li};
+        const int oj[4] = {lj, lj, lj + 1, lj - 1};
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const R *o = b + (size_t)nbr[q] * 5;
-        const bool xb = q < 2, first = (q & 1) == 0;
-        const R a = first ? (t - o[0]) - (xb ? dl : (R)0) : (o[0] - t) - (xb ? dl : (R)0);
-        R s, c;
-        sincos_r<R>(a, s, c);
-        const R dot = (dx * o[2] + dy * o[3]) + dz * o[4];
-        const R tt = (R)1 + tau * o[1];
-        const R w = tt * dot;
-        // f = cos with coefficient (alpha | beta), f = sin with gamma (x bonds only)
-        const R kc = (R)(xb ? al : be), ks = xb ? (R)ga : (R)0;
-        const R f = kc * c + ks * s;       // coefficient-weighted f(a)
-        const R fp = ks * c - kc * s;      // coefficient-weighted f'(a)
-        g0 += first ? w * fp : -(w * fp);
-        g1 += o[1] * dot * f;
-        g2 += tt * f * o[2];
-        g3 += tt * f * o[3];
-        g4 += tt * f * o[4];
+        for (int m = 0; m < 4; m++) {
+            const bool xb = m < 2, first = (m & 1) == 0;
+            const int a = oi[m], b = oj[m];
+            const R cq = t.c[a][b], sq = t.s[a][b];
+            R c, s;
+            if (first)
+                bond_cs<R>(c0, s0, cq, sq, xb ? cd : (R)1, xb ? sd : (R)0, c, s);
+            else
+                bond_cs<R>(cq, sq, c0, s0, xb ? cd : (R)1, xb ? sd : (R)0, c, s);
+            const R otau = t.x[0][a][b], ox = t.x[1][a][b], oy = t.x[2][a][b], oz = t.x[3][a][b];
+            const R dot = (dx * ox + dy * oy) + dz * oz;
+            const R tt = (R)1 + tau * otau;
+            const R w = tt * dot;
+            // f = cos with coefficient (alpha | beta), f = sin with gamma (x bonds only)
+            const R kc = (R)(xb ? al : be), ks = xb ? (R)ga : (R)0;
+            const R f = kc * c + ks * s;   // coefficient-weighted f(a)
+            const R fp = ks * c - kc * s;  // coefficient-weighted f'(a)
+            g0 += first ? w * fp : -(w * fp);
+            g1 += otau * dot * f;
+            g2 += tt * f * ox;
+            g3 += tt * f * oy;
+            g4 += tt * f * oz;
+        }
+        R *o = grad + ((size_t)r * L * L + (size_t)i * L + j) * 5;
+        o[0] = g0;
+        o[1] = g1;
+        o[2] = g2;
+        o[3] = g3;
+        o[4] = g4;
     }
-    R *out = grad + ((size_t)r * L * L + site) * 5;
-    out[0] = g0;
-    out[1] = g1;
-    out[2] = g2;
-    out[3] = g3;
-    out[4] = g4;
 }
+
+static int tiles_of(int L) { return ((L + kTileRows - 1) / kTileRows) * ((L + kTileCols - 1) / kTileCols); }
+static dim3 site_grid(int L, int R_) { return dim3((unsigned)tiles_of(L), (unsigned)R_); }
 
 template <class R, int COMP>
 static int terms_impl(void *stream, const void *state, int R_, int L, const double *twist, double *out) {
     MC_REQUIRE(L >= 2, "invalid lattice size %d", L);
-    return launch_reduce<3, false>(stream, TwistAcc<R, COMP>{(const R *)state, twist, L}, R_, L, nullptr,
-                                   nullptr, out);
+    MC_REQUIRE(R_ >= 1, "invalid replica count %d", R_);
+    const int ntiles = tiles_of(L), nblk = (ntiles + kTermsTiles - 1) / kTermsTiles;
+    void *base = nullptr;
+    int rc = scratch((size_t)R_ * nblk * 3 * sizeof(double), &base);
+    if (rc) return rc;
+    double *partials = reinterpret_cast<double *>(base);
+    hipLaunchKernelGGL((terms_tile_kernel<R, COMP>), dim3((unsigned)nblk, (unsigned)R_), dim3(kTileThreads), 0,
+                       as_stream(stream), (const R *)state, L, twist, ntiles, nblk, partials);
+    MC_CHECK_LAUNCH();
+    hipLaunchKernelGGL((reduce_finish_kernel<3, false, TermsOut>), dim3(R_), dim3(64), 0, as_stream(stream),
+                       TermsOut{}, R_, nblk, nullptr, nullptr, out, partials);
+    MC_CHECK_LAUNCH();
+    return MC_OK;
 }
-
-static dim3 site_grid(int L, int R_) { return dim3((unsigned)(((long)L * L + 255) / 256), (unsigned)R_); }
 
 }  // namespace mc
 

@@ -1,12 +1,14 @@
 // Deterministic per-replica reductions for the measurement and energy kernels.
 //
-// Grid (nblk, R) x 256 threads.  Each block strides over a replica's sites,
-// reduces its float64 channels with wavefront shuffles and LDS, writes one
-// partial per channel, and the last block of each replica (agent-scope
-// acq_rel counter) sums the partials in block order and writes the outputs.
-// The order of every addition is fixed by (L, nblk), so the same state
-// always gives bit-identical results, and an energy-only launch reproduces
-// channel 0 of the full measurement exactly.
+// Two launches.  reduce_kernel: grid (nblk, R) x 256 threads; each block
+// strides over a replica's sites, reduces its float64 channels with wavefront
+// shuffles and LDS and writes one partial per channel.  reduce_finish_kernel:
+// one wavefront per replica adds the partials in a fixed order and writes the
+// outputs.  The kernel boundary orders the partials before their sum, so no
+// block needs a device-scope acquire/release (on gfx950 each of those writes
+// back and invalidates the L2).  The order of every addition is fixed by
+// (L, nblk), so the same state always gives bit-identical results, and an
+// energy-only launch reproduces channel 0 of the full measurement exactly.
 #pragma once
 
 #include "mc_common.h"
@@ -30,19 +32,38 @@ __device__ __forceinline__ double wave_sum(double x) {
     return x;
 }
 
+// One block's C channel sums (v per thread) into partials[0..C-1] (thread 0).
+template <int C, int NT>
+__device__ __forceinline__ void block_partial(const double (&v)[C], double *partials) {
+    __shared__ double sh[C][NT / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        const double s = wave_sum(v[c]);
+        if (lane == 0) sh[c][wave] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            double t = sh[c][0];
+#pragma unroll
+            for (int w = 1; w < NT / 64; w++) t += sh[c][w];
+            partials[c] = t;
+        }
+    }
+}
+
 // Acc must provide:
 //   template <bool ENERGY_ONLY> __device__ void site(int r, int i, int j,
 //        double (&v)[CA], bool cached) const;
 //   __device__ void finalize(const double (&t)[CA], double energy,
 //        double *out, int R, int dest) const;   (full measurement only)
 template <int CA, bool ENERGY_ONLY, class Acc>
-__global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
-    Acc acc, int R, int L, int nblk, const int32_t *__restrict__ tind,
-    const double *__restrict__ cached, double *__restrict__ out,
-    double *__restrict__ partials, unsigned *__restrict__ counters) {
+__global__ __launch_bounds__(kReduceThreads) void reduce_kernel(Acc acc, int L, int nblk, bool use_cache,
+                                                                double *__restrict__ partials) {
     constexpr int C = ENERGY_ONLY ? 1 : CA;
-    const int r = blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
-    const bool use_cache = cached != nullptr;
+    const int r = blockIdx.y, b = blockIdx.x;
     double v[CA];
 #pragma unroll
     for (int c = 0; c < CA; c++) v[c] = 0.0;
@@ -51,7 +72,7 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
     // per site
     const int n = L * L, stride = nblk * kReduceThreads;
     const int di = stride / L, dj = stride - di * L;
-    int k = b * kReduceThreads + tid;
+    int k = b * kReduceThreads + threadIdx.x;
     int i = k / L, j = k - i * L;
     for (; k < n; k += stride) {
         acc.template site<ENERGY_ONLY>(r, i, j, v, use_cache);
@@ -62,39 +83,38 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
             i += 1;
         }
     }
-    __shared__ double sh[C][kReduceThreads / 64];
-    const int lane = tid & 63, wave = tid >> 6;
+    double w[C];
 #pragma unroll
-    for (int c = 0; c < C; c++) {
-        const double s = wave_sum(v[c]);
-        if (lane == 0) sh[c][wave] = s;
-    }
-    __syncthreads();
-    __shared__ bool is_last;
-    if (tid == 0) {
-        double *p = partials + ((size_t)r * nblk + b) * C;
-#pragma unroll
-        for (int c = 0; c < C; c++) p[c] = ((sh[c][0] + sh[c][1]) + sh[c][2]) + sh[c][3];
-        const unsigned prev = __hip_atomic_fetch_add(&counters[r], 1u, __ATOMIC_ACQ_REL,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-        is_last = prev == (unsigned)(nblk - 1);
-    }
-    __syncthreads();
-    if (!is_last || tid != 0) return;
+    for (int c = 0; c < C; c++) w[c] = v[c];
+    block_partial<C, kReduceThreads>(w, partials + ((size_t)r * nblk + b) * C);
+}
+
+// One 64-lane wavefront per replica: lane l adds partials l, l + 64, ... in
+// order, then the lanes are combined by a fixed shuffle tree.
+template <int CA, bool ENERGY_ONLY, class Acc>
+__global__ __launch_bounds__(64) void reduce_finish_kernel(Acc acc, int R, int nblk,
+                                                           const int32_t *__restrict__ tind,
+                                                           const double *__restrict__ cached,
+                                                           double *__restrict__ out,
+                                                           const double *__restrict__ partials) {
+    constexpr int C = ENERGY_ONLY ? 1 : CA;
+    const int r = blockIdx.x, lane = threadIdx.x;
     double t[CA];
 #pragma unroll
     for (int c = 0; c < CA; c++) t[c] = 0.0;
     const double *p = partials + (size_t)r * nblk * C;
-    for (int bb = 0; bb < nblk; bb++) {
+    for (int bb = lane; bb < nblk; bb += 64) {
 #pragma unroll
         for (int c = 0; c < C; c++) t[c] += p[(size_t)bb * C + c];
     }
-    counters[r] = 0u;
+#pragma unroll
+    for (int c = 0; c < C; c++) t[c] = wave_sum(t[c]);
+    if (lane != 0) return;
     if (ENERGY_ONLY) {
         out[r] = t[0];
     } else {
         const int dest = tind ? tind[r] : r;
-        acc.finalize(t, use_cache ? cached[r] : t[0], out, R, dest);
+        acc.finalize(t, cached ? cached[r] : t[0], out, R, dest);
     }
 }
 
@@ -106,19 +126,15 @@ int launch_reduce(void *stream, const Acc &acc, int R, int L, const int32_t *tin
     MC_REQUIRE(L >= 1, "invalid lattice size %d", L);
     constexpr int C = ENERGY_ONLY ? 1 : CA;
     const int nblk = reduce_blocks(L);
-    const size_t counter_bytes = (((size_t)R * sizeof(unsigned)) + 255) & ~(size_t)255;
-    // Counters live at the front of the scratch region and are kept zero.
     void *base = nullptr;
-    const size_t need = (size_t)64 * 1024 * sizeof(unsigned) + (size_t)R * nblk * C * sizeof(double);
-    MC_REQUIRE(counter_bytes <= (size_t)64 * 1024 * sizeof(unsigned),
-               "too many replicas for one reduction launch: %d", R);
-    int rc = scratch(need, &base);
+    int rc = scratch((size_t)R * nblk * C * sizeof(double), &base);
     if (rc) return rc;
-    unsigned *counters = reinterpret_cast<unsigned *>(base);
-    double *partials = reinterpret_cast<double *>(reinterpret_cast<char *>(base) +
-                                                  (size_t)64 * 1024 * sizeof(unsigned));
+    double *partials = reinterpret_cast<double *>(base);
     hipLaunchKernelGGL((reduce_kernel<CA, ENERGY_ONLY, Acc>), dim3(nblk, R), dim3(kReduceThreads), 0,
-                       as_stream(stream), acc, R, L, nblk, tind, cached, out, partials, counters);
+                       as_stream(stream), acc, L, nblk, cached != nullptr, partials);
+    MC_CHECK_LAUNCH();
+    hipLaunchKernelGGL((reduce_finish_kernel<CA, ENERGY_ONLY, Acc>), dim3(R), dim3(64), 0,
+                       as_stream(stream), acc, R, nblk, tind, cached, out, partials);
     MC_CHECK_LAUNCH();
     return MC_OK;
 }

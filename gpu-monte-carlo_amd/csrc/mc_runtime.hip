@@ -27,7 +27,7 @@ int fail_hip(hipError_t err, const char *what) {
 }
 
 // ---------------------------------------------------------------- scratch
-// One region per device.  Layout: [counters (zeroed) | partials].
+// One region per device for the reductions' per-block partials.
 struct Scratch {
     void *ptr = nullptr;
     size_t bytes = 0;
