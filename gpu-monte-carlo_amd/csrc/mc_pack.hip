@@ -23,13 +23,17 @@
 // must be fully co-resident (checked on the host against the occupancy of
 // its LDS size); every wait is bounded by a watchdog regardless.
 //
-// Exchanges.  Energies are integers (mc_strip.h), so a group sums its strip
-// energies with device-scope atomic adds into two running totals per replica
-// (by exchange parity) and counts arriving workgroups on a monotone counter;
-// after every workgroup of the group has arrived, each reads the R totals,
-// takes the difference to the totals it saw two exchanges earlier, and runs
-// the identical matching (core/parallel_tempering.py:44-91) on identical
-// inputs.  Counters of pair `left` and trip records of walker w are kept by
+// Exchanges.  Energies are integers (mc_strip.h).  Each task publishes the
+// strip energy of each of its replicas as two tagged words (low and high
+// halves, tag = global exchange index + 1, double-buffered by exchange
+// parity); every workgroup of the group polls all R * S word pairs, adds each
+// replica's strips as integers and runs the identical matching
+// (core/parallel_tempering.py:44-91) on identical inputs.  A word carries its
+// own tag, so no fence or cache maintenance is needed (an acquire/release at
+// device scope writes back and invalidates the L2 on gfx950).  A workgroup
+// cannot publish exchange x + 2 before every workgroup of its group has
+// published exchange x + 1, which each does only after reading exchange x,
+// so two buffers suffice.  Counters of pair `left` and trip records of walker w are kept by
 // the group's workgroup left (w) mod its workgroup count; the group's first
 // workgroup writes the PT state back at the end of the launch.
 //
@@ -95,8 +99,7 @@ struct PackGroup {
     uint64_t *halo;            // [R][S][2 (row 2, row H)][2 (slot)][L] words
     uint64_t *ghost_rng;       // [R][S][L] records
     void *ghost_state;         // [R][S][L]
-    long long *energy_acc;     // [2][R] running integer energy totals
-    unsigned *arrive;          // workgroups arrived at this launch's exchanges
+    uint64_t *energy_words;    // [2 (exchange parity)][R][S][2 (lo, hi)] tagged words
 };
 
 struct PackWork {
@@ -117,7 +120,7 @@ struct PackArgs {
 
 // Dynamic LDS of a workgroup of a group: K strips, then the group's PT state
 struct PackLds {
-    unsigned stride, beta, w2t, t2w, thr, sig, moves, cnt, e, g, prev, trips, total;
+    unsigned stride, beta, w2t, t2w, thr, sig, moves, cnt, e, g, sum, trips, total;
 };
 
 __host__ __device__ inline PackLds pack_lds(int model, int L, int R, int H, int K) {
@@ -143,8 +146,8 @@ __host__ __device__ inline PackLds pack_lds(int model, int L, int R, int H, int 
     p += 8u * R;
     o.g = p;
     p += 16u * (R > 1 ? R - 1 : 1);
-    o.prev = p;
-    p += 16u * R;
+    o.sum = p;
+    p += 8u * R;
     o.trips = p;
     p += 40u * R;
     o.total = p;
@@ -173,7 +176,7 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
     unsigned *moves = reinterpret_cast<unsigned *>(smem + lay.moves);  // [R] accepted since the last flush
     double *pt_e = reinterpret_cast<double *>(smem + lay.e);
     uint64_t *pt_g = reinterpret_cast<uint64_t *>(smem + lay.g);
-    long long *prev = reinterpret_cast<long long *>(smem + lay.prev);  // [2][R]
+    long long *psum = reinterpret_cast<long long *>(smem + lay.sum);  // [R] integer energies of an exchange
     // round-trip records and swap counters of the walkers and pairs this
     // workgroup keeps (index mod the group's workgroups), in LDS for the launch
     long long *pt_trips = reinterpret_cast<long long *>(smem + lay.trips);  // [5][R]
@@ -184,8 +187,6 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
         pt_w2t[k] = use_pt ? g.w2t[k] : k;
         pt_t2w[k] = use_pt ? g.t2w[k] : k;
         moves[k] = 0;
-        prev[k] = 0;
-        prev[R + k] = 0;
     }
     if (use_pt) {
         for (int k = tid; k < 2 * (R - 1); k += NT) pt_g[k] = g.pt_rng[k];
@@ -195,7 +196,6 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
     int parity = use_pt ? *g.parity : 0;
     const long long sweep0 = a.clock[1];
     bool exchanged = false;
-    int q = 0;  // exchanges of this launch
     __syncthreads();
 
     // pass map: colour rows of the K strips spread over the threads
@@ -427,9 +427,12 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
                         for (int c = 0; c < 4; c++) M[c] += red[c][first + x];
                     }
                     const int r = r0 + tid;
-                    if (do_exchange)
-                        __hip_atomic_fetch_add(g.energy_acc + (size_t)(q & 1) * R + r, E, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
+                    if (do_exchange) {
+                        const long long xi = sweep_global / g.exchange_interval;
+                        uint64_t *dst = g.energy_words + ((((size_t)(xi & 1) * R + r) * S + s) * 2);
+                        publish_word(dst, (unsigned)xi + 1u, (uint32_t)(unsigned long long)E);
+                        publish_word(dst + 1, (unsigned)xi + 1u, (uint32_t)((unsigned long long)E >> 32));
+                    }
                     if (do_sample && sidx < a.max_samples) {
                         double *dst = g.sample_partials + (((size_t)sidx * R + r) * S + s) * kChannels;
                         dst[0] = energy_from_fixed(MODEL, J, E);
@@ -444,36 +447,29 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
 
         if (do_exchange) {
             flush_moves();  // moves so far belong to the current temperatures
-            // make this workgroup's energy adds visible before it arrives
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            for (int k = tid; k < R; k += NT) psum[k] = 0;
             __syncthreads();
+            // every strip energy of the group's replicas, added as integers
+            const long long xi = sweep_global / g.exchange_interval;
+            const unsigned tag = (unsigned)xi + 1u;
+            const uint64_t *words = g.energy_words + (size_t)(xi & 1) * R * S * 2;
             int bad = 0;
-            if (tid == 0) {
-                __hip_atomic_fetch_add(g.arrive, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                const unsigned target = (unsigned)(q + 1) * (unsigned)g.wg_count;
-                unsigned spins = 0;
-                while (__hip_atomic_load(g.arrive, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-                    __builtin_amdgcn_s_sleep(1);
-                    if ((++spins & 1023u) == 0 &&
-                        (spins > kWatchdog ||
-                         __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-                        bad = 1;
-                        break;
-                    }
+            for (int k = tid; k < R * S; k += NT) {
+                uint32_t lo, hi;
+                if (!poll_word(words + 2 * (size_t)k, tag, lo, a.error) ||
+                    !poll_word(words + 2 * (size_t)k + 1, tag, hi, a.error)) {
+                    bad = 1;
+                    break;
                 }
+                atomicAdd(reinterpret_cast<unsigned long long *>(psum + k / S),
+                          ((unsigned long long)hi << 32) | lo);
             }
             if (__syncthreads_or(bad)) {
                 if (tid == 0) atomicOr(a.error, BATCH_ERR_WATCHDOG);
                 failed = true;
                 break;
             }
-            for (int k = tid; k < R; k += NT) {
-                long long *tot = g.energy_acc + (size_t)(q & 1) * R + k;
-                const long long now = __hip_atomic_load(tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const long long e = now - prev[(q & 1) * R + k];
-                prev[(q & 1) * R + k] = now;
-                pt_e[k] = energy_from_fixed(MODEL, J, e);
-            }
+            for (int k = tid; k < R; k += NT) pt_e[k] = energy_from_fixed(MODEL, J, psum[k]);
             __syncthreads();
             const int P = R - 1;
             for (int left = parity + 2 * tid; left < R - 1; left += 2 * NT) {
@@ -526,7 +522,6 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
             }
             if (R > 2) parity = 1 - parity;
             exchanged = true;
-            q++;
             __syncthreads();
         }
         if (do_sample && lw == 0 && sidx < a.max_samples)
@@ -571,8 +566,8 @@ __global__ __launch_bounds__(NT) void pack_kernel(PackArgs a) {
 }
 
 // Before each launch: ghost rows from the lattice, halo words of sweep 0
-// (tag 1, slot 0; slot 1 cleared), exchange totals and arrival counters
-// cleared, the launch's base sweep, the error word.  blockIdx.y = group.
+// (tag 1, slot 0; slot 1 cleared), exchange words cleared, the launch's base
+// sweep, the error word.  blockIdx.y = group.
 __global__ void pack_prologue_kernel(PackArgs a) {
     const PackGroup g = a.groups[blockIdx.y];
     const int L = g.L, S = g.S, H = g.H, R = g.R;
@@ -604,9 +599,9 @@ __global__ void pack_prologue_kernel(PackArgs a) {
         h[2 * L] = ((uint64_t)1 << 32) | pH;
         h[3 * L] = 0;
     }
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)4 * R * S; k += stride)
+        g.energy_words[k] = 0;
     if (blockIdx.x == 0) {
-        for (int k = threadIdx.x; k < 2 * R; k += blockDim.x) g.energy_acc[k] = 0;
-        if (threadIdx.x == 0) *g.arrive = 0;
         if (blockIdx.y == 0 && threadIdx.x == 0) {
             *a.error = 0;
             a.clock[1] = a.clock[0];
@@ -814,7 +809,7 @@ int mc_pack_add_group(void *pack, int model, int L, int R, int H, int K, int exc
     const size_t sb = model == MODEL_ISING ? 1 : 4;
     if (hipMalloc(&gh.halo, rows * 4 * sizeof(uint64_t)) != hipSuccess ||
         hipMalloc(&gh.ghost_rng, rows * 16) != hipSuccess || hipMalloc(&gh.ghost_state, rows * sb) != hipSuccess ||
-        hipMalloc(&gh.acc, 2 * (size_t)R * sizeof(long long) + 64) != hipSuccess) {
+        hipMalloc(&gh.acc, (size_t)4 * R * (L / H) * sizeof(uint64_t)) != hipSuccess) {
         (void)hipFree(gh.halo);
         (void)hipFree(gh.ghost_rng);
         (void)hipFree(gh.ghost_state);
@@ -825,8 +820,7 @@ int mc_pack_add_group(void *pack, int model, int L, int R, int H, int K, int exc
     g.halo = static_cast<uint64_t *>(gh.halo);
     g.ghost_rng = static_cast<uint64_t *>(gh.ghost_rng);
     g.ghost_state = gh.ghost_state;
-    g.energy_acc = static_cast<long long *>(gh.acc);
-    g.arrive = reinterpret_cast<unsigned *>(static_cast<char *>(gh.acc) + 2 * (size_t)R * sizeof(long long));
+    g.energy_words = static_cast<uint64_t *>(gh.acc);
     *group_index = (int)p->groups.size();
     p->groups.push_back(gh);
     p->lds = std::max(p->lds, (size_t)pack_lds(model, L, R, H, K).total);
