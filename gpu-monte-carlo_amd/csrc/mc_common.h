@@ -3,8 +3,12 @@
 // Numerical policy (identical in every kernel):
 //  * float expressions are evaluated in source order without contraction
 //    (the library is compiled with -ffp-contract=off);
-//  * a float32 transcendental is the float64 function rounded once to
-//    float32 (cos, sin, exp, log); sqrt and division are IEEE correctly
+//  * a float32 transcendental (cos, sin, exp, log) is this project's float32
+//    function of mc_fastmath.h (Cody-Waite reduction and polynomials with
+//    explicit fused multiply-adds, at most 1.5 ulp on the arguments the
+//    updates pass), which the CPU oracle evaluates identically; the Ising
+//    acceptance thresholds exp(-beta dE) are the float64 exponential rounded
+//    once to float32 (exact_expf); sqrt and division are IEEE correctly
 //    rounded; float64 paths call the float64 functions;
 //  * per-site RNG streams are numba's xoroshiro128+ records {s0, s1}.
 #pragma once
@@ -14,6 +18,7 @@
 #include <stdint.h>
 
 #include "../../include/mcgpu.h"
+#include "mc_fastmath.h"
 
 namespace mc {
 
@@ -86,16 +91,39 @@ __device__ __forceinline__ void store_rng(uint64_t *states, size_t k, const Rng 
 }
 
 // ---------------------------------------------------- transcendentals
-__device__ __forceinline__ float rcos(float x) { return (float)::cos((double)x); }
-__device__ __forceinline__ float rsin(float x) { return (float)::sin((double)x); }
-__device__ __forceinline__ float rexp(float x) { return (float)::exp((double)x); }
-__device__ __forceinline__ float rlog(float x) { return (float)::log((double)x); }
+__device__ __forceinline__ float rcos(float x) {
+    float s, c;
+    fast_sincosf(x, s, c);
+    return c;
+}
+__device__ __forceinline__ float rsin(float x) {
+    float s, c;
+    fast_sincosf(x, s, c);
+    return s;
+}
+__device__ __forceinline__ float rexp(float x) { return fast_expf(x); }
+__device__ __forceinline__ float rlog(float x) { return fast_logf(x); }
+// the Ising thresholds: float64 exp rounded once to float32
+__device__ __forceinline__ float exact_expf(float x) { return (float)::exp((double)x); }
 __device__ __forceinline__ float rsqrt_(float x) { return ::sqrtf(x); }
 __device__ __forceinline__ double rcos(double x) { return ::cos(x); }
 __device__ __forceinline__ double rsin(double x) { return ::sin(x); }
 __device__ __forceinline__ double rexp(double x) { return ::exp(x); }
 __device__ __forceinline__ double rlog(double x) { return ::log(x); }
 __device__ __forceinline__ double rsqrt_(double x) { return ::sqrt(x); }
+
+// The reference's Metropolis test "dE <= 0 or u < exp(-beta dE)" with the
+// weight evaluated on every lane and the two tests combined without
+// short-circuit control flow (the argument is capped at 0; a NaN stays NaN
+// and is rejected as before).  hipcc for gfx950 has dropped the accepted
+// value of a variable assigned under the short-circuit form once the
+// exponential is an inlined function with control flow of its own.
+template <class R>
+__device__ __forceinline__ bool metropolis_accept(R dE, R u, R beta) {
+    R x = -beta * dE;
+    x = x > R(0) ? R(0) : x;
+    return (dE <= R(0)) | (u < rexp(x));
+}
 
 template <class R> struct RealConst;
 template <> struct RealConst<float> {

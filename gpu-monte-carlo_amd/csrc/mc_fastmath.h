@@ -33,10 +33,13 @@ __device__ __forceinline__ void fast_sincosf(float x, float &sn, float &cs) {
     cs = ((q + 1) & 2) ? -c1 : c1;
 }
 
+// Branch-free: the arguments at or below -104 (and NaN) take the select at
+// the end, so an inlined call adds no control flow to the caller.
 __device__ __forceinline__ float fast_expf(float x) {
-    if (!(x > -104.0f)) return x != x ? x : 0.0f;
-    const float k = rintf(x * 0x1.715476p+0f);
-    float r = fmaf(-k, 0x1.62e430p-1f, x);
+    const bool tiny = !(x > -104.0f);  // also NaN
+    const float xr = tiny ? 0.0f : x;
+    const float k = rintf(xr * 0x1.715476p+0f);
+    float r = fmaf(-k, 0x1.62e430p-1f, xr);
     r = fmaf(-k, -0x1.05c610p-29f, r);
     float p = fmaf(r, 0x1.a01a02p-13f, 0x1.6c16c2p-10f);
     p = fmaf(p, r, 0x1.111112p-7f);
@@ -46,11 +49,11 @@ __device__ __forceinline__ float fast_expf(float x) {
     p = fmaf(p, r, 1.0f);
     p = fmaf(p, r, 1.0f);
     int e = (int)k;
-    if (e < -125) {
-        p *= 0x1p-64f;
-        e += 64;
-    }
-    return p * __int_as_float((e + 127) << 23);
+    const bool low = e < -125;
+    p = low ? p * 0x1p-64f : p;
+    e = low ? e + 64 : e;
+    const float v = p * __int_as_float((e + 127) << 23);
+    return tiny ? (x != x ? x : 0.0f) : v;
 }
 
 // Metropolis acceptance weight exp(min(x, 0)) for the fast path, branch free:

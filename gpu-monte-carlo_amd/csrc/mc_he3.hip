@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void he3_color_kernel(R *__restrict__ st, uint
     const R diff = dpot + c.K * dgrad;
     const R u = g.template uniform<R>();
     store_rng(rng, k, g);
-    if (diff <= R(0) || u < rexp(-betas[r] * diff)) store_site(blk, L, i, j, psi);
+    if (metropolis_accept(diff, u, betas[r])) store_site(blk, L, i, j, psi);
 }
 
 // conditional field of the fixed neighbours (gaussian_fast.py:26-40)
@@ -233,9 +233,13 @@ __global__ __launch_bounds__(256) void he3_gaussian_color_kernel(R *__restrict__
     const R u = g.template uniform<R>();
     store_rng(rng, k, g);
     counts[k] += 1;
-    if (!isfinite(log_ratio)) {
-        counts[2 * N + k] += 1;
-    } else if (log_ratio >= R(0) || u < rexp(log_ratio)) {
+    // nonfinite: counted and rejected; else log r >= 0 or u < exp(log r),
+    // evaluated without short-circuit control flow (see metropolis_accept)
+    const bool finite = isfinite(log_ratio);
+    const R x = log_ratio > R(0) ? R(0) : log_ratio;
+    const bool accept = finite & ((log_ratio >= R(0)) | (u < rexp(x)));
+    if (!finite) counts[2 * N + k] += 1;
+    if (accept) {
         counts[N + k] += 1;
         store_site(blk, L, i, j, psi);
     }

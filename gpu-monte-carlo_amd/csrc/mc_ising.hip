@@ -26,7 +26,7 @@ __global__ __launch_bounds__(256) void ising_color_kernel(int8_t *__restrict__ s
     Rng g = load_rng(rng, k);
     const float u = g.uniform_f32();
     store_rng(rng, k, g);
-    if (dE <= 0.0f || u < rexp(-betas[r] * dE)) b[i * L + j] = (int8_t)(-s);
+    if (dE <= 0.0f || u < exact_expf(-betas[r] * dE)) b[i * L + j] = (int8_t)(-s);
 }
 
 struct IsingAcc {

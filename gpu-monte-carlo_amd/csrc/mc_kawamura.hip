@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void kawamura_color_kernel(R *__restrict__ st,
     R pe = kaw_bonds(ntau, dot, pcs, tau, J);
     R dE = pe - energy;
     R u = g.template uniform<R>();
-    if (dE <= zero || u < rexp(-beta * dE)) {
+    if (metropolis_accept(dE, u, beta)) {
         theta = pt;
         energy = pe;
 #pragma unroll
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(256) void kawamura_color_kernel(R *__restrict__ st,
     pe = kaw_bonds(ntau, dot, cs, ptau, J);
     dE = pe - energy;
     u = g.template uniform<R>();
-    if (dE <= zero || u < rexp(-beta * dE)) {
+    if (metropolis_accept(dE, u, beta)) {
         tau = ptau;
         energy = pe;
     }
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void kawamura_color_kernel(R *__restrict__ st,
     pe = kaw_bonds(ntau, pdot, cs, tau, J);
     dE = pe - energy;
     u = g.template uniform<R>();
-    if (dE <= zero || u < rexp(-beta * dE)) {
+    if (metropolis_accept(dE, u, beta)) {
         dx = x;
         dy = y;
         dz = z;

@@ -92,7 +92,7 @@ __device__ __forceinline__ void publish_word(uint64_t *p, unsigned seq, uint32_t
 __device__ __forceinline__ float ising_threshold(int v, float beta, float J) {
     const float dE = (float)v * J;
     if (dE <= 0.0f) return __builtin_inff();
-    return rexp(-beta * dE);
+    return exact_expf(-beta * dE);
 }
 
 // Block reduction of one int64 and C float64 channels over NT threads; the

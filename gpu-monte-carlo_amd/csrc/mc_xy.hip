@@ -36,7 +36,7 @@ __global__ __launch_bounds__(256) void xy_color_kernel(R *__restrict__ th, uint6
     const R dE = e_new - e_old;
     const R v = g.template uniform<R>();
     store_rng(rng, k, g);
-    if (dE <= zero || v < rexp(-betas[r] * dE)) b[i * L + j] = p;
+    if (metropolis_accept(dE, v, betas[r])) b[i * L + j] = p;
 }
 
 // Angle into [-pi, pi) after an unbounded move: subtract whole turns, then
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void xy_gauss_kernel(R *__restrict__ th, uint6
         const R dE = e_new - e_old;
         const R v = g.template uniform<R>();
         store_rng(rng, k, g);
-        if (dE <= zero || v < rexp(-betas[r] * dE)) {
+        if (metropolis_accept(dE, v, betas[r])) {
             b[i * L + j] = p;
             acc = 1;
         }

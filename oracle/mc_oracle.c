@@ -96,10 +96,13 @@ void orc_rng_draw_f64(uint64_t *states, int64_t index, int64_t count, double *ou
 }
 
 /* ------------------------------------------------------ transcendentals */
-static inline float rcos_f32(float x) { return (float)cos((double)x); }
-static inline float rsin_f32(float x) { return (float)sin((double)x); }
-static inline float rexp_f32(float x) { return (float)exp((double)x); }
-static inline float rlog_f32(float x) { return (float)log((double)x); }
+/* float32: the fast functions of mc_oracle_fast.c (the kernels' policy);
+ * the Ising thresholds: float64 exp rounded once to float32 */
+static inline float rcos_f32(float x) { float s, c; orc_fast_sincosf(x, &s, &c); return c; }
+static inline float rsin_f32(float x) { float s, c; orc_fast_sincosf(x, &s, &c); return s; }
+static inline float rexp_f32(float x) { return orc_fast_expf(x); }
+static inline float rlog_f32(float x) { return orc_fast_logf(x); }
+static inline float exact_expf_f32(float x) { return (float)exp((double)x); }
 static inline float rsqrt_f32(float x) { return sqrtf(x); }
 static inline float rfloor_f32(float x) { return floorf(x); }
 static inline float rfabs_f32(float x) { return fabsf(x); }
@@ -132,7 +135,7 @@ void orc_ising_color(int8_t *spins, uint64_t *rng, const float *betas, int R, in
                                + spins[ISITE(L, r, i, (j + L - 1) % L)] + spins[ISITE(L, r, i, (j + 1) % L)];
                 const float dE = (float)(2 * s * nb) * J;
                 const float u = orc_uniform_f32(rng + 2 * k);
-                if (dE <= 0.0f || u < rexp_f32(-betas[r] * dE)) spins[k] = (int8_t)(-s);
+                if (dE <= 0.0f || u < exact_expf_f32(-betas[r] * dE)) spins[k] = (int8_t)(-s);
             }
         }
     }

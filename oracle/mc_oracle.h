@@ -11,13 +11,14 @@
  * output (C, R) float64.
  *
  * Transcendental policy (the reference's float32 kernels call NVIDIA libdevice,
- * whose rounding cannot be reproduced): a float32 transcendental is the
- * float64 function rounded once to float32, e.g. cosf(x) := (float)cos((double)x).
+ * whose rounding cannot be reproduced): a float32 transcendental (cos, sin,
+ * exp, log) is one of the float32 functions of mc_oracle_fast.c (Cody-Waite
+ * reduction and polynomials with explicit fmaf, at most 1.5 ulp on the
+ * updates' arguments), exactly as the HIP kernels evaluate it; the Ising
+ * acceptance thresholds are the float64 exp rounded once to float32.
  * float64 paths call the float64 libm function.  sqrt and division are IEEE.
  * Arithmetic is evaluated in source order without contraction
  * (-ffp-contract=off), like the reference's Python expressions.
- *
- * Parity status per function is listed in oracle/README.md.
  */
 #ifndef MC_ORACLE_H
 #define MC_ORACLE_H
