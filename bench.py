@@ -337,11 +337,27 @@ def bench_gradients(calls=20):
             stop.record()
             torch.cuda.synchronize()
             ms = start.elapsed_time(stop) / calls
-            # the kernels alone: terms (forward) and state gradient (backward), queued back to back
+            # the kernels alone: terms (forward) and state gradient (backward) through the C-ABI on
+            # preallocated buffers, queued back to back
+            from mcgpu import _lib
+            terms_fn, grad_fn = autograd._KERNELS[name][0], autograd._KERNELS[name][1]
+            code, stream, ptr = autograd._code(state), autograd._stream(), autograd._ptr
+            tw = torch.zeros(Rg, dtype=torch.float64, device="cuda")
+            ones = torch.ones(Rg, dtype=torch.float64, device="cuda")
+            zeros = torch.zeros(Rg, dtype=torch.float64, device="cuda")
+            out3 = torch.empty((3, Rg), dtype=torch.float64, device="cuda")
+            gout = torch.empty_like(state)
+
+            def kernels():
+                _lib.call(terms_fn, stream, code, ptr(state), Rg, Lg, ptr(tw), ptr(out3))
+                _lib.call(grad_fn, stream, code, ptr(state), Rg, Lg, ptr(tw), ptr(ones), ptr(ones), ptr(zeros),
+                          ptr(gout))
+
+            kernels()
+            torch.cuda.synchronize()
             start.record()
             for _ in range(calls):
-                autograd._terms(name, state, twist.detach())
-                autograd._grad(name, state, twist.detach(), 1.0, 1.0, 0.0)
+                kernels()
             stop.record()
             torch.cuda.synchronize()
             kms = start.elapsed_time(stop) / calls
@@ -371,14 +387,16 @@ def copy_probe(mib):
 
 
 def measured_traffic(strip_rows, threads, sweeps):
-    """HBM bytes of a launch of `sweeps` sweeps from the committed counter runs, or None."""
+    """HBM bytes of a launch of `sweeps` sweeps from the committed counter runs of the same
+    strip height and workgroup size, or None."""
     try:
         t = json.loads(TRAFFIC_FILE.read_text())
     except (OSError, ValueError):
         return None, None
-    if int(t.get("strip_rows", -1)) != strip_rows or int(t.get("threads", 256)) != threads:
-        return None, None
-    return float(t["traffic_bytes_per_sweep"]) * sweeps, t.get("summary")
+    for c in t.get("configs", []):
+        if int(c["strip_rows"]) == strip_rows and int(c["threads"]) == threads:
+            return float(c["traffic_bytes_per_sweep"]) * sweeps, c.get("summary")
+    return None, None
 
 
 def cpu_baseline(target_s, sigmas):

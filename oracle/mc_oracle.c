@@ -6,6 +6,7 @@
  */
 #define _GNU_SOURCE
 #include "mc_oracle.h"
+#include "oracle_fastmath.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -98,10 +99,10 @@ void orc_rng_draw_f64(uint64_t *states, int64_t index, int64_t count, double *ou
 /* ------------------------------------------------------ transcendentals */
 /* float32: the fast functions of mc_oracle_fast.c (the kernels' policy);
  * the Ising thresholds: float64 exp rounded once to float32 */
-static inline float rcos_f32(float x) { float s, c; orc_fast_sincosf(x, &s, &c); return c; }
-static inline float rsin_f32(float x) { float s, c; orc_fast_sincosf(x, &s, &c); return s; }
-static inline float rexp_f32(float x) { return orc_fast_expf(x); }
-static inline float rlog_f32(float x) { return orc_fast_logf(x); }
+static inline float rcos_f32(float x) { float s, c; ofm_sincosf(x, &s, &c); return c; }
+static inline float rsin_f32(float x) { float s, c; ofm_sincosf(x, &s, &c); return s; }
+static inline float rexp_f32(float x) { return ofm_expf(x); }
+static inline float rlog_f32(float x) { return ofm_logf(x); }
 static inline float exact_expf_f32(float x) { return (float)exp((double)x); }
 static inline float rsqrt_f32(float x) { return sqrtf(x); }
 static inline float rfloor_f32(float x) { return floorf(x); }
