@@ -12,8 +12,8 @@ launches per sweep).  Then:
   (utils/Analysis/crossings.py:5-31) are therefore identical, and the study
   checks that they are.
 * XY: the packed engine uses the float32 fast update (cached cos/sin), the
-  plain path the float32 libm update, so the chains (independent seeds) agree
-  in distribution.  For every size and temperature the helicity modulus and
+  plain path the reference's float32 update (the cosine of every angle
+  difference), so the chains (independent seeds) agree in distribution.  For every size and temperature the helicity modulus and
   the Binder cumulant U = 1 - <m^4>/(2 <m^2>^2) (models/xy/analysis.py:47-54)
   are compared with blocked-jackknife errors, and so are the temperatures
   where each size's helicity crosses 2T/pi and where consecutive sizes'
