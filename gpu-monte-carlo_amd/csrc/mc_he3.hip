@@ -208,8 +208,12 @@ __device__ __forceinline__ R gaussian_proposal(Cpx<R> *psi, const Cpx<R> *field,
     return log_ratio;
 }
 
+// Occupancy: the complex64 update needs 167 VGPRs unconstrained (three waves
+// per SIMD); asking for four waves (128 VGPRs, a few spills) measured 24%
+// faster on MI355X, five waves and a constrained complex128 update slower.
 template <class R>
-__global__ __launch_bounds__(256) void he3_gaussian_color_kernel(R *__restrict__ st, uint64_t *__restrict__ rng,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1))) void
+he3_gaussian_color_kernel(R *__restrict__ st, uint64_t *__restrict__ rng,
                                                                  const R *__restrict__ betas, int L,
                                                                  int color, He3Coef<R> c, R rho, R mass,
                                                                  int64_t *__restrict__ counts, size_t N) {
