@@ -155,6 +155,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     __shared__ long long red_e[NT / 64];
     __shared__ float thr[5];
     __shared__ int red_i[NT / 64];
+    __shared__ float4 ln_tab[MODEL == MODEL_XY_GAUSS ? kLnCells : 1];  // log cells of gauss_draw
 
     const size_t rep = (size_t)r * L * L;
     const int row0 = s * H;  // first global row of the strip
@@ -242,6 +243,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             slot_draw[p][m] = SiteDraw{0.0f, 0.0f};
             slot_pr[p][m] = slot_pc[p][m] = slot_ps[p][m] = slot_dc[p][m] = slot_ds[p][m] = 0.0f;
         }
+    if (MODEL == MODEL_XY_GAUSS) ln_table_to_lds(ln_tab);
     __syncthreads();
     if (MODEL != MODEL_ISING) {
 #pragma unroll
@@ -267,8 +269,15 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             } else {
                 const int self = slot_self[p][m], lf = slot_lf[p][m], rt = slot_rt[p][m];
                 const int up = self - L, dn = self + L;
-                const float Cs = ((lc[up] + lc[dn]) + lc[lf]) + lc[rt];
-                const float Ss = ((ls[up] + ls[dn]) + ls[lf]) + ls[rt];
+                // All eight neighbour reads are issued before the first add:
+                // left to itself the scheduler interleaves each read with a
+                // wait and an add, seven LDS round trips in a row per pass
+                // (measured on MI355X: 124.5 -> 144 site-updates/ns).
+                const float cu = lc[up], cd = lc[dn], cl = lc[lf], cr = lc[rt];
+                const float su = ls[up], sd = ls[dn], sl = ls[lf], sr = ls[rt];
+                __builtin_amdgcn_sched_barrier(0);
+                const float Cs = ((cu + cd) + cl) + cr;
+                const float Ss = ((su + sd) + sl) + sr;
                 float t = slot_dc[p][m] * Cs;
                 t = fmaf(slot_ds[p][m], Ss, t);
                 const float dE = -J * t;
@@ -418,7 +427,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
                     Rng g;
                     g.s0 = slot_s0[p][m];
                     g.s1 = slot_s1[p][m];
-                    const SiteDraw d = site_draw<MODEL>(g);
+                    const SiteDraw d = site_draw<MODEL>(g, ln_tab);
                     slot_draw[p][m] = d;
                     slot_s0[p][m] = g.s0;
                     slot_s1[p][m] = g.s1;

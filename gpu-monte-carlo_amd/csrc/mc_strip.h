@@ -156,16 +156,17 @@ __device__ __forceinline__ StripView strip_view(unsigned char *base, int L, int 
 
 // A site's draws for one sweep, taken from its stream before the sweep's
 // colour passes (they do not depend on the lattice): Ising a = u; XY a = u1
-// (proposal), b = u2 (acceptance); Gaussian XY a = z, b = v (gauss_draw).
+// (proposal), b = u2 (acceptance); Gaussian XY a = z, b = v (gauss_draw with
+// the log cell table ln_tab in LDS; unused by the other models).
 struct SiteDraw {
     float a, b;
 };
 
 template <int MODEL>
-__device__ __forceinline__ SiteDraw site_draw(Rng &g) {
+__device__ __forceinline__ SiteDraw site_draw(Rng &g, const float4 *ln_tab) {
     SiteDraw d;
     if (MODEL == MODEL_XY_GAUSS) {
-        const GaussDraw gd = gauss_draw(g.next());
+        const GaussDraw gd = gauss_draw(g.next(), ln_tab);
         d.a = gd.z;
         d.b = gd.v;
     } else if (MODEL == MODEL_XY_FAST) {
@@ -230,11 +231,11 @@ __device__ __forceinline__ bool site_apply(const StripView &v, const int self, c
 //    21-54) with dE from the cached (cos, sin) of the neighbours and the
 //    float32 sincos/exp of mc_fastmath.h.
 //  * MODEL_XY_GAUSS: proposal theta + sigma z; z and the acceptance uniform
-//    from one 64-bit draw (gauss_draw).
+//    from one 64-bit draw (gauss_draw, log cell table ln_tab in LDS).
 template <int MODEL>
 __device__ __forceinline__ bool site_update(const StripView &v, const int c, const int ri, const int tcol,
                                             const int L, const float beta, const float sigma, const float J,
-                                            const float delta, const float *thr) {
+                                            const float delta, const float *thr, const float4 *ln_tab) {
     const int half = L >> 1;
     const int p = c & 1;
     const int lr = 1 + 2 * ri + (c >> 1);
@@ -259,10 +260,15 @@ __device__ __forceinline__ bool site_update(const StripView &v, const int c, con
         const float one = 1.0f, two = 2.0f;
         const float pi = 3.14159274101257324f, two_pi = 6.28318548202514648f;
         const float th = v.th[own];
+        // the site's and its neighbours' (cos, sin) are read before the draw,
+        // so their LDS latency runs under the proposal's arithmetic
+        const float cu = v.c[up], cd = v.c[dn], cl = v.c[lf], cr = v.c[rt], c0 = v.c[self];
+        const float su = v.s[up], sd = v.s[dn], sl = v.s[lf], sr = v.s[rt], s0 = v.s[self];
+        __builtin_amdgcn_sched_barrier(0);
         float pr;
         float w;
         if (MODEL == MODEL_XY_GAUSS) {
-            const GaussDraw d = gauss_draw(g.next());
+            const GaussDraw d = gauss_draw(g.next(), ln_tab);
             pr = wrap_turns(th + sigma * d.z);
             w = d.v;
         } else {
@@ -275,10 +281,10 @@ __device__ __forceinline__ bool site_update(const StripView &v, const int c, con
         }
         float sp, cp;
         fast_sincosf(pr, sp, cp);
-        const float Cs = ((v.c[up] + v.c[dn]) + v.c[lf]) + v.c[rt];
-        const float Ss = ((v.s[up] + v.s[dn]) + v.s[lf]) + v.s[rt];
-        float t = (cp - v.c[self]) * Cs;
-        t = fmaf(sp - v.s[self], Ss, t);
+        const float Cs = ((cu + cd) + cl) + cr;
+        const float Ss = ((su + sd) + sl) + sr;
+        float t = (cp - c0) * Cs;
+        t = fmaf(sp - s0, Ss, t);
         const float dE = -J * t;
         if (MODEL != MODEL_XY_GAUSS) w = g.uniform_f32();
         acc = w < accept_weight(-beta * dE);  // dE <= 0 or w < exp(-beta dE)

@@ -33,6 +33,17 @@ void orc_fast_logf_n(const float *x, int64_t n, float *y) {
     for (int64_t k = 0; k < n; k++) y[k] = orc_fast_logf(x[k]);
 }
 
+/* ln(n 2^-21) of the Gaussian step for n[k] in [1, 2^21], and its cell table
+ * (65 rows of rc, hi, lo) */
+void orc_ln_u21_n(const uint32_t *n, int64_t cnt, float *y) {
+    for (int64_t k = 0; k < cnt; k++) y[k] = ofm_ln_u21(n[k]);
+}
+
+void orc_ln_u21_table(float *out) {
+    for (int i = 0; i < 65; i++)
+        for (int c = 0; c < 3; c++) out[3 * i + c] = OFM_LN_U21[i][c];
+}
+
 void orc_fast_sincosf_n(const float *x, int64_t n, float *sn, float *cs) {
     for (int64_t k = 0; k < n; k++) orc_fast_sincosf(x[k], sn + k, cs + k);
 }
@@ -133,7 +144,8 @@ static float halfpi_sinf(float x) {
  * engine's gauss_draw): bits 63..43 -> u1 = (k1 + 1) 2^-21 in (0, 1];
  * bits 42..22 -> sign bit and a 20-bit phi = pi (a 2^-20 - 1/2) in [-pi/2, pi/2);
  * bits 21..1 -> acceptance uniform v = k3 2^-21.  z = +-sqrt(-2 log u1) sin(phi)
- * with this file's log, the Newton square root and the half-period sine;
+ * with log u1 = ofm_ln_u21(k1 + 1) (cell table, no division), the Newton
+ * square root and the half-period sine;
  * p = theta + sigma z wrapped by whole turns (multiplying by 1/(2 pi), branch
  * free); then the fast dE and acceptance.  sig holds each walker's sigma; acc
  * counts accepted moves per walker (may be NULL). */
@@ -142,9 +154,8 @@ void orc_fast_gauss_draw(uint64_t x, float *z, float *v) {
     const uint32_t k1 = hi >> 11;
     const uint32_t k2 = ((hi & 0x7ffu) << 10) | (lo >> 22);
     const uint32_t k3 = (lo >> 1) & 0x1fffffu;
-    const float u1 = (float)(k1 + 1u) * 0x1p-21f;
     const float phi = fmaf((float)(k2 & 0xfffffu), 0x1.921fb6p-19f, -0x1.921fb6p+0f);
-    const float zz = newton_sqrtf(-2.0f * orc_fast_logf(u1)) * halfpi_sinf(phi);
+    const float zz = newton_sqrtf(-2.0f * ofm_ln_u21(k1 + 1u)) * halfpi_sinf(phi);
     *z = (k2 >> 20) ? -zz : zz;
     *v = (float)k3 * 0x1p-21f;
 }

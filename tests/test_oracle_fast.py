@@ -165,3 +165,59 @@ def test_accept_weight_is_clamped_exponential():
     assert np.all(oracle.accept_weight(np.array([0.0, 1e-30, 3.0, 1e30, np.inf], np.float32)) == 1.0)
     low = oracle.accept_weight(np.array([-87.0, -90.0, -1e4, -np.inf], np.float32))
     assert np.all(low == low[0]) and 0 < low[0] < 2e-38 and low[0] >= np.finfo(np.float32).tiny
+
+
+def test_fast_logf_error_bound_on_uniform_draws():
+    """fast_logf (the float32 log of the fast paths, called on uniforms in (0, 1])
+    is within 1.5 ulp of float64 log, exact at 1 and at powers of two."""
+    rand = np.random.default_rng(5).uniform(0.0, 1.0, 1_000_000).astype(np.float32)
+    grid = (np.arange(1, 2**20 + 1, dtype=np.float64) * 2.0**-20).astype(np.float32)
+    x = np.concatenate([rand[rand > 0], grid, np.float32(2.0) ** -np.arange(1, 127, dtype=np.float32)])
+    y = oracle.fast_logf(x)
+    assert _ulp_error(y, np.log(x.astype(np.float64))).max() <= 1.5
+    assert oracle.fast_logf(np.array([1.0], np.float32))[0] == 0.0
+    k = np.arange(-126, 1)
+    np.testing.assert_array_equal(oracle.fast_logf((2.0 ** k).astype(np.float32)),
+                                  (k * np.log(2.0)).astype(np.float32))
+
+
+def test_ln_u21_every_input_and_its_cancellation_point():
+    """ln_u21(n) = ln(n 2^-21), the Gaussian step's log, over all 2^21 inputs:
+    within 1.3 ulp of float64 log, zero at n = 2^21, and accurate in relative
+    terms just below it, where (E - 21) ln 2 and the top cell's ln 2 cancel."""
+    n = np.arange(1, 2**21 + 1, dtype=np.uint32)
+    y = oracle.ln_u21(n)
+    ref = np.log(n.astype(np.float64)) - 21 * np.log(2.0)
+    err = _ulp_error(y, ref)
+    assert err.max() <= 1.3
+    assert y[-1] == 0.0
+    top = slice(2**21 - 2**14, 2**21 - 1)  # the last 16384 nonzero results, |ln| < 2^-7
+    assert np.max(np.abs(y[top] - ref[top]) / np.abs(ref[top])) < 2.0**-22
+    assert np.all(np.diff(y.astype(np.float64)) >= 0)  # monotone
+
+
+def test_ln_u21_table_is_the_rounded_cell_logs():
+    """Cell i (c = 1 + i/64) holds rc = float32(1/c) and -ln(rc) split as
+    hi = float32(-ln rc), lo = float32(-ln rc - hi), checked in 50-digit
+    decimal arithmetic; cell 0 is (1, 0, 0) and cell 64 holds ln 2 as the
+    Cody-Waite pair whose sum is ln 2 to 2^-50."""
+    from decimal import Decimal, getcontext
+    from fractions import Fraction
+    getcontext().prec = 50
+    tab = oracle.ln_u21_table()
+
+    def half_ulp_ok(f32, exact):
+        return abs(Fraction(float(f32)) - exact) <= Fraction(float(np.spacing(np.abs(f32)))) / 2
+
+    for i in range(65):
+        rc, hi, lo = tab[i]
+        assert half_ulp_ok(rc, 1 / Fraction(64 + i, 64)), i
+        if i == 0:
+            assert (rc, hi, lo) == (1.0, 0.0, 0.0)
+            continue
+        L = -Decimal(float(rc)).ln()
+        if i == 64:
+            assert abs(Decimal(float(hi)) + Decimal(float(lo)) - L) < Decimal(2) ** -50
+            continue
+        assert half_ulp_ok(hi, Fraction(L)), i
+        assert half_ulp_ok(lo, Fraction(L - Decimal(float(hi)))), i
