@@ -82,6 +82,9 @@ struct BatchArgs {
     // sampling (sample_interval == 0: none)
     int sample_interval;
     long long sample_first;     // global sweep of sample 0
+    // this launch's first exchange and sample sweeps (n; nsweeps if none) and
+    // their indices, so the sweep loop needs no 64-bit division
+    long long x_next, x_index, s_next, s_index;
     int max_samples;
     double *sample_partials;    // [max_samples][R][S][kChannels]
     int32_t *sample_w2t;        // [max_samples][R]
@@ -151,6 +154,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     uint32_t *pt_part = reinterpret_cast<uint32_t *>(sig_tab + R);  // 2 (walker, partner) x S x (lo, hi)
     __shared__ long long my_trips[5];  // this walker's round-trip record (strip 0)
     __shared__ int x_acc;              // the exchange decision of this walker's pair
+    __shared__ double x_u, x_logu;     // its PT draw and log (when u > 0)
     __shared__ double red[kChannels][NT / 64];
     __shared__ long long red_e[NT / 64];
     __shared__ float thr[5];
@@ -199,6 +203,9 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     // fixed thread -> (column, colour-row offset) map of every pass: a thread
     // takes items trow, trow + rows_per_pass, ... (at most MAXP) of each pass
     const int tcol = tid % half, trow = tid / half, rows_per_pass = NT / half;
+    // the observables' sites tid, tid + NT, ... of rows 1..H, stepped by
+    // (NT / L, NT % L) rows and columns instead of a division per site
+    const int obs_i = tid / L, obs_j = tid % L, obs_di = NT / L, obs_dj = NT % L;
 
     // ---- the thread's sites: LDS positions and RNG streams in registers for
     // the whole launch (the ghost row's streams are a copy of its owner's)
@@ -415,6 +422,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     //   P5  c3 of all 2/3 rows; publish rows 2 and H
     // P5 of sweep n and P1 of sweep n+1 touch disjoint sites and P2 starts
     // after a barrier, so P5 needs no barrier of its own.
+    long long x_next = a.x_next, x_index = a.x_index, s_next = a.s_next, s_index = a.s_index;
     for (int n = 0; n < a.nsweeps; n++) {
         const long long sweep_global = a.sweep0 + n + 1;
         const float beta = my_beta;
@@ -465,9 +473,8 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
         apply(4, beta);
         publish(n);
 
-        const bool do_exchange = use_pt && (sweep_global % a.exchange_interval == 0);
-        const bool do_sample = a.sample_interval > 0 && sweep_global >= a.sample_first &&
-                               ((sweep_global - a.sample_first) % a.sample_interval == 0);
+        const bool do_exchange = n == x_next;
+        const bool do_sample = n == s_next;
         if (!do_exchange && !do_sample) continue;
         __syncthreads();  // the observables read every row
 
@@ -479,8 +486,17 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
         double nacc = 0.0;  // the workgroup's accepted moves travel with the sums
         {
             double m[4] = {0.0, 0.0, 0.0, 0.0};
-            for (int idx = tid; idx < H * L; idx += NT)
-                site_observables<MODEL>(view, 1 + idx / L, idx % L, L, do_sample, ef, m);
+            int oi = obs_i, oj = obs_j;
+#pragma unroll 2
+            for (int idx = tid; idx < H * L; idx += NT) {
+                site_observables<MODEL>(view, 1 + oi, oj, L, do_sample, ef, m);
+                oi += obs_di;
+                oj += obs_dj;
+                if (oj >= L) {
+                    oj -= L;
+                    oi += 1;
+                }
+            }
             if (do_sample) {
                 double w[5] = {m[0], m[1], m[2], m[3], (double)n_acc};
                 block_sum<5, NT>(ef, w, red_e, red);
@@ -500,7 +516,8 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
 
         if (do_exchange) {
             exchanged = true;
-            const long long xi = sweep_global / a.exchange_interval;
+            const long long xi = x_index++;
+            x_next += a.exchange_interval;
             const unsigned tag = (unsigned)xi + 1u;
             // energy words by ladder temperature, a ring of four exchanges
             uint64_t *ring = a.energy + (size_t)(xi & 3) * R * S * 2;
@@ -516,6 +533,23 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             const int partner = (rel >= 0 && !(rel & 1) && my_t < R - 1) ? my_t + 1
                                 : (rel > 0 && (rel & 1)) ? my_t - 1 : -1;
             const int my_left = partner < 0 ? -1 : (partner < my_t ? partner : my_t);
+            // every attempted pair's stream takes one draw (energies are
+            // integer sums here, always finite).  The draws do not depend on
+            // the energies, so they are taken while the energy words are in
+            // flight, by the threads of the second wavefront (the first one
+            // polls); the thread of this walker's pair keeps u and log u.
+            for (int left = parity + 2 * ((tid + NT - 64) % NT); left < R - 1; left += 2 * NT) {
+                Rng g;
+                g.s0 = pt_g[2 * left];
+                g.s1 = pt_g[2 * left + 1];
+                const double u = g.uniform_f64();
+                pt_g[2 * left] = g.s0;
+                pt_g[2 * left + 1] = g.s1;
+                if (left == my_left) {
+                    x_u = u;
+                    x_logu = u == 0.0 ? 0.0 : ::log(u);
+                }
+            }
             // gather the strips of this walker and of its partner only
             int bad = 0;
             const int nw = partner >= 0 ? 4 * S : 2 * S;
@@ -532,26 +566,17 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
                 if (tid == 0) atomicOr(a.error, BATCH_ERR_WATCHDOG);
                 goto done;
             }
-            // every attempted pair's stream takes one draw (energies are
-            // integer sums here, always finite); this walker's pair decides
-            for (int left = parity + 2 * tid; left < R - 1; left += 2 * NT) {
-                Rng g;
-                g.s0 = pt_g[2 * left];
-                g.s1 = pt_g[2 * left + 1];
-                const double u = g.uniform_f64();
-                pt_g[2 * left] = g.s0;
-                pt_g[2 * left + 1] = g.s1;
-                if (left == my_left) {
-                    const double e0 = energy_from_fixed(MODEL, J, strip_total(pt_part, S));
-                    const double e1 = energy_from_fixed(MODEL, J, strip_total(pt_part + 2 * S, S));
-                    const double ea = my_t == left ? e0 : e1, eb = my_t == left ? e1 : e0;
-                    const double lr = ((double)ladder[left] - (double)ladder[left + 1]) * (ea - eb);
-                    x_acc = (lr >= 0.0 || u == 0.0 || ::log(u) < lr) ? 1 : 0;
-                }
-            }
+            // this walker's pair decides (core/parallel_tempering.py:44-69)
             if (tid == 0) {
-                if (my_left < 0) x_acc = 0;
-                my_e = energy_from_fixed(MODEL, J, strip_total(pt_part, S));
+                const double e0 = energy_from_fixed(MODEL, J, strip_total(pt_part, S));
+                my_e = e0;
+                x_acc = 0;
+                if (my_left >= 0) {
+                    const double e1 = energy_from_fixed(MODEL, J, strip_total(pt_part + 2 * S, S));
+                    const double ea = my_t == my_left ? e0 : e1, eb = my_t == my_left ? e1 : e0;
+                    const double lr = ((double)ladder[my_left] - (double)ladder[my_left + 1]) * (ea - eb);
+                    x_acc = (lr >= 0.0 || x_u == 0.0 || x_logu < lr) ? 1 : 0;
+                }
             }
             __syncthreads();
             const bool acc = x_acc != 0;
@@ -585,7 +610,8 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
         }
 
         if (do_sample) {
-            const long long idx = (sweep_global - a.sample_first) / a.sample_interval;
+            const long long idx = s_index++;
+            s_next += a.sample_interval;
             if (idx < a.max_samples) {
                 if (tid == 0) {
                     double *dst = a.sample_partials + (((size_t)idx * R + r) * S + s) * kChannels;
@@ -756,6 +782,24 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, int threads, void
     a.w2t = w2t; a.t2w = t2w; a.pt_rng = pt_rng; a.counts = counts; a.trips = trips;
     a.sample_interval = sample_interval; a.sample_first = sample_first; a.max_samples = max_samples;
     a.sample_partials = sample_partials; a.sample_w2t = sample_w2t;
+    // first exchange sweep: the first global sweep g >= sweep0 + 1 with
+    // g % exchange_interval == 0; first sample sweep: the first
+    // g >= max(sweep0 + 1, sample_first) with (g - sample_first) % sample_interval == 0
+    const long long g1 = sweep0 + 1;
+    a.x_next = a.s_next = nsweeps;
+    a.x_index = a.s_index = 0;
+    if (exchange_interval > 0) {
+        const long long first = g1 + (exchange_interval - g1 % exchange_interval) % exchange_interval;
+        a.x_next = std::min<long long>(first - g1, nsweeps);
+        a.x_index = first / exchange_interval;
+    }
+    if (sample_interval > 0) {
+        const long long first =
+            g1 <= sample_first ? sample_first
+                               : g1 + (sample_interval - (g1 - sample_first) % sample_interval) % sample_interval;
+        a.s_next = std::min<long long>(first - g1, nsweeps);
+        a.s_index = (first - sample_first) / sample_interval;
+    }
     a.halo = halo; a.energy = energy; a.error = error;
 
     const void *kern = kernel_ptr(model, threads, batch_items(L, H, threads));

@@ -203,20 +203,33 @@ __device__ __forceinline__ void ln_table_to_lds(float4 *lds) {
     for (int k = threadIdx.x; k < kLnCells; k += blockDim.x) lds[k] = kLnU21[k];
 }
 
-__device__ __forceinline__ float ln_u21(unsigned n, const float4 *__restrict__ tab) {
-    const unsigned b = __float_as_uint((float)n);
-    const unsigned i = (((b >> 16) & 0x7fu) + 1u) >> 1;
-    const float m = __uint_as_float((b & 0x7fffffu) | 0x3f800000u);
-    const float ef = (float)((int)(b >> 23) - 148);  // E - 21
-    const float4 c = tab[i];
-    const float t = fmaf(m, c.x, -1.0f);
+// ln_u21 in two steps, so a caller can issue the table read early and do
+// independent work while it is in flight: the cell (one LDS read), then the
+// arithmetic.
+struct LnCell {
+    float4 c;    // rc, hi, lo
+    unsigned b;  // bits of float(n)
+};
+__device__ __forceinline__ LnCell ln_u21_cell(unsigned n, const float4 *__restrict__ tab) {
+    LnCell k;
+    k.b = __float_as_uint((float)n);
+    k.c = tab[(((k.b >> 16) & 0x7fu) + 1u) >> 1];
+    return k;
+}
+__device__ __forceinline__ float ln_u21_finish(const LnCell &k) {
+    const float m = __uint_as_float((k.b & 0x7fffffu) | 0x3f800000u);
+    const float ef = (float)((int)(k.b >> 23) - 148);  // E - 21
+    const float t = fmaf(m, k.c.x, -1.0f);
     const float t2 = t * t;
     float p = fmaf(t, -0.25f, 0x1.555556p-2f);
     p = fmaf(p, t, -0.5f);
     const float lp = fmaf(t2, p, t);
-    const float hi = fmaf(ef, 0x1.62e430p-1f, c.y);
-    const float lo = fmaf(ef, -0x1.05c610p-29f, c.z);
+    const float hi = fmaf(ef, 0x1.62e430p-1f, k.c.y);
+    const float lo = fmaf(ef, -0x1.05c610p-29f, k.c.z);
     return hi + (lo + lp);
+}
+__device__ __forceinline__ float ln_u21(unsigned n, const float4 *__restrict__ tab) {
+    return ln_u21_finish(ln_u21_cell(n, tab));
 }
 
 // Gaussian angle step of the batched fast path from ONE 64-bit draw x:
@@ -234,11 +247,15 @@ __device__ __forceinline__ GaussDraw gauss_draw(uint64_t x, const float4 *__rest
     const unsigned k1 = hi >> 11;
     const unsigned k2 = ((hi & 0x7ffu) << 10) | (lo >> 22);
     const unsigned k3 = (lo >> 1) & 0x1fffffu;
+    const LnCell cell = ln_u21_cell(k1 + 1u, tab);
+    // the sine and v while the cell read is in flight
     const float phi = fmaf((float)(k2 & 0xfffffu), 0x1.921fb6p-19f, -0x1.921fb6p+0f);  // pi 2^-20, -pi/2
-    const float z = newton_sqrtf(-2.0f * ln_u21(k1 + 1u, tab)) * halfpi_sinf(phi);
+    const float sn = halfpi_sinf(phi);
     GaussDraw d;
-    d.z = (k2 >> 20) ? -z : z;
     d.v = (float)k3 * 0x1p-21f;
+    __builtin_amdgcn_sched_barrier(0);
+    const float z = newton_sqrtf(-2.0f * ln_u21_finish(cell)) * sn;
+    d.z = (k2 >> 20) ? -z : z;
     return d;
 }
 
