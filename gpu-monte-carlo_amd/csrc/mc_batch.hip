@@ -159,7 +159,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     __shared__ long long red_e[NT / 64];
     __shared__ float thr[5];
     __shared__ int red_i[NT / 64];
-    __shared__ float4 ln_tab[MODEL == MODEL_XY_GAUSS ? kLnCells : 1];  // log cells of gauss_draw
+    __shared__ float4 q_tab[MODEL == MODEL_XY_GAUSS ? kNormalCells : 1];  // normal quantile table of gauss_draw
 
     const size_t rep = (size_t)r * L * L;
     const int row0 = s * H;  // first global row of the strip
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             slot_draw[p][m] = SiteDraw{0.0f, 0.0f};
             slot_pr[p][m] = slot_pc[p][m] = slot_ps[p][m] = slot_dc[p][m] = slot_ds[p][m] = 0.0f;
         }
-    if (MODEL == MODEL_XY_GAUSS) ln_table_to_lds(ln_tab);
+    if (MODEL == MODEL_XY_GAUSS) normal_table_to_lds(q_tab);
     __syncthreads();
     if (MODEL != MODEL_ISING) {
 #pragma unroll
@@ -435,7 +435,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
                     Rng g;
                     g.s0 = slot_s0[p][m];
                     g.s1 = slot_s1[p][m];
-                    const SiteDraw d = site_draw<MODEL>(g, ln_tab);
+                    const SiteDraw d = site_draw<MODEL>(g, q_tab);
                     slot_draw[p][m] = d;
                     slot_s0[p][m] = g.s0;
                     slot_s1[p][m] = g.s1;

@@ -4,7 +4,8 @@
 // x^10) and exp (to x^7) in Horner form; log splits x = 2^e m with
 // m in [sqrt(1/2), sqrt(2)) and sums the atanh series of s = (m-1)/(m+1) to
 // s^11.  Errors: sin/cos <= 1.5 ulp on [-2 pi, 2 pi], exp <= 1 ulp for
-// normal results, log <= 1.5 ulp for normal x.
+// normal results, log <= 1.5 ulp for normal x.  The Gaussian step of the
+// batched XY update samples the normal quantile from a table of cubic pieces.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -97,165 +98,218 @@ __device__ __forceinline__ float fast_logf(float x) {
     return fmaf(ef, 0x1.62e430p-1f, fmaf(ef, -0x1.05c610p-29f, lm));
 }
 
-// sqrt(y) for y >= 0 as y * rsqrt(y): seed from the exponent/mantissa bits
-// (Lomont's constant 0x5f375a86), three Newton steps x <- x (1.5 - y/2 x^2) with
-// fused multiply-adds.  Relative error < 2e-7; exactly reproducible on a CPU.
-__device__ __forceinline__ float newton_sqrtf(float y) {
-    float x = __uint_as_float(0x5f375a86u - (__float_as_uint(y) >> 1));
-    const float h = 0.5f * y;
-    x = x * fmaf(-(h * x), x, 1.5f);
-    x = x * fmaf(-(h * x), x, 1.5f);
-    x = x * fmaf(-(h * x), x, 1.5f);
-    return y * x;
-}
-
-// sin(x) for |x| <= pi/2: odd Taylor series to x^13 in Horner form (error < 7e-10).
-__device__ __forceinline__ float halfpi_sinf(float x) {
-    const float x2 = x * x;
-    float p = fmaf(x2, 0x1.612462p-33f, -0x1.ae6456p-26f);  // 1/13!, -1/11!
-    p = fmaf(x2, p, 0x1.71de3ap-19f);                        // 1/9!
-    p = fmaf(x2, p, -0x1.a01a02p-13f);                       // -1/7!
-    p = fmaf(x2, p, 0x1.111112p-7f);                         // 1/5!
-    p = fmaf(x2, p, -0x1.555556p-3f);                        // -1/3!
-    return fmaf(x * x2, p, x);
-}
-
-// ln(n 2^-21) for an integer n in [1, 2^21], without a division (the
-// Gaussian step's radius).  n = 2^E m with m in [1, 2); i = round(64 (m - 1))
-// picks one of 65 cells c = 1 + i/64, so |m rc - 1| <= 2^-7 with
-// rc = float(1/c); the table holds rc and -ln(rc) = hi + lo (two floats), and
-//   ln(n 2^-21) = ((E - 21) ln2_hi + hi) + (((E - 21) ln2_lo + lo) + log1p(t)),
-//   t = m rc - 1 (one rounding), log1p(t) = t - t^2/2 + t^3/3 - t^4/4.
-// Cell 64 (c = 2) holds ln 2 in the constant's own two parts, so for n just
-// below 2^21 the large terms cancel exactly.  Error <= 1.3 ulp over all 2^21
-// inputs.  The kernels copy the table to LDS (ln_table_to_lds) and index it
-// there.
-constexpr int kLnCells = 65;
-static __constant__ float4 kLnU21[kLnCells] = {
-    {0x1p+0f, 0x0p+0f, 0x0p+0f, 0.0f},
-    {0x1.f81f82p-1f, 0x1.fc0a8ap-7f, -0x1.e07f84p-32f, 0.0f},
-    {0x1.f07c2p-1f, 0x1.f8299p-6f, 0x1.cf067p-31f, 0.0f},
-    {0x1.e9131ap-1f, 0x1.77459cp-5f, -0x1.cd22dcp-33f, 0.0f},
-    {0x1.e1e1e2p-1f, 0x1.f0a30ap-5f, 0x1.162a76p-37f, 0.0f},
-    {0x1.dae608p-1f, 0x1.341d74p-4f, 0x1.86f478p-30f, 0.0f},
-    {0x1.d41d42p-1f, 0x1.6f0d28p-4f, -0x1.a35296p-29f, 0.0f},
-    {0x1.cd8568p-1f, 0x1.a926d8p-4f, 0x1.495aaep-29f, 0.0f},
-    {0x1.c71c72p-1f, 0x1.e27074p-4f, 0x1.c55e5cp-29f, 0.0f},
-    {0x1.c0e07p-1f, 0x1.0d77e8p-3f, 0x1.9a11ccp-28f, 0.0f},
-    {0x1.bacf92p-1f, 0x1.29552cp-3f, 0x1.07fd4cp-29f, 0.0f},
-    {0x1.b4e81cp-1f, 0x1.44d2b4p-3f, -0x1.cd20b6p-29f, 0.0f},
-    {0x1.af286cp-1f, 0x1.5ff306p-3f, 0x1.4f27aap-32f, 0.0f},
-    {0x1.a98ef6p-1f, 0x1.7ab89p-3f, 0x1.043642p-29f, 0.0f},
-    {0x1.a41a42p-1f, 0x1.9525a8p-3f, 0x1.e8ad7p-32f, 0.0f},
-    {0x1.9ec8eap-1f, 0x1.af3c92p-3f, -0x1.dfdp-29f, 0.0f},
-    {0x1.99999ap-1f, 0x1.c8ff7ap-3f, 0x1.e6a68ap-29f, 0.0f},
-    {0x1.948b1p-1f, 0x1.e27076p-3f, -0x1.d50d1ap-31f, 0.0f},
-    {0x1.8f9c18p-1f, 0x1.fb918cp-3f, -0x1.50e0dep-30f, 0.0f},
-    {0x1.8acb9p-1f, 0x1.0a325p-2f, 0x1.4e721ep-27f, 0.0f},
-    {0x1.861862p-1f, 0x1.1675cap-2f, -0x1.0a8b3ep-27f, 0.0f},
-    {0x1.818182p-1f, 0x1.22941ep-2f, 0x1.b3de5ap-28f, 0.0f},
-    {0x1.7d05f4p-1f, 0x1.2e8e2cp-2f, -0x1.1ee2dp-30f, 0.0f},
-    {0x1.78a4c8p-1f, 0x1.3a64c6p-2f, -0x1.a5ae86p-28f, 0.0f},
-    {0x1.745d18p-1f, 0x1.4618bap-2f, 0x1.0e2f66p-29f, 0.0f},
-    {0x1.702e06p-1f, 0x1.51aad8p-2f, -0x1.e903eap-29f, 0.0f},
-    {0x1.6c16c2p-1f, 0x1.5d1bdap-2f, 0x1.560274p-28f, 0.0f},
-    {0x1.681682p-1f, 0x1.686c8p-2f, 0x1.cd8a5ap-29f, 0.0f},
-    {0x1.642c86p-1f, 0x1.739d7ep-2f, 0x1.5de804p-29f, 0.0f},
-    {0x1.605816p-1f, 0x1.7eaf84p-2f, -0x1.bea81ep-29f, 0.0f},
-    {0x1.5c9882p-1f, 0x1.89a33ap-2f, 0x1.18284cp-27f, 0.0f},
-    {0x1.58ed24p-1f, 0x1.94793ep-2f, 0x1.c422e2p-27f, 0.0f},
-    {0x1.555556p-1f, 0x1.9f323cp-2f, 0x1.97f30ap-27f, 0.0f},
-    {0x1.51d07ep-1f, 0x1.a9ceccp-2f, -0x1.197decp-28f, 0.0f},
-    {0x1.4e5e0ap-1f, 0x1.b44f7ap-2f, -0x1.c66e14p-27f, 0.0f},
-    {0x1.4afd6ap-1f, 0x1.beb4dap-2f, -0x1.58e484p-30f, 0.0f},
-    {0x1.47ae14p-1f, 0x1.c8ff7ep-2f, -0x1.959768p-32f, 0.0f},
-    {0x1.446f86p-1f, 0x1.d32fe8p-2f, 0x1.e01d7ap-27f, 0.0f},
-    {0x1.414142p-1f, 0x1.dd469ep-2f, -0x1.3e3b54p-30f, 0.0f},
-    {0x1.3e22ccp-1f, 0x1.e74426p-2f, -0x1.052f0ep-27f, 0.0f},
-    {0x1.3b13b2p-1f, 0x1.f128f4p-2f, -0x1.0a1f2p-27f, 0.0f},
-    {0x1.381382p-1f, 0x1.faf586p-2f, 0x1.9e3ccap-28f, 0.0f},
-    {0x1.3521dp-1f, 0x1.02552ap-1f, -0x1.2d178p-28f, 0.0f},
-    {0x1.323e34p-1f, 0x1.0723e6p-1f, 0x1.a39be8p-26f, 0.0f},
-    {0x1.2f684cp-1f, 0x1.0be72ep-1f, 0x1.295418p-32f, 0.0f},
-    {0x1.2c9fb4p-1f, 0x1.109f3cp-1f, -0x1.5a566cp-26f, 0.0f},
-    {0x1.29e412p-1f, 0x1.154c3ep-1f, 0x1.fa6af8p-28f, 0.0f},
-    {0x1.27350cp-1f, 0x1.19ee6ap-1f, 0x1.d9f25cp-27f, 0.0f},
-    {0x1.24924ap-1f, 0x1.1e85f4p-1f, 0x1.9c1036p-27f, 0.0f},
-    {0x1.21fb78p-1f, 0x1.23130ep-1f, -0x1.90503p-27f, 0.0f},
-    {0x1.1f7048p-1f, 0x1.2795ep-1f, 0x1.d13624p-26f, 0.0f},
-    {0x1.1cf06ap-1f, 0x1.2c0eap-1f, 0x1.7123a6p-27f, 0.0f},
-    {0x1.1a7b96p-1f, 0x1.307d74p-1f, -0x1.561de8p-26f, 0.0f},
-    {0x1.181182p-1f, 0x1.34e288p-1f, 0x1.8e70fp-28f, 0.0f},
-    {0x1.15b1e6p-1f, 0x1.393e0ep-1f, -0x1.b53abcp-26f, 0.0f},
-    {0x1.135c82p-1f, 0x1.3d9024p-1f, 0x1.de2aep-26f, 0.0f},
-    {0x1.111112p-1f, 0x1.41d8fcp-1f, 0x1.88ce58p-26f, 0.0f},
-    {0x1.0ecf56p-1f, 0x1.4618bep-1f, -0x1.d8e84ep-27f, 0.0f},
-    {0x1.0c9714p-1f, 0x1.4a4f88p-1f, -0x1.13f04ep-27f, 0.0f},
-    {0x1.0a681p-1f, 0x1.4e7d82p-1f, 0x1.6dd6eep-27f, 0.0f},
-    {0x1.08421p-1f, 0x1.52a2d4p-1f, -0x1.34874ap-26f, 0.0f},
-    {0x1.0624dep-1f, 0x1.56bf9cp-1f, -0x1.e6062ep-28f, 0.0f},
-    {0x1.041042p-1f, 0x1.5ad402p-1f, 0x1.a6b3e8p-26f, 0.0f},
-    {0x1.020408p-1f, 0x1.5ee02ap-1f, 0x1.6482cep-26f, 0.0f},
-    {0x1p-1f, 0x1.62e43p-1f, -0x1.05c61p-29f, 0.0f},
+// The standard normal quantile for the Gaussian step, as a table of cubic
+// pieces.  Input: a 21-bit integer k and the point w = (2k + 1) 2^-23 in
+// (0, 1/2); output Q(w) = -Phi^-1(w) > 0 (Phi the normal CDF).  float(2k + 1)
+// is exact; with its bits b, cell = 8 E + (top three mantissa bits) =
+// (b >> 20) - 1016, where 2^E <= 2k + 1 < 2^(E+1) (E = 0..21, 176 cells,
+// eight per octave of w), and t in [0, 1) is the remaining 20 mantissa bits.
+// Each cell holds c0..c3 of Q = ((c3 t + c2) t + c1) t + c0, fitted to Q in
+// float64 at 64 Chebyshev points of the cell and rounded to float32 (cells
+// that no 2k + 1 reaches are fitted the same way).  Error over all 2^21
+// inputs: <= 2.01 ulp of max(Q, 1) (|error| < 4.7e-7; max Q = 5.17).  The
+// kernels copy the table to LDS (normal_table_to_lds) and index it there.
+constexpr int kNormalCells = 176;
+static __constant__ float4 kNormalQ[kNormalCells] = {
+    {0x1.4aa936p+2f, -0x1.7eed2ep-6f, 0x1.6f09f4p-10f, -0x1.969384p-14f},
+    {0x1.493fa4p+2f, -0x1.55c132p-6f, 0x1.238bcep-10f, -0x1.241342p-14f},
+    {0x1.47faf8p+2f, -0x1.34b276p-6f, 0x1.da7048p-11f, -0x1.b1d31p-15f},
+    {0x1.46d44p+2f, -0x1.198fb6p-6f, 0x1.89a3d4p-11f, -0x1.4b0df2p-15f},
+    {0x1.45c658p+2f, -0x1.02e17cp-6f, 0x1.4beb5cp-11f, -0x1.02684cp-15f},
+    {0x1.44cd54p+2f, -0x1.df4622p-7f, 0x1.1bb44ap-11f, -0x1.9b30d8p-16f},
+    {0x1.43e628p+2f, -0x1.be3372p-7f, 0x1.eaa308p-12f, -0x1.4c8e36p-16f},
+    {0x1.430e66p+2f, -0x1.a17858p-7f, 0x1.ac83acp-12f, -0x1.10cd08p-16f},
+    {0x1.424418p+2f, -0x1.883e4ap-6f, 0x1.77579ep-10f, -0x1.9f823p-14f},
+    {0x1.40d1aep+2f, -0x1.5e2314p-6f, 0x1.2a30bep-10f, -0x1.2a893ap-14f},
+    {0x1.3f8504p+2f, -0x1.3c52e8p-6f, 0x1.e552b4p-11f, -0x1.bb7afep-15f},
+    {0x1.3e56fep+2f, -0x1.20905ap-6f, 0x1.92b9c4p-11f, -0x1.5276fcp-15f},
+    {0x1.3d425ap+2f, -0x1.095bc4p-6f, 0x1.539f8cp-11f, -0x1.0838cap-15f},
+    {0x1.3c4318p+2f, -0x1.eb5578p-7f, 0x1.2252b2p-11f, -0x1.a47cep-16f},
+    {0x1.3b5616p+2f, -0x1.c97cd2p-7f, 0x1.f6236ep-12f, -0x1.541bc6p-16f},
+    {0x1.3a78dcp+2f, -0x1.ac14eep-7f, 0x1.b69ad2p-12f, -0x1.1705e2p-16f},
+    {0x1.39a966p+2f, -0x1.924286p-6f, 0x1.80387ap-10f, -0x1.a90a6ap-14f},
+    {0x1.382d7ep+2f, -0x1.672778p-6f, 0x1.314c5ap-10f, -0x1.316f08p-14f},
+    {0x1.36d83ap+2f, -0x1.448852p-6f, 0x1.f0f8dap-11f, -0x1.c5cb1cp-15f},
+    {0x1.35a256p+2f, -0x1.281ab2p-6f, 0x1.9c7414p-11f, -0x1.5a61d8p-15f},
+    {0x1.348672p+2f, -0x1.10563cp-6f, 0x1.5bdfe6p-11f, -0x1.0e6facp-15f},
+    {0x1.338074p+2f, -0x1.f854dap-7f, 0x1.296a2ap-11f, -0x1.ae6d6ep-16f},
+    {0x1.328d28p+2f, -0x1.d5a81p-7f, 0x1.013b9ap-11f, -0x1.5c2f9ep-16f},
+    {0x1.31aa08p+2f, -0x1.b786f8p-7f, 0x1.c16c44p-12f, -0x1.1daddp-16f},
+    {0x1.30d502p+2f, -0x1.9d113p-6f, 0x1.89bep-10f, -0x1.b33ddp-14f},
+    {0x1.2f4edap+2f, -0x1.70e3bp-6f, 0x1.38ecd2p-10f, -0x1.38d19ap-14f},
+    {0x1.2df04cp+2f, -0x1.4d6666p-6f, 0x1.fd7a44p-11f, -0x1.d0d6fcp-15f},
+    {0x1.2cb1e8p+2f, -0x1.304118p-6f, 0x1.a6e6aep-11f, -0x1.62ddb8p-15f},
+    {0x1.2b8e2ep+2f, -0x1.17e216p-6f, 0x1.64bd8p-11f, -0x1.151902p-15f},
+    {0x1.2a80e6p+2f, -0x1.033256p-6f, 0x1.310982p-11f, -0x1.b915f8p-16f},
+    {0x1.2986cep+2f, -0x1.e2d3d2p-7f, 0x1.07dc3p-11f, -0x1.64d9bp-16f},
+    {0x1.289d4ap+2f, -0x1.c3eb8ep-7f, 0x1.cd0f08p-12f, -0x1.24d218p-16f},
+    {0x1.27c24p+2f, -0x1.a8c608p-6f, 0x1.93fca4p-10f, -0x1.be30f4p-14f},
+    {0x1.2630fcp+2f, -0x1.7b712ap-6f, 0x1.4122c8p-10f, -0x1.40c006p-14f},
+    {0x1.24c85cp+2f, -0x1.5704aep-6f, 0x1.05794ep-10f, -0x1.dcb57ep-15f},
+    {0x1.2380cp+2f, -0x1.39197ap-6f, 0x1.b228f6p-11f, -0x1.6bfc5cp-15f},
+    {0x1.225482p+2f, -0x1.2013e2p-6f, 0x1.6e4c6ap-11f, -0x1.1c42e2p-15f},
+    {0x1.213f52p+2f, -0x1.0ad5dcp-6f, 0x1.39423p-11f, -0x1.c48d48p-16f},
+    {0x1.203dd6p+2f, -0x1.f124d8p-7f, 0x1.0f02ccp-11f, -0x1.6e2caep-16f},
+    {0x1.1f4d6p+2f, -0x1.d165a4p-7f, 0x1.d99e48p-12f, -0x1.2c824p-16f},
+    {0x1.1e6bc8p+2f, -0x1.b58264p-6f, 0x1.9f0c8ap-10f, -0x1.c9fbfp-14f},
+    {0x1.1cce6cp+2f, -0x1.86ee82p-6f, 0x1.4a01e2p-10f, -0x1.494c0ap-14f},
+    {0x1.1b5ad4p+2f, -0x1.617f8p-6f, 0x1.0cc14cp-10f, -0x1.e9817ep-15f},
+    {0x1.1a092cp+2f, -0x1.42be5p-6f, 0x1.be569cp-11f, -0x1.75d2aep-15f},
+    {0x1.18d3a6p+2f, -0x1.29047ep-6f, 0x1.78a472p-11f, -0x1.23fde6p-15f},
+    {0x1.17b5d4p+2f, -0x1.132c76p-6f, 0x1.4228ep-11f, -0x1.d0ee34p-16f},
+    {0x1.16ac46p+2f, -0x1.0063d2p-6f, 0x1.16c19ap-11f, -0x1.783ea2p-16f},
+    {0x1.15b43ap+2f, -0x1.e01f92p-7f, 0x1.e73a4p-12f, -0x1.34d0a4p-16f},
+    {0x1.14cb78p+2f, -0x1.c36eb6p-6f, 0x1.ab0a5ap-10f, -0x1.d6bb42p-14f},
+    {0x1.1320e4p+2f, -0x1.9380e6p-6f, 0x1.53a18p-10f, -0x1.528a96p-14f},
+    {0x1.11a14ap+2f, -0x1.6cf962p-6f, 0x1.14a892p-10f, -0x1.f75ad2p-15f},
+    {0x1.1044ap+2f, -0x1.4d4fdcp-6f, 0x1.cb90a4p-11f, -0x1.807972p-15f},
+    {0x1.0f04ecp+2f, -0x1.32d23ep-6f, 0x1.83e1fap-11f, -0x1.2c5dc2p-15f},
+    {0x1.0ddda4p+2f, -0x1.1c52d2p-6f, 0x1.4bd64cp-11f, -0x1.de58ap-16f},
+    {0x1.0ccb38p+2f, -0x1.08f952p-6f, 0x1.1f2e5ap-11f, -0x1.8329bep-16f},
+    {0x1.0bcad8p+2f, -0x1.f04d4p-7f, 0x1.f60982p-12f, -0x1.3dd30ep-16f},
+    {0x1.0ada38p+2f, -0x1.d2bc96p-6f, 0x1.b8186ap-10f, -0x1.e490cep-14f},
+    {0x1.09211ap+2f, -0x1.a1560cp-6f, 0x1.5e1daap-10f, -0x1.5c94a8p-14f},
+    {0x1.079448p+2f, -0x1.799ccep-6f, 0x1.1d4682p-10f, -0x1.0333b6p-14f},
+    {0x1.062b7ep+2f, -0x1.58f5e6p-6f, 0x1.d9fecp-11f, -0x1.8c0e3ep-15f},
+    {0x1.04e092p+2f, -0x1.3da29cp-6f, 0x1.90272ep-11f, -0x1.3579fep-15f},
+    {0x1.03aed6p+2f, -0x1.266c6cp-6f, 0x1.56683cp-11f, -0x1.ecf2aap-16f},
+    {0x1.0292ap+2f, -0x1.1274acp-6f, 0x1.28634p-11f, -0x1.8f0d6p-16f},
+    {0x1.01890cp+2f, -0x1.011784p-6f, 0x1.031d52p-11f, -0x1.47a3a4p-16f},
+    {0x1.008fbap+2f, -0x1.e3a9a2p-6f, 0x1.c6603ap-10f, -0x1.f3a546p-14f},
+    {0x1.fd8d06p+1f, -0x1.b0a6d6p-6f, 0x1.69984ep-10f, -0x1.678842p-14f},
+    {0x1.fa561ep+1f, -0x1.879ec6p-6f, 0x1.26b774p-10f, -0x1.0b6a7ep-14f},
+    {0x1.f769ap+1f, -0x1.65e226p-6f, 0x1.e9d118p-11f, -0x1.98b4b2p-15f},
+    {0x1.f4baep+1f, -0x1.49a486p-6f, 0x1.9d9d92p-11f, -0x1.3f6fp-15f},
+    {0x1.f24032p+1f, -0x1.31a5bep-6f, 0x1.6202e4p-11f, -0x1.fcea6p-16f},
+    {0x1.eff208p+1f, -0x1.1d0036p-6f, 0x1.328038p-11f, -0x1.9c0f6cp-16f},
+    {0x1.edca62p+1f, -0x1.0b0aeep-6f, 0x1.0c0332p-11f, -0x1.5262p-16f},
+    {0x1.ebc462p+1f, -0x1.f6836cp-6f, 0x1.d61462p-10f, -0x1.0214f4p-13f},
+    {0x1.e80e16p+1f, -0x1.c1bb16p-6f, 0x1.763aep-10f, -0x1.7389cap-14f},
+    {0x1.e4b68p+1f, -0x1.974258p-6f, 0x1.311e16p-10f, -0x1.146d7ap-14f},
+    {0x1.e1abf6p+1f, -0x1.7453aap-6f, 0x1.fb42aep-11f, -0x1.a6982p-15f},
+    {0x1.dee15cp+1f, -0x1.571398p-6f, 0x1.ac7814p-11f, -0x1.4a5f5p-15f},
+    {0x1.dc4cb2p+1f, -0x1.3e376ep-6f, 0x1.6ed2c2p-11f, -0x1.073beap-15f},
+    {0x1.d9e62ap+1f, -0x1.28d1f6p-6f, 0x1.3dac8p-11f, -0x1.aa5e1ap-16f},
+    {0x1.d7a78ap+1f, -0x1.163492p-6f, 0x1.15d94cp-11f, -0x1.5e34b6p-16f},
+    {0x1.d58bdp+1f, -0x1.05d69ap-5f, 0x1.e7733cp-10f, -0x1.0b2d7p-13f},
+    {0x1.d1ad38p+1f, -0x1.d4eef6p-6f, 0x1.843886p-10f, -0x1.80c5cap-14f},
+    {0x1.ce30dep+1f, -0x1.a8ddcep-6f, 0x1.3ca53ep-10f, -0x1.1e5ea6p-14f},
+    {0x1.cb047cp+1f, -0x1.849bc6p-6f, 0x1.074e48p-10f, -0x1.b5edb2p-15f},
+    {0x1.c81a78p+1f, -0x1.663cep-6f, 0x1.bcf5d4p-11f, -0x1.567566p-15f},
+    {0x1.c56876p+1f, -0x1.4c6ad4p-6f, 0x1.7d0f04p-11f, -0x1.10f002p-15f},
+    {0x1.c2e662p+1f, -0x1.36300ap-6f, 0x1.4a18d8p-11f, -0x1.ba324ep-16f},
+    {0x1.c08dc6p+1f, -0x1.22d7aap-6f, 0x1.20cb1p-11f, -0x1.6b4b54p-16f},
+    {0x1.be596cp+1f, -0x1.11d426p-5f, 0x1.faca62p-10f, -0x1.154142p-13f},
+    {0x1.ba4d2p+1f, -0x1.eabaeep-6f, 0x1.93d10ap-10f, -0x1.8f7356p-14f},
+    {0x1.b6a706p+1f, -0x1.bce248p-6f, 0x1.49827ap-10f, -0x1.296818p-14f},
+    {0x1.b3542p+1f, -0x1.972564p-6f, 0x1.121d0cp-10f, -0x1.c6f756p-15f},
+    {0x1.b04652p+1f, -0x1.7785fap-6f, 0x1.cf65e2p-11f, -0x1.63e5f6p-15f},
+    {0x1.ad72d8p+1f, -0x1.5ca0ecp-6f, 0x1.8cfce4p-11f, -0x1.1bbc5ep-15f},
+    {0x1.aad14ap+1f, -0x1.457758p-6f, 0x1.580274p-11f, -0x1.cbd2e6p-16f},
+    {0x1.a85af6p+1f, -0x1.314d7cp-6f, 0x1.2d0f14p-11f, -0x1.79e11cp-16f},
+    {0x1.a60a6ep+1f, -0x1.1f906p-5f, 0x1.083dcp-9f, -0x1.207ec2p-13f},
+    {0x1.a1c9bap+1f, -0x1.01dff6p-5f, 0x1.a554d8p-10f, -0x1.9fd74cp-14f},
+    {0x1.9df3a6p+1f, -0x1.d3e774p-6f, 0x1.57f962p-10f, -0x1.35be84p-14f},
+    {0x1.9a746ap+1f, -0x1.ac805cp-6f, 0x1.1e476cp-10f, -0x1.da07bep-15f},
+    {0x1.973d58p+1f, -0x1.8b77fep-6f, 0x1.e42c5ep-11f, -0x1.72f338p-15f},
+    {0x1.944338p+1f, -0x1.6f5ceep-6f, 0x1.9ef428p-11f, -0x1.27d6eap-15f},
+    {0x1.917d46p+1f, -0x1.5725e8p-6f, 0x1.67b704p-11f, -0x1.df990ap-16f},
+    {0x1.8ee486p+1f, -0x1.420f7ep-6f, 0x1.3aeab4p-11f, -0x1.8a40bcp-16f},
+    {0x1.8c735p+1f, -0x1.2f809ep-5f, 0x1.148164p-9f, -0x1.2d2074p-13f},
+    {0x1.87f5b8p+1f, -0x1.106d36p-5f, 0x1.b92a62p-10f, -0x1.b248b8p-14f},
+    {0x1.83e7c6p+1f, -0x1.eebe04p-6f, 0x1.68602ep-10f, -0x1.43a4aap-14f},
+    {0x1.8034cep+1f, -0x1.c5735ep-6f, 0x1.2c172ep-10f, -0x1.ef87c2p-15f},
+    {0x1.7ccd7ap+1f, -0x1.a2d1p-6f, 0x1.fbc946p-11f, -0x1.83f138p-15f},
+    {0x1.79a612p+1f, -0x1.855568p-6f, 0x1.b36518p-11f, -0x1.358436p-15f},
+    {0x1.76b568p+1f, -0x1.6bebbp-6f, 0x1.799a04p-11f, -0x1.f5f638p-16f},
+    {0x1.73f42ep+1f, -0x1.55c7bcp-6f, 0x1.4ab6ccp-11f, -0x1.9cc956p-16f},
+    {0x1.715c7cp+1f, -0x1.4249a6p-5f, 0x1.227fbap-9f, -0x1.3b70fep-13f},
+    {0x1.6c9708p+1f, -0x1.21a0e8p-5f, 0x1.cfd51ep-10f, -0x1.c736e4p-14f},
+    {0x1.6846fp+1f, -0x1.07471cp-5f, 0x1.7b25ap-10f, -0x1.536ff6p-14f},
+    {0x1.645692p+1f, -0x1.e319d2p-6f, 0x1.3beb22p-10f, -0x1.03fee8p-14f},
+    {0x1.60b5d4p+1f, -0x1.bea154p-6f, 0x1.0b7092p-10f, -0x1.974bdep-15f},
+    {0x1.5d5868p+1f, -0x1.9f910ap-6f, 0x1.cae01ap-11f, -0x1.451c5ep-15f},
+    {0x1.5a34aep+1f, -0x1.84c6dap-6f, 0x1.8e2b62p-11f, -0x1.07be3ap-15f},
+    {0x1.5742fcp+1f, -0x1.6d6cd2p-6f, 0x1.5ce5a6p-11f, -0x1.b1f572p-16f},
+    {0x1.547d16p+1f, -0x1.58db46p-5f, 0x1.329f2ap-9f, -0x1.4bd0a4p-13f},
+    {0x1.4f6122p+1f, -0x1.365f2ap-5f, 0x1.e9fe08p-10f, -0x1.df318p-14f},
+    {0x1.4ac126p+1f, -0x1.1a866ap-5f, 0x1.90d7e6p-10f, -0x1.658ee6p-14f},
+    {0x1.46865ep+1f, -0x1.038cp-5f, 0x1.4e3ce4p-10f, -0x1.120bf8p-14f},
+    {0x1.429fdp+1f, -0x1.e07f12p-6f, 0x1.1b2358p-10f, -0x1.ad8f06p-15f},
+    {0x1.3f008ap+1f, -0x1.bf998ep-6f, 0x1.e61dc6p-11f, -0x1.5711bp-15f},
+    {0x1.3b9e62p+1f, -0x1.a3363ep-6f, 0x1.a60e5cp-11f, -0x1.1674a8p-15f},
+    {0x1.38714p+1f, -0x1.8a73fp-6f, 0x1.7208cap-11f, -0x1.ca643ap-16f},
+    {0x1.357292p+1f, -0x1.74a20cp-5f, 0x1.4561bep-9f, -0x1.5ebc9p-13f},
+    {0x1.2febe8p+1f, -0x1.5005cp-5f, 0x1.043d54p-9f, -0x1.faf3ap-14f},
+    {0x1.2ae8eap+1f, -0x1.326e6ap-5f, 0x1.aa29bap-10f, -0x1.7a916cp-14f},
+    {0x1.26518p+1f, -0x1.19fe1cp-5f, 0x1.63a44ep-10f, -0x1.225c86p-14f},
+    {0x1.2213b8p+1f, -0x1.05737p-5f, 0x1.2d7fa6p-10f, -0x1.c7712p-15f},
+    {0x1.1e21d2p+1f, -0x1.e7dc58p-6f, 0x1.0300b6p-10f, -0x1.6bf95ap-15f},
+    {0x1.1a710ep+1f, -0x1.c999ecp-6f, 0x1.c20a6p-11f, -0x1.279a2ap-15f},
+    {0x1.16f8d4p+1f, -0x1.af3168p-6f, 0x1.8ad082p-11f, -0x1.e6e5a4p-16f},
+    {0x1.13b22ap+1f, -0x1.97e7dep-5f, 0x1.5b639p-9f, -0x1.74d86p-13f},
+    {0x1.0da39p+1f, -0x1.70cceep-5f, 0x1.16232ap-9f, -0x1.0db8f8p-13f},
+    {0x1.0821aep+1f, -0x1.51291cp-5f, 0x1.c7e8d6p-10f, -0x1.9333eep-14f},
+    {0x1.0312ep+1f, -0x1.370178p-5f, 0x1.7ccbfcp-10f, -0x1.358528p-14f},
+    {0x1.fcc812p+0f, -0x1.2100dp-5f, 0x1.431458p-10f, -0x1.e5e15p-15f},
+    {0x1.f40d06p+0f, -0x1.0e38e4p-5f, 0x1.15be0ep-10f, -0x1.849716p-15f},
+    {0x1.ebdda4p+0f, -0x1.fbfc58p-6f, 0x1.e2ebeep-11f, -0x1.3bd14ap-15f},
+    {0x1.e4279ap+0f, -0x1.dfa3d6p-6f, 0x1.a7eaecp-11f, -0x1.04458ep-15f},
+    {0x1.dcdbfcp+0f, -0x1.c6a14p-5f, 0x1.75383p-9f, -0x1.8efbbp-13f},
+    {0x1.cf5518p+0f, -0x1.9c9946p-5f, 0x1.2b1bf6p-9f, -0x1.20f80ep-13f},
+    {0x1.c2fcd4p+0f, -0x1.7a8f84p-5f, 0x1.eaafd2p-10f, -0x1.b070ep-14f},
+    {0x1.b79c42p+0f, -0x1.5e66e2p-5f, 0x1.9a1fdp-10f, -0x1.4c4cap-14f},
+    {0x1.ad0a62p+0f, -0x1.46b2dcp-5f, 0x1.5c283cp-10f, -0x1.0510fp-14f},
+    {0x1.a327c2p+0f, -0x1.3274acp-5f, 0x1.2b6f9ep-10f, -0x1.a1f514p-15f},
+    {0x1.99dbb4p+0f, -0x1.20f4b8p-5f, 0x1.046954p-10f, -0x1.53f97ap-15f},
+    {0x1.91128p+0f, -0x1.11ab4p-5f, 0x1.c94de2p-11f, -0x1.18689ap-15f},
+    {0x1.88bc1ep+0f, -0x1.042d96p-4f, 0x1.92b0a2p-9f, -0x1.ae5932p-13f},
+    {0x1.79352ap+0f, -0x1.db0134p-5f, 0x1.42bfacp-9f, -0x1.38259ep-13f},
+    {0x1.6af4cp+0f, -0x1.b647dp-5f, 0x1.08aacep-9f, -0x1.d3c9b6p-14f},
+    {0x1.5dbf88p+0f, -0x1.97e95cp-5f, 0x1.ba2ec6p-10f, -0x1.67f4b4p-14f},
+    {0x1.516928p+0f, -0x1.7e5d8ap-5f, 0x1.770e2cp-10f, -0x1.1b2dcp-14f},
+    {0x1.45cf94p+0f, -0x1.6891e4p-5f, 0x1.423498p-10f, -0x1.c5f9f6p-15f},
+    {0x1.3ad806p+0f, -0x1.55c072p-5f, 0x1.17d0f8p-10f, -0x1.71c8a6p-15f},
+    {0x1.306d12p+0f, -0x1.4556b4p-5f, 0x1.ea8ad2p-11f, -0x1.316dap-15f},
+    {0x1.267d4ap+0f, -0x1.36e172p-4f, 0x1.af11e4p-9f, -0x1.d5c90ep-13f},
+    {0x1.13d80ep+0f, -0x1.1ea8eep-4f, 0x1.57cb48p-9f, -0x1.55ddaep-13f},
+    {0x1.028eb6p+0f, -0x1.0b27aep-4f, 0x1.182d34p-9f, -0x1.011d48p-13f},
+    {0x1.e4c092p-1f, -0x1.f645b2p-5f, 0x1.d08a44p-10f, -0x1.8d650cp-14f},
+    {0x1.c63812p-1f, -0x1.db8beap-5f, 0x1.866c78p-10f, -0x1.3a35ecp-14f},
+    {0x1.a938b8p-1f, -0x1.c4f8ap-5f, 0x1.4bc6c6p-10f, -0x1.fabb1p-15f},
+    {0x1.8d8726p-1f, -0x1.b1b564p-5f, 0x1.1c74e6p-10f, -0x1.9face2p-15f},
+    {0x1.72f38cp-1f, -0x1.a123aap-5f, 0x1.eb3d8p-11f, -0x1.5a37b6p-15f},
+    {0x1.5956b4p-1f, -0x1.92c6fcp-4f, 0x1.a7f71ep-9f, -0x1.0e68dp-12f},
+    {0x1.2884p-1f, -0x1.7b6a2ap-4f, 0x1.4373a2p-9f, -0x1.94a31cp-13f},
+    {0x1.f481cap-2f, -0x1.698c5p-4f, 0x1.f03d2p-10f, -0x1.3c2d7ep-13f},
+    {0x1.9be76ep-2f, -0x1.5be0f6p-4f, 0x1.7a5c46p-10f, -0x1.014e38p-13f},
+    {0x1.464964p-2f, -0x1.518d8ep-4f, 0x1.1a569ap-10f, -0x1.b3c836p-14f},
+    {0x1.e5ca36p-3f, -0x1.4a001p-4f, 0x1.91ef3p-11f, -0x1.80233p-14f},
+    {0x1.422c1ap-3f, -0x1.44d772p-4f, 0x1.0268dp-11f, -0x1.60d38ap-14f},
+    {0x1.412d5ep-4f, -0x1.41d5f6p-4f, 0x1.fa318cp-13f, -0x1.520752p-14f},
 };
 
-__device__ __forceinline__ void ln_table_to_lds(float4 *lds) {
-    for (int k = threadIdx.x; k < kLnCells; k += blockDim.x) lds[k] = kLnU21[k];
-}
-
-// ln_u21 in two steps, so a caller can issue the table read early and do
-// independent work while it is in flight: the cell (one LDS read), then the
-// arithmetic.
-struct LnCell {
-    float4 c;    // rc, hi, lo
-    unsigned b;  // bits of float(n)
-};
-__device__ __forceinline__ LnCell ln_u21_cell(unsigned n, const float4 *__restrict__ tab) {
-    LnCell k;
-    k.b = __float_as_uint((float)n);
-    k.c = tab[(((k.b >> 16) & 0x7fu) + 1u) >> 1];
-    return k;
-}
-__device__ __forceinline__ float ln_u21_finish(const LnCell &k) {
-    const float m = __uint_as_float((k.b & 0x7fffffu) | 0x3f800000u);
-    const float ef = (float)((int)(k.b >> 23) - 148);  // E - 21
-    const float t = fmaf(m, k.c.x, -1.0f);
-    const float t2 = t * t;
-    float p = fmaf(t, -0.25f, 0x1.555556p-2f);
-    p = fmaf(p, t, -0.5f);
-    const float lp = fmaf(t2, p, t);
-    const float hi = fmaf(ef, 0x1.62e430p-1f, k.c.y);
-    const float lo = fmaf(ef, -0x1.05c610p-29f, k.c.z);
-    return hi + (lo + lp);
-}
-__device__ __forceinline__ float ln_u21(unsigned n, const float4 *__restrict__ tab) {
-    return ln_u21_finish(ln_u21_cell(n, tab));
+__device__ __forceinline__ void normal_table_to_lds(float4 *lds) {
+    for (int k = threadIdx.x; k < kNormalCells; k += blockDim.x) lds[k] = kNormalQ[k];
 }
 
 // Gaussian angle step of the batched fast path from ONE 64-bit draw x:
-//   bits 63..43 -> k1, u1 = (k1 + 1) 2^-21 in (0, 1]
-//   bits 42..22 -> sign bit b and 20-bit a: phi = pi (a 2^-20 - 1/2) in [-pi/2, pi/2)
+//   bits 63..43 -> k, z = (-1)^s Q((2k + 1) 2^-23) (inverse-CDF sampling)
+//   bit 42      -> the sign s, so the proposal is exactly symmetric
 //   bits 21..1  -> k3, acceptance uniform v = k3 2^-21 in [0, 1)
-// z = (-1)^b sqrt(-2 log u1) sin(phi): Box-Muller with the sine's sign taken
-// from an independent bit, so the proposal is exactly symmetric.  log u1 is
-// ln_u21(k1 + 1) with the cell table `tab` (in LDS).
+// `tab` is the quantile table in LDS.
 struct GaussDraw {
     float z, v;
 };
 __device__ __forceinline__ GaussDraw gauss_draw(uint64_t x, const float4 *__restrict__ tab) {
     const unsigned hi = (unsigned)(x >> 32), lo = (unsigned)x;
-    const unsigned k1 = hi >> 11;
-    const unsigned k2 = ((hi & 0x7ffu) << 10) | (lo >> 22);
-    const unsigned k3 = (lo >> 1) & 0x1fffffu;
-    const LnCell cell = ln_u21_cell(k1 + 1u, tab);
-    // the sine and v while the cell read is in flight
-    const float phi = fmaf((float)(k2 & 0xfffffu), 0x1.921fb6p-19f, -0x1.921fb6p+0f);  // pi 2^-20, -pi/2
-    const float sn = halfpi_sinf(phi);
+    const unsigned b = __float_as_uint((float)(2u * (hi >> 11) + 1u));
+    const float4 c = tab[(b >> 20) - 1016u];
+    const float t = (float)(b & 0xfffffu) * 0x1p-20f;
+    const float z = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
     GaussDraw d;
-    d.v = (float)k3 * 0x1p-21f;
-    __builtin_amdgcn_sched_barrier(0);
-    const float z = newton_sqrtf(-2.0f * ln_u21_finish(cell)) * sn;
-    d.z = (k2 >> 20) ? -z : z;
+    d.z = (hi & 0x400u) ? -z : z;
+    d.v = (float)((lo >> 1) & 0x1fffffu) * 0x1p-21f;
     return d;
 }
 

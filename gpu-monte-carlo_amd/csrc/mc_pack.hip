@@ -158,7 +158,7 @@ __host__ __device__ inline PackLds pack_lds(int model, int L, int R, int H, int 
 // (scalar loads), rather than held in registers for the whole launch.
 template <int MODEL, int NT>
 __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork &w, unsigned char *smem,
-                         long long *red_e, double (*red)[NT / 16], const float4 *ln_tab) {
+                         long long *red_e, double (*red)[NT / 16], const float4 *q_tab) {
     const PackGroup &g = *gp;
     const int L = g.L, H = g.H, S = g.S, R = g.R, K = g.K;
     const int HH = H >> 1, half = L >> 1;
@@ -348,7 +348,7 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
                     const int nrows = c < 2 ? HH + 1 : HH;
                     for (int ri = prow; ri < nrows; ri += rpp) {
                         const int counted = !((c < 2) & (ri == HH));
-                        if (site_update<MODEL>(pv, c, ri, tcol, L, pbeta, psig, J, delta, pthr, ln_tab))
+                        if (site_update<MODEL>(pv, c, ri, tcol, L, pbeta, psig, J, delta, pthr, q_tab))
                             n_acc += counted;
                     }
                 }
@@ -555,17 +555,17 @@ __global__ __launch_bounds__(NT) void pack_kernel(PackArgs a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ long long red_e[NT / 16];
     __shared__ double red[4][NT / 16];
-    __shared__ float4 ln_tab[kLnCells];  // Gaussian XY (ordered by pack_run's first barrier)
+    __shared__ float4 q_tab[kNormalCells];  // Gaussian XY (ordered by pack_run's first barrier)
     const PackWork w = a.work[blockIdx.x];
     const PackGroup *g = a.groups + w.group;
     const int model = g->model;
     if (model == MODEL_ISING) {
-        pack_run<MODEL_ISING, NT>(a, g, w, smem, red_e, red, ln_tab);
+        pack_run<MODEL_ISING, NT>(a, g, w, smem, red_e, red, q_tab);
     } else if (model == MODEL_XY_FAST) {
-        pack_run<MODEL_XY_FAST, NT>(a, g, w, smem, red_e, red, ln_tab);
+        pack_run<MODEL_XY_FAST, NT>(a, g, w, smem, red_e, red, q_tab);
     } else {
-        ln_table_to_lds(ln_tab);
-        pack_run<MODEL_XY_GAUSS, NT>(a, g, w, smem, red_e, red, ln_tab);
+        normal_table_to_lds(q_tab);
+        pack_run<MODEL_XY_GAUSS, NT>(a, g, w, smem, red_e, red, q_tab);
     }
 }
 

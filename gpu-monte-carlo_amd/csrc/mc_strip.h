@@ -157,16 +157,16 @@ __device__ __forceinline__ StripView strip_view(unsigned char *base, int L, int 
 // A site's draws for one sweep, taken from its stream before the sweep's
 // colour passes (they do not depend on the lattice): Ising a = u; XY a = u1
 // (proposal), b = u2 (acceptance); Gaussian XY a = z, b = v (gauss_draw with
-// the log cell table ln_tab in LDS; unused by the other models).
+// the normal quantile table q_tab in LDS; unused by the other models).
 struct SiteDraw {
     float a, b;
 };
 
 template <int MODEL>
-__device__ __forceinline__ SiteDraw site_draw(Rng &g, const float4 *ln_tab) {
+__device__ __forceinline__ SiteDraw site_draw(Rng &g, const float4 *q_tab) {
     SiteDraw d;
     if (MODEL == MODEL_XY_GAUSS) {
-        const GaussDraw gd = gauss_draw(g.next(), ln_tab);
+        const GaussDraw gd = gauss_draw(g.next(), q_tab);
         d.a = gd.z;
         d.b = gd.v;
     } else if (MODEL == MODEL_XY_FAST) {
@@ -231,11 +231,11 @@ __device__ __forceinline__ bool site_apply(const StripView &v, const int self, c
 //    21-54) with dE from the cached (cos, sin) of the neighbours and the
 //    float32 sincos/exp of mc_fastmath.h.
 //  * MODEL_XY_GAUSS: proposal theta + sigma z; z and the acceptance uniform
-//    from one 64-bit draw (gauss_draw, log cell table ln_tab in LDS).
+//    from one 64-bit draw (gauss_draw, quantile table q_tab in LDS).
 template <int MODEL>
 __device__ __forceinline__ bool site_update(const StripView &v, const int c, const int ri, const int tcol,
                                             const int L, const float beta, const float sigma, const float J,
-                                            const float delta, const float *thr, const float4 *ln_tab) {
+                                            const float delta, const float *thr, const float4 *q_tab) {
     const int half = L >> 1;
     const int p = c & 1;
     const int lr = 1 + 2 * ri + (c >> 1);
@@ -268,7 +268,7 @@ __device__ __forceinline__ bool site_update(const StripView &v, const int c, con
         float pr;
         float w;
         if (MODEL == MODEL_XY_GAUSS) {
-            const GaussDraw d = gauss_draw(g.next(), ln_tab);
+            const GaussDraw d = gauss_draw(g.next(), q_tab);
             pr = wrap_turns(th + sigma * d.z);
             w = d.v;
         } else {

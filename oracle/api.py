@@ -25,7 +25,7 @@ __all__ = [
     "kawamura_energy",
     "he3_potential", "he3_gradient_difference", "he3_local_energy", "he3_color",
     "he3_sweeps", "he3_gaussian_color", "he3_measure", "he3_energy",
-    "pt_exchange", "fast_sincosf", "fast_expf", "fast_logf", "ln_u21", "ln_u21_table",
+    "pt_exchange", "fast_sincosf", "fast_expf", "fast_logf", "normal_q",
     "accept_weight", "xyfast_cache", "xyfast_color", "xyfast_gauss_color", "fast_gauss_draw",
     "xyfast_sweeps", "xyfast_observables",
 ]
@@ -115,8 +115,7 @@ def _declare(lib):
         "orc_xyfast_sweeps": (None, [P, P, P, P, P, I32, I32, F32, F32, I64]),
         "orc_xyfast_gauss_color": (None, [P, P, P, P, P, P, I32, I32, I32, F32, P]),
         "orc_fast_gauss_draw": (None, [ctypes.c_uint64, P, P]),
-        "orc_ln_u21_n": (None, [P, I64, P]),
-        "orc_ln_u21_table": (None, [P]),
+        "orc_normal_q_n": (None, [P, I64, P]),
         "orc_xyfast_observables": (None, [P, P, I32, I32, F32, P]),
         "orc_pt_exchange_f64": (None, [P, P, P, P, P, P, P, I32, I32, I64]),
     }
@@ -437,19 +436,12 @@ def fast_logf(x):
     return y
 
 
-def ln_u21(n):
-    """ln(n 2^-21) of the Gaussian step for integers n in [1, 2^21]."""
-    n = np.ascontiguousarray(n, dtype=np.uint32)
-    y = np.empty(n.shape, np.float32)
-    load().orc_ln_u21_n(n.ctypes.data, n.size, y.ctypes.data)
+def normal_q(k):
+    """Q((2k + 1) 2^-23) = -Phi^-1 of the Gaussian step for 21-bit integers k."""
+    k = np.ascontiguousarray(k, dtype=np.uint32)
+    y = np.empty(k.shape, np.float32)
+    load().orc_normal_q_n(k.ctypes.data, k.size, y.ctypes.data)
     return y
-
-
-def ln_u21_table():
-    """The 65 cells (rc, hi, lo) of ln_u21."""
-    t = np.empty((65, 3), np.float32)
-    load().orc_ln_u21_table(t.ctypes.data)
-    return t
 
 
 def xyfast_cache(theta):

@@ -142,8 +142,7 @@ float orc_fast_logf(float x);
 float orc_accept_weight(float x);
 void orc_fast_logf_n(const float *x, int64_t n, float *y);
 void orc_fast_gauss_draw(uint64_t x, float *z, float *v);
-void orc_ln_u21_n(const uint32_t *n, int64_t cnt, float *y);
-void orc_ln_u21_table(float *out);
+void orc_normal_q_n(const uint32_t *k, int64_t cnt, float *y);
 void orc_xyfast_gauss_color(float *th, float *c, float *s, uint64_t *rng, const float *betas, const float *sig,
                             int R, int L, int color, float J, int64_t *acc);
 void orc_xyfast_init_cache(const float *th, int64_t n, float *c, float *s);

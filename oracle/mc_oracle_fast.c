@@ -33,15 +33,9 @@ void orc_fast_logf_n(const float *x, int64_t n, float *y) {
     for (int64_t k = 0; k < n; k++) y[k] = orc_fast_logf(x[k]);
 }
 
-/* ln(n 2^-21) of the Gaussian step for n[k] in [1, 2^21], and its cell table
- * (65 rows of rc, hi, lo) */
-void orc_ln_u21_n(const uint32_t *n, int64_t cnt, float *y) {
-    for (int64_t k = 0; k < cnt; k++) y[k] = ofm_ln_u21(n[k]);
-}
-
-void orc_ln_u21_table(float *out) {
-    for (int i = 0; i < 65; i++)
-        for (int c = 0; c < 3; c++) out[3 * i + c] = OFM_LN_U21[i][c];
+/* the normal quantile of the Gaussian step for 21-bit k[i] */
+void orc_normal_q_n(const uint32_t *k, int64_t cnt, float *y) {
+    for (int64_t i = 0; i < cnt; i++) y[i] = ofm_normal_q(k[i]);
 }
 
 void orc_fast_sincosf_n(const float *x, int64_t n, float *sn, float *cs) {
@@ -115,49 +109,18 @@ void orc_xyfast_color(float *th, float *c, float *s, uint64_t *rng, const float 
     }
 }
 
-/* sqrt(y) = y rsqrt(y): seed 0x5f375a86 - (bits >> 1) (Lomont), three Newton
- * steps x <- x (1.5 - y/2 x^2) with fused multiply-adds */
-static float newton_sqrtf(float y) {
-    union { float f; uint32_t u; } v;
-    v.f = y;
-    v.u = 0x5f375a86u - (v.u >> 1);
-    float x = v.f;
-    const float h = 0.5f * y;
-    x = x * fmaf(-(h * x), x, 1.5f);
-    x = x * fmaf(-(h * x), x, 1.5f);
-    x = x * fmaf(-(h * x), x, 1.5f);
-    return y * x;
-}
-
-/* sin(x), |x| <= pi/2: odd Taylor series to x^13, Horner form */
-static float halfpi_sinf(float x) {
-    const float x2 = x * x;
-    float p = fmaf(x2, 0x1.612462p-33f, -0x1.ae6456p-26f);
-    p = fmaf(x2, p, 0x1.71de3ap-19f);
-    p = fmaf(x2, p, -0x1.a01a02p-13f);
-    p = fmaf(x2, p, 0x1.111112p-7f);
-    p = fmaf(x2, p, -0x1.555556p-3f);
-    return fmaf(x * x2, p, x);
-}
-
 /* Gaussian step of the fast path from ONE 64-bit draw x (the batched
- * engine's gauss_draw): bits 63..43 -> u1 = (k1 + 1) 2^-21 in (0, 1];
- * bits 42..22 -> sign bit and a 20-bit phi = pi (a 2^-20 - 1/2) in [-pi/2, pi/2);
- * bits 21..1 -> acceptance uniform v = k3 2^-21.  z = +-sqrt(-2 log u1) sin(phi)
- * with log u1 = ofm_ln_u21(k1 + 1) (cell table, no division), the Newton
- * square root and the half-period sine;
- * p = theta + sigma z wrapped by whole turns (multiplying by 1/(2 pi), branch
- * free); then the fast dE and acceptance.  sig holds each walker's sigma; acc
- * counts accepted moves per walker (may be NULL). */
+ * engine's gauss_draw): bits 63..43 -> k, z = +-Q((2k + 1) 2^-23) with Q the
+ * normal quantile of ofm_normal_q (inverse-CDF sampling); bit 42 -> the sign;
+ * bits 21..1 -> acceptance uniform v = k3 2^-21.  orc_xyfast_gauss_color then
+ * proposes p = theta + sigma z wrapped by whole turns (multiplying by
+ * 1/(2 pi), branch free), with the fast dE and acceptance.  sig holds each
+ * walker's sigma; acc counts accepted moves per walker (may be NULL). */
 void orc_fast_gauss_draw(uint64_t x, float *z, float *v) {
     const uint32_t hi = (uint32_t)(x >> 32), lo = (uint32_t)x;
-    const uint32_t k1 = hi >> 11;
-    const uint32_t k2 = ((hi & 0x7ffu) << 10) | (lo >> 22);
-    const uint32_t k3 = (lo >> 1) & 0x1fffffu;
-    const float phi = fmaf((float)(k2 & 0xfffffu), 0x1.921fb6p-19f, -0x1.921fb6p+0f);
-    const float zz = newton_sqrtf(-2.0f * ofm_ln_u21(k1 + 1u)) * halfpi_sinf(phi);
-    *z = (k2 >> 20) ? -zz : zz;
-    *v = (float)k3 * 0x1p-21f;
+    const float q = ofm_normal_q(hi >> 11);
+    *z = (hi & 0x400u) ? -q : q;
+    *v = (float)((lo >> 1) & 0x1fffffu) * 0x1p-21f;
 }
 
 static float fast_wrap(float p) {

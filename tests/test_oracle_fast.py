@@ -181,43 +181,17 @@ def test_fast_logf_error_bound_on_uniform_draws():
                                   (k * np.log(2.0)).astype(np.float32))
 
 
-def test_ln_u21_every_input_and_its_cancellation_point():
-    """ln_u21(n) = ln(n 2^-21), the Gaussian step's log, over all 2^21 inputs:
-    within 1.3 ulp of float64 log, zero at n = 2^21, and accurate in relative
-    terms just below it, where (E - 21) ln 2 and the top cell's ln 2 cancel."""
-    n = np.arange(1, 2**21 + 1, dtype=np.uint32)
-    y = oracle.ln_u21(n)
-    ref = np.log(n.astype(np.float64)) - 21 * np.log(2.0)
-    err = _ulp_error(y, ref)
-    assert err.max() <= 1.3
-    assert y[-1] == 0.0
-    top = slice(2**21 - 2**14, 2**21 - 1)  # the last 16384 nonzero results, |ln| < 2^-7
-    assert np.max(np.abs(y[top] - ref[top]) / np.abs(ref[top])) < 2.0**-22
-    assert np.all(np.diff(y.astype(np.float64)) >= 0)  # monotone
-
-
-def test_ln_u21_table_is_the_rounded_cell_logs():
-    """Cell i (c = 1 + i/64) holds rc = float32(1/c) and -ln(rc) split as
-    hi = float32(-ln rc), lo = float32(-ln rc - hi), checked in 50-digit
-    decimal arithmetic; cell 0 is (1, 0, 0) and cell 64 holds ln 2 as the
-    Cody-Waite pair whose sum is ln 2 to 2^-50."""
-    from decimal import Decimal, getcontext
-    from fractions import Fraction
-    getcontext().prec = 50
-    tab = oracle.ln_u21_table()
-
-    def half_ulp_ok(f32, exact):
-        return abs(Fraction(float(f32)) - exact) <= Fraction(float(np.spacing(np.abs(f32)))) / 2
-
-    for i in range(65):
-        rc, hi, lo = tab[i]
-        assert half_ulp_ok(rc, 1 / Fraction(64 + i, 64)), i
-        if i == 0:
-            assert (rc, hi, lo) == (1.0, 0.0, 0.0)
-            continue
-        L = -Decimal(float(rc)).ln()
-        if i == 64:
-            assert abs(Decimal(float(hi)) + Decimal(float(lo)) - L) < Decimal(2) ** -50
-            continue
-        assert half_ulp_ok(hi, Fraction(L)), i
-        assert half_ulp_ok(lo, Fraction(L - Decimal(float(hi)))), i
+def test_normal_quantile_table_every_input():
+    """normal_q(k) = Q((2k + 1) 2^-23) with Q = -Phi^-1, the Gaussian step's
+    inverse-CDF table, over all 2^21 inputs: within 2.01 ulp of max(Q, 1) of
+    scipy's float64 ndtri, positive, and with mean square 1 (the points are the
+    midpoints of 2^22 equal-probability bins of (0, 1/2), so the mean of Q^2
+    is the variance of the discretised normal)."""
+    k = np.arange(2**21, dtype=np.uint32)
+    y = oracle.normal_q(k)
+    q = -special.ndtri((2.0 * k.astype(np.float64) + 1.0) * 2.0**-23)
+    scale = np.spacing(np.maximum(q, 1.0).astype(np.float32)).astype(np.float64)
+    assert np.max(np.abs(y - q) / scale) <= 2.01
+    assert np.all(y > 0) and 5.16 < y[0] < 5.17  # Q(2^-23)
+    assert abs(np.mean(y.astype(np.float64)**2) - 1.0) < 1e-4
+    assert abs(np.mean(y.astype(np.float64)**4) - 3.0) < 1e-3

@@ -125,7 +125,7 @@ def test_counters_equal_moves_and_huge_sigma_stays_wrapped():
 
 
 def test_fast_gauss_draw_moments_and_exact_symmetry():
-    """One 64-bit draw gives z ~ N(0, 1) (Box-Muller) and v ~ U[0, 1); flipping
+    """One 64-bit draw gives z ~ N(0, 1) (inverse CDF) and v ~ U[0, 1); flipping
     the sign bit (bit 42) negates z exactly and leaves v."""
     n = 100_000
     x = np.random.default_rng(9).integers(0, 2**63, size=n, dtype=np.uint64) * np.uint64(2) + np.uint64(1)
@@ -140,15 +140,15 @@ def test_fast_gauss_draw_moments_and_exact_symmetry():
     assert np.array_equal(z2, -z) and np.array_equal(v2, v)
 
 
-def test_fast_gauss_draw_matches_box_muller_from_its_bit_fields():
+def test_fast_gauss_draw_is_the_normal_quantile_of_its_bit_fields():
     x = np.random.default_rng(21).integers(0, 2**63, size=20_000, dtype=np.uint64) * np.uint64(2)
     z, v = oracle.fast_gauss_draw(x)
     xi = x.astype(object)
-    k1 = np.array([(int(a) >> 43) & 0x1FFFFF for a in xi], float)
-    k2 = np.array([(int(a) >> 22) & 0x1FFFFF for a in xi])
+    k = np.array([(int(a) >> 43) & 0x1FFFFF for a in xi], float)
+    sign = np.array([(int(a) >> 42) & 1 for a in xi])
     k3 = np.array([(int(a) >> 1) & 0x1FFFFF for a in xi], float)
-    u1 = (k1 + 1) * 2.0**-21
-    phi = np.pi * ((k2 & 0xFFFFF) * 2.0**-20 - 0.5)
-    ref = np.sqrt(-2 * np.log(u1)) * np.sin(phi) * np.where(k2 >> 20, -1.0, 1.0)
-    assert np.max(np.abs(z - ref)) < 2e-6 * max(1.0, np.abs(ref).max())
+    q = -special.ndtri((2 * k + 1) * 2.0**-23)
+    ref = np.where(sign, -q, q)
+    scale = np.spacing(np.maximum(q, 1.0).astype(np.float32)).astype(np.float64)
+    assert np.max(np.abs(z - ref) / scale) <= 2.01
     np.testing.assert_array_equal(v, (k3 * 2.0**-21).astype(np.float32))
