@@ -210,13 +210,13 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     // ---- the thread's sites: LDS positions and RNG streams in registers for
     // the whole launch (the ghost row's streams are a copy of its owner's)
     constexpr int NP = 5;
-    // The halo words of sweep n are loaded once, early in the draw phase
-    // (after the draws of pass 0's sites), and consumed after two colour
-    // passes; only a word that had not arrived yet is polled again.
-    // Measured on MI355X (XY L=128 R=32, H=16, 512 threads), the position in
-    // the draw phase: none 108, after pass 0's draws 121, after all draws 119
-    // site-updates/ns.
-    constexpr int kPrefetchPass = 1;
+    // The halo words of sweep n are loaded once, in the draw phase (after the
+    // draws of the first kPrefetchPass passes' sites), and consumed after two
+    // colour passes; only a word that had not arrived yet is polled again.
+    // Measured on MI355X (XY L=128 R=32 Gaussian, H=16, 512 threads), the
+    // position: before any draws 160.8, after 1, 2, 3, 4 passes' draws 167.3,
+    // 166.0, 169.6, 166.4, after all draws 166.0 site-updates/ns.
+    constexpr int kPrefetchPass = 3;
     int slot_self[NP][MAXP], slot_lf[NP][MAXP], slot_rt[NP][MAXP];
     bool slot_on[NP][MAXP], slot_own[NP][MAXP];
     uint64_t slot_s0[NP][MAXP], slot_s1[NP][MAXP];
