@@ -17,7 +17,10 @@ float64 difference against the combined error.
 Magnetisation relaxes diffusively under local updates (tau ~ L^2 sweeps), so
 jackknife blocks are sized from the measured integrated autocorrelation time
 (Sokal's automatic window, c = 6): at least 4 tau_int of the slowest series,
-with at least 10 blocks; tau_int and the block size are reported per chain.
+with at least 10 blocks.  Both chains run the same dynamics (same widths and
+exchange schedule), so both use the block size of the larger of their two
+tau_int estimates, which are noisy at the slowest temperature; tau_int is
+reported per chain.
 
     python -m main.precision_study --out data/precision_xy.json
 """
@@ -123,14 +126,13 @@ def run(L=128, R=32, thermalization=10000, measurement=50000, sample_interval=10
     n_samples = len(run32["samples"]["energy"])
     tau32 = autocorrelation(run32["samples"], L)
     tau64 = autocorrelation(run64["samples"], L)
-    b32 = block_size or block_size_for(tau32, n_samples)
-    b64 = block_size or block_size_for(tau64, n_samples)
+    b32 = b64 = block_size or max(block_size_for(tau32, n_samples), block_size_for(tau64, n_samples))
     e32 = estimates(run32["samples"], temps, L, J, b32)
     e64 = estimates(run64["samples"], temps, L, J, b64)
     report = {"L": L, "replicas": R, "temperatures": temps.tolist(), "sigmas": sigmas.tolist(),
               "thermalization_sweeps": thermalization, "measurement_sweeps": measurement,
               "sample_interval": sample_interval, "n_samples": n_samples,
-              "block_size": {"float32": b32, "float64": b64},
+              "block_size": {"float32": b32, "float64": b64}, "block_size_given": block_size is not None,
               "tau_int_samples": {"float32": tau32, "float64": tau64},
               "float32_seconds": t32, "float64_seconds": t64,
               "move_acceptance_float32": br.acceptance_by_temperature().tolist(), "estimators": {}}
@@ -152,8 +154,10 @@ def markdown(report):
              f"sweeps per chain, samples every {report['sample_interval']} sweeps ({report['n_samples']} samples). "
              "Gaussian widths adapted on the float32 chain, then frozen and shared.  Chains are independent "
              "(different seeds).", "",
-             f"Jackknife blocks: {b['float32']} samples (float32 chain), {b['float64']} (float64 chain), sized "
-             "from the largest integrated autocorrelation time over temperatures (in samples):", "",
+             f"Jackknife blocks: {b['float32']} samples for both chains"
+             + (" (given)" if report.get("block_size_given") else ", sized from the largest integrated "
+                "autocorrelation time over temperatures and the two chains")
+             + ". Integrated autocorrelation times (in samples):", "",
              "| series | tau_int float32 | tau_int float64 |", "|---|---|---|"]
     for k in tau["float32"]:
         lines.append(f"| {k} | {tau['float32'][k]:.1f} | {tau['float64'][k]:.1f} |")
@@ -177,9 +181,12 @@ def main():
     ap.add_argument("--thermalization", type=int, default=10000)
     ap.add_argument("--measurement", type=int, default=50000)
     ap.add_argument("--sample-interval", type=int, default=10)
+    ap.add_argument("--block-size", type=int, default=None,
+                    help="jackknife block in samples (default: from tau_int)")
     ap.add_argument("--out", type=Path, default=Path("data/precision_xy.json"))
     args = ap.parse_args()
-    report = run(args.L, args.R, args.thermalization, args.measurement, args.sample_interval)
+    report = run(args.L, args.R, args.thermalization, args.measurement, args.sample_interval,
+                 block_size=args.block_size)
     args.out.parent.mkdir(parents=True, exist_ok=True)
     args.out.write_text(json.dumps(report))
     args.out.with_suffix(".md").write_text(markdown(report))
