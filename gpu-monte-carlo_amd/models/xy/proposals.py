@@ -2,9 +2,11 @@
 
 The reference draws a uniform window theta + delta (2u - 1)
 (models/xy/update.py:24-31).  ``XYModel(proposal="gaussian")`` instead draws
-theta + sigma_t z with z standard normal (Box-Muller of two uniforms), where
-sigma_t belongs to the ladder temperature t of the walker.  The proposal is
-symmetric, so the reference's Metropolis acceptance is unchanged.
+theta + sigma_t z with z standard normal, where sigma_t belongs to the ladder
+temperature t of the walker: the plain path takes z by Box-Muller from two
+uniforms, the batched engines by inverse-CDF sampling from one 64-bit draw
+(a table of the normal quantile).  The proposal is symmetric, so the
+reference's Metropolis acceptance is unchanged.
 
 Widths are adapted toward a target acceptance (0.5 by default) during
 thermalisation only -- ``adapt`` -- and are then frozen, so the measured

@@ -201,7 +201,8 @@ int mc_kawamura_twist_grad(void *stream, int dtype, const void *state, int R, in
  * one lattice size inside one cooperative launch.  model 0 = Ising (the
  * reference update, bit-identical to mc_ising_update_color), model 1 = XY
  * float32 fast update (cached cos/sin, float32 sincos/exp), model 2 = the
- * same with Gaussian proposals (sigmas: R entries per ladder temperature).
+ * same with Gaussian proposals (sigmas: R entries per ladder temperature; z
+ * by inverse-CDF sampling from one 64-bit draw per site).
  * accepted (R entries per ladder temperature, may be NULL) counts accepted
  * moves.  The state is split into L/H strips of H rows per replica; each
  * workgroup (threads = 256 or 512) keeps its strip on chip.  Workspace sizes
