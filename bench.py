@@ -42,6 +42,10 @@ every 10 sweeps.  One step = 10 sweeps + 1 exchange.
   every 10 sweeps, in float32 and float64 (main/precision_models.py holds
   their ladders and parameters); algorithmic bytes 2 x state + 32 per
   site-update (Kawamura 72 / 112 B, He3 128 / 224 B).
+* he3_reference_protocol: the reference's own published benchmark (He3
+  Cartesian sampler, L = 32, 48, 64, 80, 100 replicas; updates only, with
+  exchanges, and with a measurement every 10 sweeps) on the drop-in path,
+  with the speedup over the reference's published ms per sweep.
 
 Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build measures one GPU.
 """
@@ -230,6 +234,66 @@ def bench_film(kind, precise, sweeps, warmup=20):
             "dtype": ("f64" if precise else "f32") if kind == "kawamura" else ("c128" if precise else "c64"),
             "ms_per_sweep": round(wall * 1e3 / sweeps, 4), "bytes_per_site_update": b,
             "roofline_frac": round(rate * b / HBM_PEAK_GBS, 4)}
+
+
+# The reference's published He3 benchmark (profiling/HE3_BENCHMARK_20261027.md:9-38, RTX 4070 SUPER,
+# 100 replicas, medians of 5 windows x 3,000 sweeps after 1,000 warm-up sweeps), ms per sweep by L
+HE3_REF_MS = {
+    "updates": {32: 0.166939, 48: 0.140702, 64: 0.156559, 80: 0.256459},
+    "pt": {32: 0.166103, 48: 0.160599, 64: 0.160094, 80: 0.257237},
+    "production": {32: 0.199653, 48: 0.183814, 64: 0.240163, 80: 0.317248},
+}
+
+
+def bench_he3_reference(sizes=(32, 48, 64, 80), replicas=100, warmup=1000, sweeps=3000, repeats=5):
+    """The reference's He3 benchmark protocol (profiling/profile_he3.py:49-140) on the drop-in path:
+    He3Model(K=1, delta=0.15, pressure=20, reduced_temperature=1), the ladder
+    make_temperature_ladder(0.15, 0.45, 100, Tc=0.257, tc_window=0.06, tc_fraction=0.6),
+    Runtime seed 42 + L, ParallelTempering every 10 sweeps with seed 43 + L; modes "updates"
+    (Runtime.sweep), "pt" (ParallelTempering.sweep) and "production" (pt plus a measurement copied
+    to the host every 10 sweeps); the median of `repeats` windows of `sweeps` sweeps, against the
+    reference's published ms per sweep."""
+    from core.parallel_tempering import ParallelTempering
+    from core.runtime import Runtime
+    from mcgpu import synchronize
+    from models.he3.model import He3Model
+    from utils.temperature_ladder import make_temperature_ladder
+    temps = make_temperature_ladder(T_min=0.15, T_max=0.45, n_T=replicas, Tc=0.257, tc_window=0.06,
+                                    tc_fraction=0.60)
+    out = {"workload": (f"He3 Cartesian Metropolis (K=1, delta=0.15, P=20 bar, t=1), {replicas} replicas, "
+                        f"exchange every 10 sweeps, median of {repeats} windows of {sweeps} sweeps after "
+                        f"{warmup} warm-up sweeps (the reference's published protocol)"),
+           "reference_hardware": "RTX 4070 SUPER (profiling/HE3_BENCHMARK_20261027.md)", "cases": []}
+    for Ln in sizes:
+        rt = Runtime(He3Model(K=1, delta=0.15, pressure=20, reduced_temperature=1), Ln, temps, seed=42 + Ln)
+        pt = ParallelTempering(rt, exchange_interval=10, seed=43 + Ln)
+        for _ in range(warmup):
+            pt.sweep()
+        pt.measure()
+        synchronize()
+        for mode in ("updates", "pt", "production"):
+            times = []
+            for _ in range(repeats):
+                synchronize()
+                t0 = time.perf_counter()
+                for step in range(1, sweeps + 1):
+                    if mode == "updates":
+                        rt.sweep()
+                    else:
+                        pt.sweep()
+                        if mode == "production" and step % 10 == 0:
+                            for v in pt.measure().values():
+                                np.asarray(v).copy()
+                synchronize()
+                times.append(time.perf_counter() - t0)
+            ms = float(np.median(times)) * 1e3 / sweeps
+            ref = HE3_REF_MS[mode][Ln]
+            out["cases"].append({"L": Ln, "mode": mode, "ms_per_sweep": round(ms, 5),
+                                 "site_updates_per_ns": round(Ln * Ln * replicas / (ms * 1e6), 4),
+                                 "reference_ms_per_sweep": ref, "speedup_vs_reference": round(ref / ms, 2)})
+            log(f"he3 reference protocol L={Ln} {mode}: {ms:.4f} ms/sweep ({ref / ms:.2f}x the reference)")
+        del pt, rt
+    return out
 
 
 def bench_ising_config1(therm=2000, meas=10000, interval=10):
@@ -451,6 +515,7 @@ def main():
     ap.add_argument("--no-films", action="store_true")
     ap.add_argument("--no-ising", action="store_true", help="skip the Ising L=32 R=8 whole-run comparison")
     ap.add_argument("--no-gradients", action="store_true", help="skip the differentiable-energy line")
+    ap.add_argument("--no-he3-ref", action="store_true", help="skip the reference's He3 benchmark protocol")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device once, so a counter run holds a "
                          "plain copy kernel of known size to scale its byte counters by")
@@ -542,6 +607,8 @@ def main():
                 log(f"{kind} {r['dtype']}: {r['value']:.2f} site-updates/ns, {r['ms_per_sweep']} ms per sweep")
                 films[f"{kind}_{r['dtype']}"] = r
         out["films"] = films
+    if not args.no_he3_ref:
+        out["he3_reference_protocol"] = bench_he3_reference()
     if not args.no_ising:
         out["ising_config1"] = bench_ising_config1()
     if not args.no_gradients:
