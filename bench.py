@@ -463,10 +463,20 @@ def measured_traffic(strip_rows, threads, sweeps):
     return None, None
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(target_s, sigmas):
+    """The oracle's Gaussian XY update on the host cores (OpenMP over replicas, at most 16
+    threads), then on one thread for a third of the time."""
     import oracle
-    threads = min(16, os.cpu_count() or 1)
-    oracle.set_threads(threads)
     theta = _model("gaussian").initialize_state(L, R, SEED)
     rng = oracle.rng_init(R * L * L, SEED)
     betas = (1.0 / temperatures()).astype(np.float32)
@@ -476,23 +486,32 @@ def cpu_baseline(target_s, sigmas):
         for color in range(4):
             oracle.xy_gauss_color(theta, rng, betas, color, 1.0, betas, sig)
 
-    sweep()  # warm caches and threads
-    sweeps = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < target_s:
-        for _ in range(EXCHANGE):
-            sweep()
-        oracle.xy_energy(theta, 1.0)
-        sweeps += EXCHANGE
-    wall = time.perf_counter() - t0
+    def measure(threads, seconds):
+        oracle.set_threads(threads)
+        sweep()  # warm caches and threads
+        sweeps = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < seconds:
+            for _ in range(EXCHANGE):
+                sweep()
+            oracle.xy_energy(theta, 1.0)
+            sweeps += EXCHANGE
+        wall = time.perf_counter() - t0
+        return sweeps * L * L * R / (wall * 1e9), sweeps, wall
+
+    threads = min(16, os.cpu_count() or 1)
+    value, sweeps, wall = measure(threads, target_s)
+    one, sweeps1, wall1 = measure(1, target_s / 3)
     return {
-        "value": sweeps * L * L * R / (wall * 1e9),
+        "value": value,
         "unit": "site-updates/ns",
         "cores": threads,
         "kind": "port",
         "sample": (f"{sweeps} sweeps of the Gaussian-proposal XY float32 update (C restatement of the "
                    f"engine's plain path, OpenMP over replicas) on L={L}, R={R} with the adapted widths, "
                    f"plus a float64 energy pass every {EXCHANGE} sweeps; {wall:.1f} s"),
+        "single_thread": {"value": one, "sample": f"{sweeps1} sweeps on one thread; {wall1:.1f} s"},
+        "host": {"cpu_model": cpu_model(), "logical_cpus": os.cpu_count()},
     }
 
 
