@@ -8,10 +8,22 @@ from mcgpu import _lib, as_pointer
 from models._device import STREAM, check_update_args
 
 
+# He3Model keeps its coefficients in a tuple, which cannot change, so the
+# float64 copy handed to the kernels is made once per tuple, not per launch.
+_COEF_CACHE = {}
+
+
 def coefficients(betas):
-    """Host pointer to the five GL coefficients as float64 (kept alive by the pointer)."""
+    """Host pointer to the five GL coefficients as float64 (kept alive by the cache or the pointer)."""
+    if isinstance(betas, tuple):
+        hit = _COEF_CACHE.get(betas)
+        if hit is not None:
+            return hit[1]
     arr = np.ascontiguousarray(np.asarray(betas, dtype=np.float64).reshape(5))
-    return arr.ctypes.data_as(ctypes.c_void_p)
+    ptr = arr.ctypes.data_as(ctypes.c_void_p)
+    if isinstance(betas, tuple) and len(_COEF_CACHE) < 256:
+        _COEF_CACHE[betas] = (arr, ptr)
+    return ptr
 
 
 def check_he3_state(state):
