@@ -63,21 +63,44 @@ struct Tile {
     R x[COMP == 5 ? 4 : 1][kHaloRows][kHaloCols];
 };
 
+// Stages a tile: every thread first issues the loads of all its halo sites
+// (three at most), then takes their sincos and writes LDS, so the loads'
+// latency is paid once per tile instead of once per site.
 template <class R, int COMP>
 __device__ __forceinline__ void stage_tile(Tile<R, COMP> &t, const R *__restrict__ b, int L, int i0, int j0) {
-    for (int q = threadIdx.x; q < kHaloRows * kHaloCols; q += kTileThreads) {
-        const int li = q / kHaloCols, lj = q - (q / kHaloCols) * kHaloCols;
-        int gi = i0 + li - 1, gj = j0 + lj - 1;
-        gi = gi < 0 ? gi + L : gi % L;
-        gj = gj < 0 ? gj + L : gj % L;
-        const R *p = b + ((size_t)gi * L + gj) * COMP;
-        R sv, cv;
-        sincos_r<R>(p[0], sv, cv);
-        t.c[li][lj] = cv;
-        t.s[li][lj] = sv;
-        if (COMP == 5) {
+    constexpr int N = kHaloRows * kHaloCols, PER = (N + kTileThreads - 1) / kTileThreads;
+    R v[PER][COMP];
 #pragma unroll
-            for (int m = 0; m < 4; m++) t.x[m][li][lj] = p[1 + m];
+    for (int k = 0; k < PER; k++) {
+        const int q = threadIdx.x + k * kTileThreads;
+        if (q < N) {
+            const int li = q / kHaloCols, lj = q - (q / kHaloCols) * kHaloCols;
+            int gi = i0 + li - 1, gj = j0 + lj - 1;
+            if (L >= kHaloRows) {  // gi, gj in [-1, 2L): one wrap
+                gi = gi < 0 ? gi + L : (gi >= L ? gi - L : gi);
+                gj = gj < 0 ? gj + L : (gj >= L ? gj - L : gj);
+            } else {
+                gi = (gi % L + L) % L;
+                gj = (gj % L + L) % L;
+            }
+            const R *p = b + ((size_t)gi * L + gj) * COMP;
+#pragma unroll
+            for (int m = 0; m < COMP; m++) v[k][m] = p[m];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int q = threadIdx.x + k * kTileThreads;
+        if (q < N) {
+            const int li = q / kHaloCols, lj = q - (q / kHaloCols) * kHaloCols;
+            R sv, cv;
+            sincos_r<R>(v[k][0], sv, cv);
+            t.c[li][lj] = cv;
+            t.s[li][lj] = sv;
+            if (COMP == 5) {
+#pragma unroll
+                for (int m = 0; m < 4; m++) t.x[m][li][lj] = v[k][1 + m];
+            }
         }
     }
 }
