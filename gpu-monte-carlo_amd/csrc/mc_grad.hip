@@ -63,46 +63,59 @@ struct Tile {
     R x[COMP == 5 ? 4 : 1][kHaloRows][kHaloCols];
 };
 
-// Stages a tile: every thread first issues the loads of all its halo sites
-// (three at most), then takes their sincos and writes LDS, so the loads'
-// latency is paid once per tile instead of once per site.
+// A thread's share of a tile's halo image in registers: the loads of all its
+// sites (three at most) are issued together, and the sincos and LDS writes
+// come afterwards, so the loads' latency is paid once per tile.
+template <class R, int COMP>
+struct TileRegs {
+    static constexpr int N = kHaloRows * kHaloCols, PER = (N + kTileThreads - 1) / kTileThreads;
+    R v[PER][COMP];
+
+    __device__ __forceinline__ void load(const R *__restrict__ b, int L, int i0, int j0) {
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int q = threadIdx.x + k * kTileThreads;
+            if (q < N) {
+                const int li = q / kHaloCols, lj = q - (q / kHaloCols) * kHaloCols;
+                int gi = i0 + li - 1, gj = j0 + lj - 1;
+                if (L >= kHaloRows) {  // gi, gj in [-1, 2L): one wrap
+                    gi = gi < 0 ? gi + L : (gi >= L ? gi - L : gi);
+                    gj = gj < 0 ? gj + L : (gj >= L ? gj - L : gj);
+                } else {
+                    gi = (gi % L + L) % L;
+                    gj = (gj % L + L) % L;
+                }
+                const R *p = b + ((size_t)gi * L + gj) * COMP;
+#pragma unroll
+                for (int m = 0; m < COMP; m++) v[k][m] = p[m];
+            }
+        }
+    }
+
+    __device__ __forceinline__ void store(Tile<R, COMP> &t) const {
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int q = threadIdx.x + k * kTileThreads;
+            if (q < N) {
+                const int li = q / kHaloCols, lj = q - (q / kHaloCols) * kHaloCols;
+                R sv, cv;
+                sincos_r<R>(v[k][0], sv, cv);
+                t.c[li][lj] = cv;
+                t.s[li][lj] = sv;
+                if (COMP == 5) {
+#pragma unroll
+                    for (int m = 0; m < 4; m++) t.x[m][li][lj] = v[k][1 + m];
+                }
+            }
+        }
+    }
+};
+
 template <class R, int COMP>
 __device__ __forceinline__ void stage_tile(Tile<R, COMP> &t, const R *__restrict__ b, int L, int i0, int j0) {
-    constexpr int N = kHaloRows * kHaloCols, PER = (N + kTileThreads - 1) / kTileThreads;
-    R v[PER][COMP];
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-        const int q = threadIdx.x + k * kTileThreads;
-        if (q < N) {
-            const int li = q / kHaloCols, lj = q - (q / kHaloCols) * kHaloCols;
-            int gi = i0 + li - 1, gj = j0 + lj - 1;
-            if (L >= kHaloRows) {  // gi, gj in [-1, 2L): one wrap
-                gi = gi < 0 ? gi + L : (gi >= L ? gi - L : gi);
-                gj = gj < 0 ? gj + L : (gj >= L ? gj - L : gj);
-            } else {
-                gi = (gi % L + L) % L;
-                gj = (gj % L + L) % L;
-            }
-            const R *p = b + ((size_t)gi * L + gj) * COMP;
-#pragma unroll
-            for (int m = 0; m < COMP; m++) v[k][m] = p[m];
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-        const int q = threadIdx.x + k * kTileThreads;
-        if (q < N) {
-            const int li = q / kHaloCols, lj = q - (q / kHaloCols) * kHaloCols;
-            R sv, cv;
-            sincos_r<R>(v[k][0], sv, cv);
-            t.c[li][lj] = cv;
-            t.s[li][lj] = sv;
-            if (COMP == 5) {
-#pragma unroll
-                for (int m = 0; m < 4; m++) t.x[m][li][lj] = v[k][1 + m];
-            }
-        }
-    }
+    TileRegs<R, COMP> g;
+    g.load(b, L, i0, j0);
+    g.store(t);
 }
 
 // Bond from site p to site q (a = theta_p - theta_q), rotated by -dl:
@@ -145,10 +158,16 @@ __global__ __launch_bounds__(kTileThreads) void terms_tile_kernel(const R *__res
     R cd, sd;
     twist_cs<R>(twist, r, cd, sd);
     double v[3] = {0.0, 0.0, 0.0};
-    for (int tile = blockIdx.x * kTermsTiles; tile < ntiles && tile < (blockIdx.x + 1) * kTermsTiles; tile++) {
+    // the next tile's loads are in flight while this one's bonds are summed
+    const int first = blockIdx.x * kTermsTiles, last = min(ntiles, first + kTermsTiles);
+    TileRegs<R, COMP> g;
+    if (first < last) g.load(b, L, (first / tc) * kTileRows, (first - (first / tc) * tc) * kTileCols);
+    for (int tile = first; tile < last; tile++) {
         const int i0 = (tile / tc) * kTileRows, j0 = (tile - (tile / tc) * tc) * kTileCols;
         __syncthreads();  // the previous tile's reads are done
-        stage_tile<R, COMP>(t, b, L, i0, j0);
+        g.store(t);
+        if (tile + 1 < last)
+            g.load(b, L, ((tile + 1) / tc) * kTileRows, ((tile + 1) - ((tile + 1) / tc) * tc) * kTileCols);
         __syncthreads();
         for (int q = tid; q < kTileRows * kTileCols; q += kTileThreads) {
             const int li = 1 + q / kTileCols, lj = 1 + (q - (q / kTileCols) * kTileCols);
