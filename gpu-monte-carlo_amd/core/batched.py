@@ -194,6 +194,10 @@ class Ensemble:
         sweep0 = self.sweeps
         if sample_first is None:
             sample_first = sweep0 + sample_interval
+        elif sample_first <= sweep0:
+            # samples are taken after sweeps of this launch only: the first
+            # sweep of the series sample_first + k * interval past sweep0
+            sample_first += (sweep0 - sample_first) // sample_interval * sample_interval + sample_interval
         last = sweep0 + nsweeps
         return sample_first, (0 if last < sample_first else (last - sample_first) // sample_interval + 1)
 
@@ -321,7 +325,9 @@ class BatchedRun(Ensemble):
         _lib.call("mc_batch_workspace_size", self.code, self.L, self.R, self.H, ctypes.byref(hw), ctypes.byref(ew))
         self._halo = device_array(max(hw.value, 1), np.uint64)
         self._energy_words = device_array(max(ew.value, 1), np.uint64)
-        self._error = device_zeros(1, np.int32)
+        self._error = device_zeros(2, np.int32)  # watchdog code, started workgroups
+        self._no_samples = (device_array((1, self.R, self.strips, CHANNELS), np.float64),
+                            device_array((1, self.R), np.int32))
         limit = max_workgroups(self.code, self.L, self.R, self.H, self.exchange_interval, self.threads)
         if self.R * (self.L // self.H) > limit:
             raise ValueError(f"grid of {self.R * (self.L // self.H)} workgroups exceeds the "
@@ -333,11 +339,8 @@ class BatchedRun(Ensemble):
 
     def __del__(self):
         s = getattr(self, "_stream", None)
-        if s is not None:
-            try:
-                _lib.load().mc_stream_destroy(s)
-            except Exception:
-                pass
+        if s is not None and not _lib.closing():
+            _lib.load().mc_stream_destroy(s)
 
     # ------------------------------------------------------------- running
     def launch(self, nsweeps, sample_interval=0, sample_first=None):
@@ -348,8 +351,11 @@ class BatchedRun(Ensemble):
         sample_first, n_samples = self._sample_plan(nsweeps, sample_interval, sample_first)
         sweep0 = self.sweeps
         S = self.strips
-        partials = device_array((max(n_samples, 1), self.R, S, CHANNELS), np.float64)
-        sw2t = device_array((max(n_samples, 1), self.R), np.int32)
+        if n_samples:
+            partials = device_array((n_samples, self.R, S, CHANNELS), np.float64)
+            sw2t = device_array((n_samples, self.R), np.int32)
+        else:  # no samples: the kernel takes no writes, so one placeholder pair serves every launch
+            partials, sw2t = self._no_samples
         _lib.call(
             "mc_batch_run", self._stream, self.code, self.L, self.R, self.H, self.threads, as_pointer(self.state),
             as_pointer(self.rng_states), as_pointer(self.betas), float(self.model.J),
@@ -368,7 +374,7 @@ class BatchedRun(Ensemble):
             _lib.check(_lib.load().mc_stream_synchronize(self._stream))
         else:
             synchronize()
-        err = self._error.copy_to_host()[0]
+        err = self._error[:1].copy_to_host()[0]
         if err:
             raise RuntimeError(f"batched launch stopped by its watchdog (code {err}): "
                                "a workgroup waited too long for a neighbour or the barrier")
@@ -460,8 +466,26 @@ def run_metadata(ens, sample_sweeps, thermalization_sweeps, measurement_sweeps, 
     }
 
 
+def group_waves(shares):
+    """First-fit grouping of launches into waves whose shares of the device
+    (workgroups / co-resident limit) add up to at most one; a share above one
+    gets a wave of its own (its launch then fails its own co-residency check)."""
+    waves, load = [], []
+    for i, share in enumerate(shares):
+        for w, used in enumerate(load):
+            if used + share <= 1.0:
+                waves[w].append(i)
+                load[w] += share
+                break
+        else:
+            waves.append([i])
+            load.append(share)
+    return waves
+
+
 class BatchedSimulation:
-    """Thermalise and sample several BatchedRuns concurrently (one stream each).
+    """Thermalise and sample several BatchedRuns, concurrently (one stream each)
+    as far as their grids fit on the device together (``waves``).
 
     Mirrors core/simulation.py: ``sample_sweeps = thermalization + step`` for
     every ``step % sample_interval == 0``; returns ``{L: {"samples", "metadata"}}``.
@@ -479,15 +503,34 @@ class BatchedSimulation:
         self.measurement_sweeps = measurement_sweeps
         self.sample_interval = sample_interval
 
+    def waves(self):
+        """Indices of the runs grouped into waves that can run at the same time.
+
+        Each run is a cooperative launch whose workgroups wait for each other,
+        so runs in flight together must fit on the device together: a wave
+        holds runs whose grids, as fractions of their own co-residency limits
+        (mc_batch_max_workgroups), add up to at most one.  Waves run one
+        after another; runs of a wave run concurrently on their own streams.
+        """
+        shares = []
+        for br in self.runs:
+            limit = max_workgroups(br.code, br.L, br.R, br.H, br.exchange_interval, br.threads)
+            shares.append(br.R * br.strips / limit)
+        return group_waves(shares)
+
     def run(self):
         total = self.thermalization_sweeps + self.measurement_sweeps
-        handles = []
-        for br in self.runs:
-            first = br.sweeps + self.thermalization_sweeps + self.sample_interval
-            handles.append(br.launch(total, self.sample_interval, first))
+        results = [None] * len(self.runs)
+        for wave in self.waves():
+            handles = []
+            for i in wave:
+                br = self.runs[i]
+                first = br.sweeps + self.thermalization_sweeps + self.sample_interval
+                handles.append((i, br.launch(total, self.sample_interval, first)))
+            for i, h in handles:
+                results[i] = self.runs[i].collect(h)
         out = {}
-        for br, h in zip(self.runs, handles):
-            samples, sweeps = br.collect(h)
+        for br, (samples, sweeps) in zip(self.runs, results):
             metadata = run_metadata(br, sweeps - (br.sweeps - total), self.thermalization_sweeps,
                                     self.measurement_sweeps, self.sample_interval, "batched")
             if br.exchange_interval:

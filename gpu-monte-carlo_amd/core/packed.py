@@ -172,11 +172,9 @@ class PackedRun:
             e._owner = weakref.ref(self)
 
     def __del__(self):
-        lib = None
-        try:
-            lib = _lib.load()
-        except Exception:
+        if _lib.closing() or _lib._lib is None:
             return
+        lib = _lib.load()
         if getattr(self, "_pack", None) is not None:
             lib.mc_pack_destroy(self._pack)
             self._pack = None
@@ -296,11 +294,8 @@ class PackedGraph:
         self._graph = g
 
     def __del__(self):
-        if getattr(self, "_graph", None) is not None:
-            try:
-                _lib.load().mc_graph_destroy(self._graph)
-            except Exception:
-                pass
+        if getattr(self, "_graph", None) is not None and not _lib.closing():
+            _lib.load().mc_graph_destroy(self._graph)
             self._graph = None
 
     def replay(self):

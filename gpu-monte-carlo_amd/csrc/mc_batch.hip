@@ -41,6 +41,10 @@
 // words yet, because the same two ladder slots pair up again two exchanges
 // later.  Each walker's first strip keeps its round-trip record and counts
 // the pairs it is the left member of; the PT streams go back from workgroup 0.
+// Every workgroup reads the shared PT state (betas, maps, PT streams, parity)
+// when it starts, and the writers of that state at the end of the launch
+// first wait until every workgroup has counted itself in (a start counter),
+// so no workgroup can read state of the launch's end, however late it starts.
 //
 // Every wait is bounded: past a watchdog count a workgroup records an error
 // and leaves, so a broken invariant ends the launch instead of hanging it.
@@ -93,7 +97,9 @@ struct BatchArgs {
                                 //   (sweep << 32 | payload)
     uint64_t *energy;           // [4 (exchange mod 4)][R (ladder temperature)][S][2 (lo, hi)] words
                                 //   (exchange tag << 32 | half of the int64 energy partial)
-    int *error;
+    int *error;                 // [0] error code, [1] workgroups started (zeroed per launch)
+    int expect_started;         // workgroups the end-of-launch writers wait for (R * S)
+    unsigned watchdog;          // polls before a wait gives up
 };
 
 // Item q of pass p (the five-pass schedule below): colour c and colour row
@@ -251,7 +257,9 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             slot_pr[p][m] = slot_pc[p][m] = slot_ps[p][m] = slot_dc[p][m] = slot_ds[p][m] = 0.0f;
         }
     if (MODEL == MODEL_XY_GAUSS) normal_table_to_lds(q_tab);
-    __syncthreads();
+    __syncthreads();  // every thread's reads of the shared PT state have completed
+    if (use_pt && tid == 0)
+        __hip_atomic_fetch_add(a.error + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (MODEL != MODEL_ISING) {
 #pragma unroll
         for (int p = 0; p < NP; p++)
@@ -355,7 +363,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             uint32_t payload;
             if (k == tid && (unsigned)(pre >> 32) == (unsigned)n) {
                 payload = (uint32_t)pre;
-            } else if (!poll_word(src + j, (unsigned)n, payload, a.error)) {
+            } else if (!poll_word(src + j, (unsigned)n, payload, a.error, a.watchdog)) {
                 bad = 1;
                 break;
             }
@@ -556,7 +564,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             for (int k = tid; k < nw; k += NT) {
                 const int t = k < 2 * S ? my_t : partner;
                 uint32_t payload;
-                if (!poll_word(ring + (size_t)t * S * 2 + (k % (2 * S)), tag, payload, a.error)) {
+                if (!poll_word(ring + (size_t)t * S * 2 + (k % (2 * S)), tag, payload, a.error, a.watchdog)) {
                     bad = 1;
                     break;
                 }
@@ -625,6 +633,14 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
 
 done:
     flush_accepted();
+    if (use_pt && (s == 0 || blockIdx.x == 0)) {
+        // the shared PT state is overwritten only after every workgroup has read it
+        int bad = 0;
+        if (tid == 0 && !wait_count(a.error + 1, a.expect_started, a.error, a.watchdog)) bad = 1;
+        if (__syncthreads_or(bad)) {
+            if (tid == 0) atomicOr(a.error, BATCH_ERR_WATCHDOG);
+        }
+    }
     if (use_pt) {  // the PT state after the last exchange
         if (s == 0 && tid == 0) {  // this walker's temperature, beta, energy and record
             a.betas[r] = my_beta;
@@ -667,8 +683,17 @@ __global__ void batch_reset_kernel(uint64_t *halo, size_t nhalo, uint64_t *energ
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < nhalo; k += stride) halo[k] = 0;
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < nenergy; k += stride) energy[k] = 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *error = 0;
+    if (blockIdx.x == 0 && threadIdx.x < 2) error[threadIdx.x] = 0;
 }
+
+// Test hooks (mc_batch_test_watchdog): extra workgroups the end-of-launch
+// writers wait for, and a shorter watchdog; both 0 in normal use.
+static int g_batch_test_missing = 0;
+// Launch mode of mc_batch_run (mc_batch_set_cooperative): a cooperative
+// launch, which the runtime admits only when every workgroup is co-resident,
+// or a plain dispatch after the same occupancy check on the host.
+static bool g_batch_cooperative = true;
+static unsigned g_batch_test_watchdog = 0;
 
 static size_t halo_words(int L, int R, int H) { return (size_t)R * (L / H) * 2 * 2 * L; }
 static size_t energy_words(int L, int R, int H) { return (size_t)4 * R * (L / H) * 2; }
@@ -725,6 +750,13 @@ static int check_shape(int model, int L, int R, int H, int threads) {
 using namespace mc;
 
 extern "C" {
+
+int mc_batch_test_watchdog(int missing_workgroups, unsigned polls) {
+    MC_REQUIRE(missing_workgroups >= 0, "missing_workgroups must be nonnegative");
+    g_batch_test_missing = missing_workgroups;
+    g_batch_test_watchdog = polls;
+    return MC_OK;
+}
 
 int mc_batch_workspace_size(int model, int L, int R, int H, size_t *halo_words_out, size_t *energy_words_out) {
     int rc = check_strips(model, L, R, H);
@@ -801,6 +833,8 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, int threads, void
         a.s_index = (first - sample_first) / sample_interval;
     }
     a.halo = halo; a.energy = energy; a.error = error;
+    a.expect_started = R * S + g_batch_test_missing;
+    a.watchdog = g_batch_test_watchdog ? g_batch_test_watchdog : kWatchdog;
 
     const void *kern = kernel_ptr(model, threads, batch_items(L, H, threads));
     if (lds > 64 * 1024)
@@ -811,8 +845,25 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, int threads, void
                        energy_words(L, R, H), error);
     MC_TRY_HIP(hipGetLastError());
     void *kargs[] = {&a};
-    MC_TRY_HIP(hipLaunchCooperativeKernel(kern, dim3((unsigned)(R * S)), dim3(threads), kargs, lds,
-                                          as_stream(stream)));
+    if (g_batch_cooperative) {
+        MC_TRY_HIP(hipLaunchCooperativeKernel(kern, dim3((unsigned)(R * S)), dim3(threads), kargs, lds,
+                                              as_stream(stream)));
+    } else {
+        // a plain dispatch of a grid the occupancy check admits: every
+        // workgroup is placed at once, as for the packed engine
+        int per_cu = 0, dev = 0, cus = 0;
+        MC_TRY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds));
+        MC_TRY_HIP(hipGetDevice(&dev));
+        MC_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        MC_REQUIRE((long long)R * S <= (long long)per_cu * cus,
+                   "%d workgroups cannot be co-resident (%d per CU x %d CUs)", R * S, per_cu, cus);
+        MC_TRY_HIP(hipLaunchKernel(kern, dim3((unsigned)(R * S)), dim3(threads), kargs, lds, as_stream(stream)));
+    }
+    return MC_OK;
+}
+
+int mc_batch_set_cooperative(int cooperative) {
+    g_batch_cooperative = cooperative != 0;
     return MC_OK;
 }
 

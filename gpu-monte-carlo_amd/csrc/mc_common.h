@@ -42,9 +42,10 @@ int fail_hip(hipError_t err, const char *what);
         }                                                       \
     } while (0)
 
-// Library-owned device scratch for reductions; grows on demand, never shrinks.
-// Returns MC_OK and sets *ptr, or an error code.
-int scratch(size_t bytes, void **ptr);
+// Library-owned device scratch for reductions, one region per (device,
+// stream); grows on demand, never shrinks.  Returns MC_OK and sets *ptr, or
+// an error code.
+int scratch(void *stream, size_t bytes, void **ptr);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 

@@ -93,8 +93,6 @@ struct PackGroup {
     uint64_t *pt_rng;          // (R-1) records
     long long *counts;         // (3, R-1)
     long long *trips;          // (5, R)
-    double *sample_partials;   // [max_samples][R][S][kChannels]
-    int32_t *sample_w2t;       // [max_samples][R]
     // workspace owned by the pack
     uint64_t *halo;            // [R][S][2 (row 2, row H)][2 (slot)][L] words
     uint64_t *ghost_rng;       // [R][S][L] records
@@ -106,6 +104,10 @@ struct PackWork {
     int group, task_first, task_count, pad;
 };
 
+// Sample buffers travel by value in the launch arguments, so a captured
+// graph keeps the buffers it was captured with, whatever later launches use.
+constexpr int kMaxPackGroups = 64;
+
 struct PackArgs {
     const PackGroup *groups;
     const PackWork *work;
@@ -116,6 +118,8 @@ struct PackArgs {
     long long sample_first;
     int max_samples;
     int *error;
+    double *sample_partials[kMaxPackGroups];  // per group [max_samples][R][S][kChannels]
+    int32_t *sample_w2t[kMaxPackGroups];      // per group [max_samples][R]
 };
 
 // Dynamic LDS of a workgroup of a group: K strips, then the group's PT state
@@ -435,7 +439,7 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
                         publish_word(dst + 1, (unsigned)xi + 1u, (uint32_t)((unsigned long long)E >> 32));
                     }
                     if (do_sample && sidx < a.max_samples) {
-                        double *dst = g.sample_partials + (((size_t)sidx * R + r) * S + s) * kChannels;
+                        double *dst = a.sample_partials[w.group] + (((size_t)sidx * R + r) * S + s) * kChannels;
                         dst[0] = energy_from_fixed(MODEL, J, E);
 #pragma unroll
                         for (int c = 0; c < 4; c++) dst[c + 1] = M[c];
@@ -526,7 +530,7 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
             __syncthreads();
         }
         if (do_sample && lw == 0 && sidx < a.max_samples)
-            for (int k = tid; k < R; k += NT) g.sample_w2t[(size_t)sidx * R + k] = pt_w2t[k];
+            for (int k = tid; k < R; k += NT) a.sample_w2t[w.group][(size_t)sidx * R + k] = pt_w2t[k];
     }
 
     flush_moves();
@@ -675,8 +679,13 @@ static int pack_upload(Pack *p) {
 }
 
 static int pack_launch(Pack *p, hipStream_t stream, int nsweeps, int sample_interval, long long sample_first,
-                       int max_samples) {
+                       int max_samples, double *const *sample_partials, int32_t *const *sample_w2t) {
     PackArgs a;
+    for (int i = 0; i < kMaxPackGroups; i++) {
+        const bool have = max_samples > 0 && i < (int)p->groups.size();
+        a.sample_partials[i] = have ? sample_partials[i] : nullptr;
+        a.sample_w2t[i] = have ? sample_w2t[i] : nullptr;
+    }
     a.groups = p->d_groups;
     a.work = p->d_work;
     a.ngroups = (int)p->groups.size();
@@ -714,17 +723,9 @@ static int pack_prepare(Pack *p, int nsweeps, int sample_interval, int max_sampl
     MC_REQUIRE((long long)p->work.size() <= (long long)per_cu * cus,
                "%zu workgroups cannot be co-resident (%d per CU x %d CUs at %zu bytes of LDS)", p->work.size(),
                per_cu, cus, p->lds);
-    for (size_t i = 0; i < p->groups.size(); i++) {
-        PackGroup &g = p->groups[i].g;
-        double *sp = sample_partials ? sample_partials[i] : nullptr;
-        int32_t *sw = sample_w2t ? sample_w2t[i] : nullptr;
-        MC_REQUIRE(max_samples == 0 || (sp && sw), "group %zu needs sample buffers", i);
-        if (g.sample_partials != sp || g.sample_w2t != sw) {
-            g.sample_partials = sp;
-            g.sample_w2t = sw;
-            p->dirty = true;
-        }
-    }
+    for (size_t i = 0; max_samples > 0 && i < p->groups.size(); i++)
+        MC_REQUIRE(sample_partials && sample_w2t && sample_partials[i] && sample_w2t[i],
+                   "group %zu needs sample buffers", i);
     return pack_upload(p);
 }
 
@@ -793,6 +794,7 @@ int mc_pack_add_group(void *pack, int model, int L, int R, int H, int K, int exc
     Pack *p = static_cast<Pack *>(pack);
     MC_REQUIRE(p, "null pack");
     MC_REQUIRE(p->work.empty(), "groups must be added before the work table is set");
+    MC_REQUIRE((int)p->groups.size() < kMaxPackGroups, "a pack holds at most %d groups", kMaxPackGroups);
     int rc = pack_check_group(p->threads, model, L, R, H, K);
     if (rc) return rc;
     MC_REQUIRE(exchange_interval >= 0, "exchange_interval must be nonnegative");
@@ -897,7 +899,8 @@ int mc_pack_run(void *pack, void *stream, int nsweeps, int sample_interval, long
     int rc = pack_prepare(p, nsweeps, sample_interval, max_samples, sample_partials, sample_w2t);
     if (rc) return rc;
     if (nsweeps == 0) return MC_OK;
-    return pack_launch(p, as_stream(stream), nsweeps, sample_interval, sample_first, max_samples);
+    return pack_launch(p, as_stream(stream), nsweeps, sample_interval, sample_first, max_samples, sample_partials,
+                       sample_w2t);
 }
 
 int mc_pack_capture(void *pack, void *stream, int nsweeps, int sample_interval, long long sample_first,
@@ -910,7 +913,7 @@ int mc_pack_capture(void *pack, void *stream, int nsweeps, int sample_interval, 
     if (rc) return rc;
     hipStream_t s = as_stream(stream);
     MC_TRY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    rc = pack_launch(p, s, nsweeps, sample_interval, sample_first, max_samples);
+    rc = pack_launch(p, s, nsweeps, sample_interval, sample_first, max_samples, sample_partials, sample_w2t);
     hipGraph_t g = nullptr;
     const hipError_t end = hipStreamEndCapture(s, &g);
     if (rc) {

@@ -127,7 +127,7 @@ int launch_reduce(void *stream, const Acc &acc, int R, int L, const int32_t *tin
     constexpr int C = ENERGY_ONLY ? 1 : CA;
     const int nblk = reduce_blocks(L);
     void *base = nullptr;
-    int rc = scratch((size_t)R * nblk * C * sizeof(double), &base);
+    int rc = scratch(stream, (size_t)R * nblk * C * sizeof(double), &base);
     if (rc) return rc;
     double *partials = reinterpret_cast<double *>(base);
     hipLaunchKernelGGL((reduce_kernel<CA, ENERGY_ONLY, Acc>), dim3(nblk, R), dim3(kReduceThreads), 0,

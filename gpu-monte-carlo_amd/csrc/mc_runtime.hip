@@ -27,39 +27,50 @@ int fail_hip(hipError_t err, const char *what) {
 }
 
 // ---------------------------------------------------------------- scratch
-// One region per device for the reductions' per-block partials.
+// Reduction partials: one region per (device, stream).  Work queued on one
+// stream runs in order, so its reductions may share a region; reductions on
+// different streams (the drop-in path's null stream, torch's current stream
+// of the autograd kernels, a batched run's own stream) never do.  A region
+// only grows; before the old one is freed, the work queued on its stream is
+// waited for.  The table is guarded by a mutex because ctypes calls drop the
+// GIL.  Regions are never freed at process exit (no HIP call from a static
+// destructor).
 struct Scratch {
-    void *ptr = nullptr;
-    size_t bytes = 0;
+    int dev;
+    void *stream;
+    void *ptr;
+    size_t bytes;
 };
-static Scratch g_scratch[64];
+static std::mutex g_scratch_lock;
+static std::vector<Scratch> *g_scratch = new std::vector<Scratch>();
 
-int scratch(size_t bytes, void **ptr) {
+int scratch(void *stream, size_t bytes, void **ptr) {
     int dev = 0;
     MC_TRY_HIP(hipGetDevice(&dev));
-    if (dev < 0 || dev >= 64) {
-        set_error(MC_ERR_HIP, "device ordinal %d out of range", dev);
-        return MC_ERR_HIP;
+    std::lock_guard<std::mutex> guard(g_scratch_lock);
+    Scratch *s = nullptr;
+    for (auto &e : *g_scratch)
+        if (e.dev == dev && e.stream == stream) s = &e;
+    if (!s) {
+        g_scratch->push_back(Scratch{dev, stream, nullptr, 0});
+        s = &g_scratch->back();
     }
-    Scratch &s = g_scratch[dev];
-    if (s.bytes < bytes) {
+    if (s->bytes < bytes) {
         size_t want = bytes < ((size_t)1 << 20) ? ((size_t)1 << 20) : bytes;
         want = (want + 4095) & ~(size_t)4095;
-        if (s.ptr) {
-            // Kernels still queued may use the old region.
-            MC_TRY_HIP(hipDeviceSynchronize());
-            MC_TRY_HIP(hipFree(s.ptr));
-            s.ptr = nullptr;
-            s.bytes = 0;
+        if (s->ptr) {
+            // reductions still queued on this stream may use the old region
+            MC_TRY_HIP(hipStreamSynchronize(as_stream(stream)));
+            MC_TRY_HIP(hipFree(s->ptr));
+            s->ptr = nullptr;
+            s->bytes = 0;
         }
         void *p = nullptr;
         MC_TRY_HIP(hipMalloc(&p, want));
-        MC_TRY_HIP(hipMemset(p, 0, want));
-        MC_TRY_HIP(hipDeviceSynchronize());
-        s.ptr = p;
-        s.bytes = want;
+        s->ptr = p;
+        s->bytes = want;
     }
-    *ptr = s.ptr;
+    *ptr = s->ptr;
     return MC_OK;
 }
 

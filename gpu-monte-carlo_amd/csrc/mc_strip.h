@@ -67,7 +67,8 @@ __host__ __device__ inline double energy_from_fixed(int model, float J, long lon
 //
 // One thread: poll a word until it carries tag `seq`.  False once the
 // watchdog expires or another workgroup has recorded an error.
-__device__ inline bool poll_word(const uint64_t *p, unsigned seq, uint32_t &payload, const int *error) {
+__device__ inline bool poll_word(const uint64_t *p, unsigned seq, uint32_t &payload, const int *error,
+                                 unsigned limit = kWatchdog) {
     unsigned n = 0;
     for (;;) {
         const uint64_t w = __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -77,8 +78,20 @@ __device__ inline bool poll_word(const uint64_t *p, unsigned seq, uint32_t &payl
         }
         __builtin_amdgcn_s_sleep(1);
         if ((++n & 1023u) == 0 &&
-            (n > kWatchdog ||
-             __hip_atomic_load(const_cast<int *>(error), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+            (n > limit || __hip_atomic_load(const_cast<int *>(error), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+            return false;
+    }
+}
+
+// Bounded wait until the counter *p reaches `target`; false when the
+// watchdog expires or another workgroup has recorded an error.
+__device__ inline bool wait_count(const int *p, int target, const int *error, unsigned limit = kWatchdog) {
+    unsigned n = 0;
+    for (;;) {
+        if (__hip_atomic_load(const_cast<int *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
+        __builtin_amdgcn_s_sleep(1);
+        if ((++n & 1023u) == 0 &&
+            (n > limit || __hip_atomic_load(const_cast<int *>(error), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
             return false;
     }
 }

@@ -3,17 +3,28 @@
 There is no fallback: if the shared library is missing or no HIP device is
 visible, the first GPU call raises.
 
-One HIP runtime per process: torch's ROCm libraries load their bundled
-libamdhip64 by its unversioned name from their own directory, which the
-dynamic linker does not match against an already loaded libamdhip64.so.7 of
-another path.  So when torch is installed, the loader first loads torch's
-runtime (without importing torch); libmcgpu then binds to it by soname, and a
-later `import torch` reuses the same file.  Device pointers and streams can
-then move freely between torch tensors and this library.
+One HIP runtime per process.  libmcgpu binds to the HIP runtime already in
+the process when there is one, else to ROCm's (its RUNPATH):
+* torch imported first: torch has loaded its bundled libamdhip64, whose
+  soname (libamdhip64.so.7) satisfies libmcgpu's dependency, so both share
+  one runtime and device pointers and streams move freely between torch
+  tensors and this library;
+* torch not imported: libmcgpu uses ROCm's runtime, the one rocprofv3
+  instruments.  torch's libraries name their runtime by its unversioned file
+  name, which the dynamic linker does not match against an already loaded
+  libamdhip64.so.7, so importing torch after the first GPU call would load a
+  second runtime; ``runtime_paths()`` lists the runtimes mapped, and the
+  torch-facing modules refuse to run with two.
+
+At interpreter exit an atexit hook waits for queued GPU work and then stops
+every HIP call from finalisers (device arrays, events, streams collected
+during shutdown), so nothing reaches a runtime that is being torn down.
 """
 
+import atexit
 import ctypes
-import importlib.util
+import os
+import sys
 import threading
 from pathlib import Path
 
@@ -74,6 +85,8 @@ SIGNATURES = {
     "mc_kawamura_twist_grad": [P, INT, P, INT, INT, P, P, P, P, P],
     "mc_batch_workspace_size": [INT, INT, INT, INT, ctypes.POINTER(SZ), ctypes.POINTER(SZ)],
     "mc_batch_max_workgroups": [INT, INT, INT, INT, INT, INT, ctypes.POINTER(INT)],
+    "mc_batch_test_watchdog": [INT, ctypes.c_uint],
+    "mc_batch_set_cooperative": [INT],
     "mc_batch_run": [P, INT, INT, INT, INT, INT, P, P, P, F32, F32, P, P, INT, I64, INT, P, P, P, P, P, P, P,
                      INT, I64, INT, P, P, P, P, P],
     "mc_pack_lds_bytes": [INT, INT, INT, INT, INT, ctypes.POINTER(SZ)],
@@ -95,22 +108,25 @@ SIGNATURES = {
 _lock = threading.RLock()
 _lib = None
 _initialised = False
+_closing = False
 
 
 class MCError(RuntimeError):
     """HIP runtime failure inside libmcgpu."""
 
 
-def _torch_hip_runtime():
-    """Path of the HIP runtime bundled with torch, or None (torch is not imported)."""
+def runtime_paths():
+    """Real paths of the HIP runtimes (libamdhip64) mapped into this process."""
+    found = set()
     try:
-        spec = importlib.util.find_spec("torch")
-    except (ImportError, ValueError):
-        return None
-    if spec is None or not spec.submodule_search_locations:
-        return None
-    path = Path(list(spec.submodule_search_locations)[0]) / "lib" / "libamdhip64.so"
-    return path if path.exists() else None
+        with open("/proc/self/maps") as f:
+            for line in f:
+                path = line.rstrip("\n").split(None, 5)[-1]
+                if "/libamdhip64.so" in path:
+                    found.add(os.path.realpath(path))
+    except OSError:
+        pass
+    return sorted(found)
 
 
 def load():
@@ -124,9 +140,6 @@ def load():
                 raise ImportError(
                     f"{LIB_PATH} is missing: build it with `python -m mcgpu.build` "
                     "(or __graft_entry__.build()); there is no CPU fallback")
-            runtime = _torch_hip_runtime()
-            if runtime is not None:
-                ctypes.CDLL(str(runtime), mode=ctypes.RTLD_GLOBAL)
             lib = ctypes.CDLL(str(LIB_PATH))
             for name, args in SIGNATURES.items():
                 fn = getattr(lib, name)
@@ -138,6 +151,22 @@ def load():
             lib.mc_version.restype = ctypes.c_char_p
             _lib = lib
     return _lib
+
+
+def closing():
+    """True once the interpreter has started to shut down (no more HIP calls)."""
+    return _closing or sys.is_finalizing()
+
+
+@atexit.register
+def _shutdown():
+    global _closing
+    if _initialised and not _closing:
+        try:
+            _lib.mc_synchronize()
+        except Exception:
+            pass
+    _closing = True
 
 
 def check(status):

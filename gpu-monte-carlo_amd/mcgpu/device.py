@@ -30,11 +30,8 @@ class _Allocation:
 
     def __del__(self):
         ptr, self.ptr = self.ptr, 0
-        if ptr:
-            try:
-                _lib.load().mc_free(ctypes.c_void_p(ptr))
-            except Exception:  # interpreter shutdown
-                pass
+        if ptr and not _lib.closing():
+            _lib.load().mc_free(ctypes.c_void_p(ptr))
 
 
 class DeviceArray:

@@ -27,11 +27,10 @@ class EventTimer:
         return float(ms.value)
 
     def __del__(self):
-        try:
-            _lib.load().mc_event_destroy(self._start)
-            _lib.load().mc_event_destroy(self._stop)
-        except Exception:
-            pass
+        if _lib.closing():
+            return
+        _lib.load().mc_event_destroy(self._start)
+        _lib.load().mc_event_destroy(self._stop)
 
 
 def device_info():

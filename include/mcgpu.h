@@ -223,6 +223,21 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, int threads, void
                  long long *trips, int sample_interval, long long sample_first, int max_samples,
                  double *sample_partials, int32_t *sample_w2t, uint64_t *halo, uint64_t *energy,
                  int *error);
+/* error points to two ints, zeroed by each launch: [0] the watchdog code
+ * (nonzero when a bounded wait gave up), [1] a count of started workgroups.
+ * The writers of the shared PT state (betas, maps, PT streams, parity) wait
+ * for that count before writing, so every workgroup reads the state the
+ * launch started from. */
+/* Test hook, no counterpart in the reference: later launches make their
+ * end-of-launch writers wait for `missing_workgroups` more workgroups than
+ * the grid has, and every bounded wait give up after `polls` polls (0: the
+ * default, about a second).  (0, 0) restores normal launches.  Lets a test
+ * force the watchdog path with every kernel still exiting normally. */
+int mc_batch_test_watchdog(int missing_workgroups, unsigned polls);
+/* Launch mode of later mc_batch_run calls, no counterpart in the reference:
+ * nonzero (the default) a cooperative launch, 0 a plain dispatch after the
+ * same co-residency check (occupancy x compute units) on the host. */
+int mc_batch_set_cooperative(int cooperative);
 /* Energies of the batched and packed paths are integer sums: Ising
  * sum s (s_down + s_right) exactly, XY the per-site (cos bond x + cos bond y)
  * in float32 rounded to multiples of 2^-40 and added in int64; the energy is
@@ -272,7 +287,10 @@ int mc_pack_run(void *pack, void *stream, int nsweeps, int sample_interval, long
                 int max_samples, double *const *sample_partials, int32_t *const *sample_w2t);
 /* The same launch captured into an executable HIP graph (prologue kernel +
  * packed kernel, no memset nodes); each mc_graph_launch continues the chains
- * by nsweeps.  Samples of all replays go to the given buffers. */
+ * by nsweeps.  Samples of all replays go to the given buffers: their
+ * addresses are launch arguments, fixed in the graph at capture, so later
+ * runs or captures with other buffers do not change where a graph writes.
+ * At most 64 groups per pack. */
 int mc_pack_capture(void *pack, void *stream, int nsweeps, int sample_interval, long long sample_first,
                     int max_samples, double *const *sample_partials, int32_t *const *sample_w2t,
                     void **graph);
