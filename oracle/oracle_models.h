@@ -192,7 +192,8 @@ REAL FN(orc_kawamura_local)(const REAL *st, int L, int r, int i, int j, REAL the
     return energy;
 }
 
-/* models/kawamura/update.py:21-65: theta move, tau flip, sphere move */
+/* models/kawamura/update.py:21-65: theta move, tau flip, sphere move.  Pinned by the exact
+ * single-site conditional (tests/test_oracle_kawamura_conditional.py). */
 void FN(orc_kawamura_color)(REAL *st, uint64_t *rng, const REAL *betas, int R, int L,
                             int color, REAL J, REAL delta) {
     const REAL one = (REAL)1, two = (REAL)2, zero = (REAL)0;
