@@ -545,6 +545,10 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if not args.no_gradients:
+        # torch's HIP runtime must be the process's only one: load it before
+        # libmcgpu binds to a runtime (the gradients line runs through torch)
+        import torch  # noqa: F401
     from mcgpu.timing import device_info
     name, cus = device_info()
     log(f"device {name}, {cus} CUs")

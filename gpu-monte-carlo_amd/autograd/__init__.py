@@ -26,6 +26,11 @@ import torch
 
 from mcgpu import _lib
 
+if len(_lib.runtime_paths()) > 1:
+    raise ImportError(
+        "two HIP runtimes are loaded (" + ", ".join(_lib.runtime_paths()) + "): torch was imported after "
+        "libmcgpu bound to ROCm's runtime; import torch (or autograd) before the first GPU call")
+
 __all__ = ["lattice_energy", "twist_derivatives", "helicity_modulus"]
 
 _KERNELS = {

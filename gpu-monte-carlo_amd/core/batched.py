@@ -1,4 +1,4 @@
-"""Batched persistent engine: whole runs in one cooperative launch per lattice size.
+"""Batched persistent engine: whole runs in one persistent launch per lattice size.
 
 The drop-in path (Runtime + ParallelTempering + Simulation) launches four
 colour kernels per sweep from Python, like the reference.  ``BatchedRun``
@@ -506,7 +506,7 @@ class BatchedSimulation:
     def waves(self):
         """Indices of the runs grouped into waves that can run at the same time.
 
-        Each run is a cooperative launch whose workgroups wait for each other,
+        Each run is a persistent launch whose workgroups wait for each other,
         so runs in flight together must fit on the device together: a wave
         holds runs whose grids, as fractions of their own co-residency limits
         (mc_batch_max_workgroups), add up to at most one.  Waves run one

@@ -235,8 +235,8 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, int threads, void
  * force the watchdog path with every kernel still exiting normally. */
 int mc_batch_test_watchdog(int missing_workgroups, unsigned polls);
 /* Launch mode of later mc_batch_run calls, no counterpart in the reference:
- * nonzero (the default) a cooperative launch, 0 a plain dispatch after the
- * same co-residency check (occupancy x compute units) on the host. */
+ * 0 (the default) a plain dispatch after a co-residency check (occupancy x
+ * compute units) on the host, nonzero a cooperative launch. */
 int mc_batch_set_cooperative(int cooperative);
 /* Energies of the batched and packed paths are integer sums: Ising
  * sum s (s_down + s_right) exactly, XY the per-site (cos bond x + cos bond y)
