@@ -21,6 +21,10 @@ every 10 sweeps.  One step = 10 sweeps + 1 exchange.
   launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE measurement
   (per sweep, scaled by a copy kernel of known size; profiles/) times the
   launch's sweeps, when that measurement used the same workload and strip height.
+  The engine keeps lattice and streams on chip, so its real bound is VALU
+  issue: `roofline.valu_issue` prices the timed launch against 256 CUs x 4
+  SIMDs x 2.4 GHz / 4 cycles per wave64 instruction, at the VALU instructions
+  per site-update counted by rocprofv3 for the same launch shape (profiles/).
 * cpu_baseline: the CPU oracle's restatement of the same Gaussian update
   (float32, OpenMP over replicas) on the same lattice, a bounded number of
   sweeps on the host cores.
@@ -70,6 +74,7 @@ ADAPT_SWEEPS, ADAPT_INTERVAL = 2000, 100
 BYTES_PER_SITE_UPDATE = 40          # XY float32: 2 * 4 B angle + 2 * 16 B RNG state
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = REPO / "profiles" / "batch_kernel_xy_gauss_L128_R32.traffic.json"
+VALU_FILE = REPO / "profiles" / "batch_kernel_xy_gauss_L128_R32.valu.json"
 FSS_ISING_L = (16, 24, 32, 48, 64, 96, 128, 256)
 FSS_XY_L = (16, 32, 64, 128, 256)
 FSS_R = 48
@@ -463,6 +468,23 @@ def measured_traffic(strip_rows, threads, sweeps):
     return None, None
 
 
+def valu_roofline(strip_rows, threads, kernel_rate):
+    """The VALU-issue bound of the batched kernel from the committed instruction counters of the
+    same launch shape: wave-instructions per second of the chip x 64 lanes / VALU instructions per
+    site-update, and the fraction of it the timed launch reached (None without matching counters)."""
+    try:
+        v = json.loads(VALU_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    if int(v["strip_rows"]) != strip_rows or int(v["threads"]) != threads:
+        return None
+    peak = float(v["peak_wave_instructions_per_s"]) * 64 / float(v["valu_instructions_per_site_update"]) / 1e9
+    return {"unit": "site-updates/ns", "peak": round(peak, 1), "achieved": round(kernel_rate, 2),
+            "frac": round(kernel_rate / peak, 4),
+            "valu_instructions_per_site_update": v["valu_instructions_per_site_update"],
+            "source": v["summary"]}
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -595,6 +617,7 @@ def main():
             "bytes_per_site_update": BYTES_PER_SITE_UPDATE,
             "kernel_ms": round(b["kernel_ms"], 3),
             "kernel_site_updates_per_ns": round(kernel_rate, 3),
+            "valu_issue": valu_roofline(b["strip_rows"], b["threads"], kernel_rate),
         },
         "move_acceptance": [round(float(x), 4) for x in b["move_acceptance"]],
         "sigmas": [round(float(x), 4) for x in b["sigmas"]],

@@ -78,6 +78,16 @@ SIGNATURES = {
     "mc_he3_energy": [P, INT, P, INT, INT, P, F64, F64, P],
     "mc_pt_exchange": [P, INT, P, P, P, P, P, P, P, INT, INT, I64],
     "mc_pt_record_walkers": [P, P, P, INT, P],
+    "mc_chain_create": [INT, INT, INT, INT, P, P, P, P, INT, P, INT, ctypes.POINTER(P)],
+    "mc_chain_destroy": [P],
+    "mc_chain_channels": [P, ctypes.POINTER(INT)],
+    "mc_chain_attach_exchange": [P, P, P, P, P, P, P],
+    "mc_chain_sweeps": [P, P, ctypes.c_longlong],
+    "mc_chain_energy": [P, P, P],
+    "mc_chain_measure": [P, P, P, P, P],
+    "mc_chain_run": [P, P, ctypes.c_longlong, ctypes.c_longlong, INT, INT, ctypes.c_longlong, INT, P, INT, P,
+                     ctypes.c_longlong, ctypes.c_longlong, ctypes.POINTER(INT), ctypes.POINTER(ctypes.c_longlong),
+                     ctypes.POINTER(ctypes.c_longlong)],
     "mc_xy_twist_terms": [P, INT, P, INT, INT, P, P],
     "mc_xy_twist_grad": [P, INT, P, INT, INT, P, P, P, P, P],
     "mc_xy_twist_hvp": [P, INT, P, INT, INT, P, P, P, P, P, P],

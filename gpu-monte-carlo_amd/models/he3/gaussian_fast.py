@@ -10,9 +10,12 @@ counters are int64 (3, R, L, L).
 
 import numpy as np
 
-from mcgpu import _lib, as_pointer, device_zeros
+from core._native import CHAIN_HE3_GAUSS, ChainSpec, unmodified
+from mcgpu import _lib, as_pointer, device_zeros, dtype_code
 from models._device import STREAM
 
+from .measure import MEASUREMENT_NAMES
+from .model import He3Model
 from .update import check_he3_state, coefficients
 
 
@@ -33,10 +36,27 @@ class FastGaussianUpdater:
         self.counts = device_zeros((3, runtime.R, runtime.L, runtime.L), dtype=np.int64)
         self.sweeps = 0
 
+    def chain_spec(self):
+        """The native chain of this sampler (core._native), or None once the model's
+        measurements are replaced."""
+        model = self.runtime.model
+        if not unmodified(model, He3Model, ("measure", "measure_with_energy", "measure_energy")):
+            return None
+        params = (*map(float, model.betas), float(model.K), float(self.rho), float(self.mass), float(model.g_d))
+        return ChainSpec(CHAIN_HE3_GAUSS, dtype_code(model.dtype), params, (self.counts,), MEASUREMENT_NAMES,
+                         self._count)
+
+    def _count(self, n):
+        self.sweeps += n
+
     def step(self):
         rt = self.runtime
         rt.invalidate_energy_cache()
         check_he3_state(rt.state)
+        chain = rt.native_chain()
+        if chain is not None:
+            chain.sweeps(1)
+            return
         model = rt.model
         code = 0 if rt.state.dtype == np.complex64 else 1
         coef = coefficients(model.betas)

@@ -2,7 +2,9 @@
 
 import numpy as np
 
+from core._native import CHAIN_HE3, ChainSpec, unmodified
 from core.model import Model
+from mcgpu import dtype_code
 from models._device import require_device
 
 from . import measure as _measure
@@ -77,6 +79,13 @@ class He3Model(Model):
     def measure_energy(self, state, output):
         require_device(state, "state")
         _measure.energy(state, self.betas, self.K, self.g_d, output)
+
+    def chain_spec(self, state, betas):
+        """The native chain of this model (core._native), or None once a method is replaced."""
+        if not unmodified(self, He3Model):
+            return None
+        params = (*map(float, self.betas), float(self.K), float(self.delta), float(self.g_d))
+        return ChainSpec(CHAIN_HE3, dtype_code(self.dtype), params, (), MEASUREMENT_NAMES, None)
 
     def parameters(self):
         return {"K": self.K, "betas": self.betas, "delta": self.delta,

@@ -15,6 +15,13 @@ class He3GaussianRuntime(Runtime):
         super().__init__(model, L, temperatures, threads=threads, seed=seed)
         self.gaussian = FastGaussianUpdater(self, rho=rho, mass=proposal_mass)
 
+    def _chain_spec(self):
+        gaussian = getattr(self, "gaussian", None)
+        return None if gaussian is None else gaussian.chain_spec()
+
+    def _sweeps_native(self):
+        return "sweep" not in vars(self) and type(self).sweep is He3GaussianRuntime.sweep
+
     def sweep(self):
         self.gaussian.step()
 

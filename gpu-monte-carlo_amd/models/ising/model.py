@@ -2,7 +2,9 @@
 
 import numpy as np
 
+from core._native import CHAIN_ISING, ChainSpec, unmodified
 from core.model import Model
+from mcgpu import MC_F32
 from models._device import require_device
 
 from . import measure as _measure
@@ -46,6 +48,12 @@ class IsingModel(Model):
 
     def parameters(self):
         return {"J": self.J}
+
+    def chain_spec(self, state, betas):
+        """The native chain of this model (core._native), or None once a method is replaced."""
+        if not unmodified(self, IsingModel):
+            return None
+        return ChainSpec(CHAIN_ISING, MC_F32, (float(self.J),), (), MEASUREMENT_NAMES, None)
 
     def measure_energy(self, state, output):
         """Leave total energies on the GPU for replica exchange."""

@@ -6,7 +6,9 @@ State components per site: theta, tau, dx, dy, dz with d a unit vector.
 
 import numpy as np
 
+from core._native import CHAIN_KAWAMURA, ChainSpec, unmodified
 from core.model import Model
+from mcgpu import dtype_code
 from models._device import require_device
 
 from . import measure as _measure
@@ -62,3 +64,10 @@ class KawamuraModel(Model):
 
     def parameters(self):
         return {"J": self.J, "delta": self.delta}
+
+    def chain_spec(self, state, betas):
+        """The native chain of this model (core._native), or None once a method is replaced."""
+        if not unmodified(self, KawamuraModel):
+            return None
+        return ChainSpec(CHAIN_KAWAMURA, dtype_code(self.dtype), (float(self.J), float(self.delta)), (),
+                         MEASUREMENT_NAMES, None)

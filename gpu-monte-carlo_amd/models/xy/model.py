@@ -13,7 +13,9 @@ import math
 
 import numpy as np
 
+from core._native import CHAIN_XY, CHAIN_XY_GAUSS, ChainSpec, unmodified
 from core.model import Model
+from mcgpu import dtype_code
 from models._device import require_device
 
 from . import measure as _measure
@@ -100,3 +102,15 @@ class XYModel(Model):
         """Leave total energies on the GPU for replica exchange."""
         require_device(state, "state")
         _measure.energy(state, self.J, output)
+
+    def chain_spec(self, state, betas):
+        """The native chain of this model (core._native), or None once a method is replaced."""
+        if not unmodified(self, XYModel):
+            return None
+        code = dtype_code(self.dtype)
+        if self.proposal == "gaussian":
+            tables = self._proposals.get(betas)
+            return ChainSpec(CHAIN_XY_GAUSS, code, (float(self.J), float(tables.T)),
+                             (tables.table_betas, tables.sigmas, tables.accepted), MEASUREMENT_NAMES,
+                             tables.count_sweeps)
+        return ChainSpec(CHAIN_XY, code, (float(self.J), float(self.delta)), (), MEASUREMENT_NAMES, None)

@@ -48,6 +48,10 @@ class GaussianTables:
         """One colour pass: L*L/4 attempts at every temperature."""
         self.attempts += self.L * self.L // 4
 
+    def count_sweeps(self, n):
+        """n whole sweeps: L*L attempts per sweep at every temperature."""
+        self.attempts += n * self.L * self.L
+
     def acceptance(self):
         acc = self.accepted.copy_to_host().astype(np.float64)
         return np.divide(acc, self.attempts, out=np.full(self.T, np.nan), where=self.attempts > 0)

@@ -168,6 +168,48 @@ int mc_pt_exchange(void *stream, int beta_dtype, const double *energies, void *b
 int mc_pt_record_walkers(void *stream, const int32_t *walker_to_temperature,
                          const int32_t *walker_ids, int n, int32_t *history_row);
 
+/* ------------------------------------------------------- native chains
+ * The drop-in loop of core/runtime.py:87-99 (Runtime.sweep: colours 0..3 of
+ * model.update_color) and core/parallel_tempering.py:245-269 (sweep, then
+ * measure_energy + _exchange_kernel every exchange_interval sweeps, and
+ * _record_walkers_kernel every history_stride rounds) for the built-in
+ * models, run from C: one call queues any number of sweeps.  A chain binds
+ * device buffers owned by the caller (state, rng, betas; the PT arrays with
+ * mc_chain_attach_exchange) and allocates nothing.
+ * kind / params / buffers:
+ *   0 Ising            J                                   -
+ *   1 XY               J, delta                            -
+ *   2 XY Gaussian      J, T                                table_betas, sigmas, accepted
+ *   3 Kawamura         J, delta                            -
+ *   4 He3 Cartesian    c0..c4, K, delta, g_d               -
+ *   5 He3 Gaussian     c0..c4, K, rho, mass, g_d           counts (int64 (3, R, L, L))
+ * dtype is the state precision (MC_F32 for Ising). */
+int mc_chain_create(int kind, int dtype, int L, int R, void *state, uint64_t *rng, void *betas,
+                    const double *params, int nparams, void *const *buffers, int nbuffers, void **chain);
+int mc_chain_destroy(void *chain);
+/* Measurement channels of the chain's model (Ising 2, XY 5, Kawamura 7, He3 10) */
+int mc_chain_channels(void *chain, int *channels);
+int mc_chain_attach_exchange(void *chain, double *energies, int32_t *walker_to_temperature,
+                             int32_t *temperature_to_walker, uint64_t *pt_rng, int64_t *counts,
+                             int64_t *trips);
+/* nsweeps sweeps: colours 0, 1, 2, 3 each */
+int mc_chain_sweeps(void *chain, void *stream, long long nsweeps);
+/* The model's measure_energy / measure (as the mc_*_energy, mc_*_measure
+ * entry points) into device buffers */
+int mc_chain_energy(void *chain, void *stream, double *out);
+int mc_chain_measure(void *chain, void *stream, const int32_t *temperature_indices,
+                     const double *cached_energies, double *out);
+/* nsweeps sweeps from global sweep sweep0; after every sweep s with
+ * s % exchange_interval == 0: energies, one matching of `parity` (flipped
+ * after each round when R > 2), and, when tracked is not NULL, the tracked
+ * walkers' temperatures into history row history_row0 + k every
+ * history_stride rounds (rows of ntracked int32; at most history_capacity).
+ * Returns the next parity, the round count and the next history row. */
+int mc_chain_run(void *chain, void *stream, long long nsweeps, long long sweep0, int exchange_interval,
+                 int parity, long long rounds0, int history_stride, const int32_t *tracked, int ntracked,
+                 int32_t *history, long long history_capacity, long long history_row0, int *parity_out,
+                 long long *rounds_out, long long *history_rows_out);
+
 /* -------------------------------------------- differentiable energies
  * No reference counterpart: the reference measures the twist sums only as
  * observables (models/xy/measure.py:17-68, models/kawamura/measure.py:14-70,

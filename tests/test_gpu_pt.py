@@ -164,16 +164,16 @@ def test_walker_history_chunks_stride_and_manual_exchanges(gpu):
         return rt, ParallelTempering(rt, exchange_interval=1, seed=163, **kw)
 
     runtime, pt = make(tracked_walkers=range(4), history_stride=1)
-    pt._history_chunk_rows = 4
     expected = [np.arange(4)]
     with patch.object(runtime, "sweep"):
-        for _ in range(10):
+        for _ in range(150):  # the trail's device buffer grows 64 -> 128 -> 256 rows
             pt.sweep()
             expected.append(pt.walker_to_temperature.copy_to_host())
     history = pt.diagnostics()["walker_history"]
-    assert len(pt._history_chunks) == 3
+    assert pt._trail.rows.shape[0] == 256 and pt._trail.used == 151
     np.testing.assert_array_equal(history["temperature_indices"], expected)
-    np.testing.assert_array_equal(history["sweeps"], np.arange(11))
+    np.testing.assert_array_equal(history["sweeps"], np.arange(151))
+    np.testing.assert_array_equal(history["exchange_rounds"], np.arange(151))
 
     runtime, pt = make(tracked_walkers=[3, 0], history_stride=3)
     with patch.object(runtime, "sweep"):
