@@ -8,11 +8,11 @@ from pathlib import Path
 from utils.data_io import save_runs
 
 
-def preflight(paths):
+def preflight(paths, duplicate_message="Duplicate simulation output paths"):
     """Refuse duplicate or existing outputs before any simulation starts."""
     paths = [Path(p) for p in paths]
     if len(paths) != len(set(paths)):
-        raise ValueError("Duplicate simulation output paths")
+        raise ValueError(duplicate_message)
     existing = [p for p in paths if p.exists()]
     if existing:
         raise FileExistsError("Output files already exist:\n" + "\n".join(map(str, existing)))

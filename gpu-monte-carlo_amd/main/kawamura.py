@@ -53,7 +53,7 @@ def main():
         raise ValueError("All lattice sizes must be even and >= 4")
     temperatures = make_temperature_ladder(**temperature_ladder)
     walker_ids = range(len(temperatures)) if tracked_walkers == "all" else tracked_walkers
-    preflight(output_path(L) for L in sizes)
+    preflight((output_path(L) for L in sizes), "Duplicate simulation output paths (check sizes)")
 
     print(f"\nKawamura simulation: {campaign}")
     print(f"J={J:g}, delta={delta:g}, sizes {sizes}, {len(temperatures)} replicas")

@@ -143,3 +143,27 @@ def test_helicity_modulus_equals_reference_estimator(gpu):
     got = helicity_modulus(snaps, temps, J, "xy").cpu().numpy()
     np.testing.assert_allclose(got, ref, rtol=1e-9, atol=1e-12)
     assert got[0] > got[-1]  # stiff at low T, soft above the BKT range
+
+
+def test_autograd_on_a_side_stream_alongside_drop_in_measurements(gpu):
+    """Reductions on torch's side stream and on the drop-in path's stream use
+    separate partial buffers: results equal the sequential ones."""
+    from autograd import lattice_energy
+    from core.runtime import Runtime
+    from models.xy.model import XYModel
+    rt = Runtime(XYModel(start="random"), 64, np.linspace(0.8, 1.0, 8), seed=11)
+    alone = rt.measure()
+    state = _xy(16, 128, torch.float64, 5)
+    J = torch.tensor(1.0, dtype=torch.float64, device=_dev())
+    twist = torch.zeros(16, dtype=torch.float64, device=_dev())
+    expected = lattice_energy(state, J, twist, "xy").cpu()
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        got = [lattice_energy(state, J, twist, "xy") for _ in range(20)]
+    measured = [rt.measure() for _ in range(20)]  # queued while the side stream works
+    side.synchronize()
+    for g in got:
+        torch.testing.assert_close(g.cpu(), expected, rtol=0, atol=0)
+    for m in measured:
+        for k in alone:
+            np.testing.assert_array_equal(m[k], alone[k])

@@ -153,3 +153,47 @@ def test_batched_simulation_runs_several_sizes_concurrently(gpu):
             assert values.shape == (10, 6)
             assert np.all(np.isfinite(values))
         np.testing.assert_array_equal(run["metadata"]["sample_sweeps"], np.arange(60, 151, 10))
+
+
+def test_campaign_shape_serialises_launches_that_do_not_fit_together(gpu):
+    """main/ising.py's batched campaign: three sizes of 100 temperatures, whose
+    grids together exceed the device; the launches run in waves and end
+    without a watchdog error, each identical to its run alone."""
+    from core.batched import BatchedRun, BatchedSimulation
+    from main.ising import ladder, sizes
+    from models.ising.model import IsingModel
+    temps = ladder()
+    make = lambda L: BatchedRun(IsingModel(), L, temps, seed=42 + L, exchange_interval=10, pt_seed=43)
+    runs = [make(L) for L in sizes]
+    sim = BatchedSimulation(runs, thermalization_sweeps=20, measurement_sweeps=40, sample_interval=10)
+    from core.batched import max_workgroups
+    shares = [r.R * r.strips / max_workgroups(r.code, r.L, r.R, r.H, r.exchange_interval, r.threads) for r in runs]
+    waves = sim.waves()
+    assert sorted(i for w in waves for i in w) == [0, 1, 2]
+    assert all(sum(shares[i] for i in w) <= 1.0 for w in waves)
+    assert len(waves) > 1 or sum(shares) <= 1.0
+    out = sim.run()
+    for L, run in zip(sizes, runs):
+        alone = make(L)
+        alone.run(60)
+        np.testing.assert_array_equal(run.snapshot(), alone.snapshot())
+        assert out[L]["samples"]["energy"].shape == (4, 100)
+
+
+def test_watchdog_expiry_raises_and_every_kernel_exits(gpu):
+    """Forced from the host: the end-of-launch writers wait for one workgroup
+    more than the grid has, with a short bounded wait; the launch ends, the
+    error reaches BatchedRun.synchronize, and the next launch runs normally."""
+    from core.batched import BatchedRun
+    from mcgpu import _lib
+    from models.xy.model import XYModel
+    run = BatchedRun(XYModel(start="random"), 32, np.linspace(0.8, 1.0, 4), seed=3, exchange_interval=5,
+                     strip_rows=8)
+    _lib.call("mc_batch_test_watchdog", 1, 1 << 14)
+    try:
+        with pytest.raises(RuntimeError, match="watchdog"):
+            run.run(20)
+    finally:
+        _lib.call("mc_batch_test_watchdog", 0, 0)
+    run.run(20)  # the error word is cleared by the next launch
+    assert np.all(np.isfinite(run.measure()["energy"]))

@@ -104,6 +104,7 @@ def test_graph_replays_continue_the_eager_chain(gpu):
             np.testing.assert_array_equal(e.walker_to_temperature.copy_to_host(),
                                           g.walker_to_temperature.copy_to_host())
         graphed[0].measure()  # eager work on the ensembles between replays
+        pg.run(0)  # an eager launch with its own (placeholder) sample buffers: the graph keeps its own
     with pytest.raises(ValueError):
         graph.replay()
     got = graph.collect()
