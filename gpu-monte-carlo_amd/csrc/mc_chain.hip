@@ -11,9 +11,44 @@
 // A chain binds device buffers owned by its caller; it allocates nothing.
 #include <string.h>
 
+#include <mutex>
+
 #include "mc_common.h"
+#include "mc_reduce.h"
 
 namespace mc {
+
+// Reduction partials of mc_chain_energy_noalloc: a file-scope device array,
+// one copy per GPU, held until the process ends.  Host code never uses the
+// array's name as an address: each device's copy is found once with
+// hipGetSymbolAddress (with that device current) and kept in
+// g_noalloc_partials.  Every call on one GPU shares its copy, so calls that
+// overlap on one GPU (different streams) race; the reduction writes every
+// partial it reads, so nothing needs clearing between calls.
+constexpr size_t kNoallocPartials = 65536;  // 512 KiB of float64: R x (blocks per replica, <= 64)
+static __device__ __align__(256) double scratch[kNoallocPartials];
+static std::mutex g_noalloc_lock;
+static double *g_noalloc_partials[64];
+
+static int noalloc_partials(double **out) {
+    int dev = 0;
+    MC_TRY_HIP(hipGetDevice(&dev));
+    MC_REQUIRE(dev >= 0 && dev < 64, "device ordinal %d out of range", dev);
+    std::lock_guard<std::mutex> guard(g_noalloc_lock);
+    if (!g_noalloc_partials[dev]) {
+        void *p = nullptr;
+        MC_TRY_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_noalloc_partials[dev] = static_cast<double *>(p);
+    }
+    *out = g_noalloc_partials[dev];
+    return MC_OK;
+}
+
+// Routes this thread's reductions to a caller-provided buffer for one scope.
+struct PartialsScope {
+    PartialsScope(double *p, size_t n) { g_partials_override = PartialsOverride{p, n}; }
+    ~PartialsScope() { g_partials_override = PartialsOverride{nullptr, 0}; }
+};
 
 enum ChainKind {
     CHAIN_ISING = 0,      // params J
@@ -212,6 +247,20 @@ int mc_chain_sweeps(void *chain, void *stream, long long nsweeps) {
 int mc_chain_energy(void *chain, void *stream, double *out) {
     Chain *c = static_cast<Chain *>(chain);
     MC_REQUIRE(c && out, "null chain or output");
+    return chain_energy(*c, stream, out);
+}
+
+int mc_chain_energy_noalloc(void *chain, void *stream, double *out) {
+    Chain *c = static_cast<Chain *>(chain);
+    MC_REQUIRE(c && out, "null chain or output");
+    const size_t need = (size_t)c->R * reduce_blocks(c->L);
+    MC_REQUIRE(need <= kNoallocPartials,
+               "R=%d replicas of L=%d need %zu reduction partials; mc_chain_energy_noalloc holds at most %zu "
+               "(use mc_chain_energy)", c->R, c->L, need, kNoallocPartials);
+    double *partials = nullptr;
+    const int rc = noalloc_partials(&partials);
+    if (rc) return rc;
+    PartialsScope scope(partials, kNoallocPartials);
     return chain_energy(*c, stream, out);
 }
 

@@ -45,7 +45,16 @@ int fail_hip(hipError_t err, const char *what);
 // Library-owned device scratch for reductions, one region per (device,
 // stream); grows on demand, never shrinks.  Returns MC_OK and sets *ptr, or
 // an error code.
-int scratch(void *stream, size_t bytes, void **ptr);
+int reduction_scratch(void *stream, size_t bytes, void **ptr);
+
+// Caller-provided partials for the reductions of this thread, in place of
+// reduction_scratch (set by the entry points that allocate nothing, for the
+// duration of one call).  ptr == nullptr: none.
+struct PartialsOverride {
+    double *ptr;
+    size_t capacity;  // doubles
+};
+extern thread_local PartialsOverride g_partials_override;
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 

@@ -343,7 +343,7 @@ static int terms_impl(void *stream, const void *state, int R_, int L, const doub
     MC_REQUIRE(R_ >= 1, "invalid replica count %d", R_);
     const int ntiles = tiles_of(L), nblk = (ntiles + kTermsTiles - 1) / kTermsTiles;
     void *base = nullptr;
-    int rc = scratch(stream, (size_t)R_ * nblk * 3 * sizeof(double), &base);
+    int rc = reduction_scratch(stream, (size_t)R_ * nblk * 3 * sizeof(double), &base);
     if (rc) return rc;
     double *partials = reinterpret_cast<double *>(base);
     hipLaunchKernelGGL((terms_tile_kernel<R, COMP>), dim3((unsigned)nblk, (unsigned)R_), dim3(kTileThreads), 0,

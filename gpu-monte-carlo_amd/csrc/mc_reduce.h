@@ -127,8 +127,16 @@ int launch_reduce(void *stream, const Acc &acc, int R, int L, const int32_t *tin
     constexpr int C = ENERGY_ONLY ? 1 : CA;
     const int nblk = reduce_blocks(L);
     void *base = nullptr;
-    int rc = scratch(stream, (size_t)R * nblk * C * sizeof(double), &base);
-    if (rc) return rc;
+    const size_t need = (size_t)R * nblk * C;
+    if (g_partials_override.ptr) {
+        MC_REQUIRE(need <= g_partials_override.capacity,
+                   "the reduction needs %zu partials; the caller's buffer holds %zu", need,
+                   g_partials_override.capacity);
+        base = g_partials_override.ptr;
+    } else {
+        int rc = reduction_scratch(stream, need * sizeof(double), &base);
+        if (rc) return rc;
+    }
     double *partials = reinterpret_cast<double *>(base);
     hipLaunchKernelGGL((reduce_kernel<CA, ENERGY_ONLY, Acc>), dim3(nblk, R), dim3(kReduceThreads), 0,
                        as_stream(stream), acc, L, nblk, cached != nullptr, partials);

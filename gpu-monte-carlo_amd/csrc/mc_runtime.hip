@@ -44,7 +44,9 @@ struct Scratch {
 static std::mutex g_scratch_lock;
 static std::vector<Scratch> *g_scratch = new std::vector<Scratch>();
 
-int scratch(void *stream, size_t bytes, void **ptr) {
+thread_local PartialsOverride g_partials_override = {nullptr, 0};
+
+int reduction_scratch(void *stream, size_t bytes, void **ptr) {
     int dev = 0;
     MC_TRY_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> guard(g_scratch_lock);

@@ -84,6 +84,7 @@ SIGNATURES = {
     "mc_chain_attach_exchange": [P, P, P, P, P, P, P],
     "mc_chain_sweeps": [P, P, ctypes.c_longlong],
     "mc_chain_energy": [P, P, P],
+    "mc_chain_energy_noalloc": [P, P, P],
     "mc_chain_measure": [P, P, P, P, P],
     "mc_chain_run": [P, P, ctypes.c_longlong, ctypes.c_longlong, INT, INT, ctypes.c_longlong, INT, P, INT, P,
                      ctypes.c_longlong, ctypes.c_longlong, ctypes.POINTER(INT), ctypes.POINTER(ctypes.c_longlong),

@@ -197,6 +197,14 @@ int mc_chain_sweeps(void *chain, void *stream, long long nsweeps);
 /* The model's measure_energy / measure (as the mc_*_energy, mc_*_measure
  * entry points) into device buffers */
 int mc_chain_energy(void *chain, void *stream, double *out);
+/* mc_chain_energy that allocates nothing.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  Its reduction partials live in a static
+ * device array of 65,536 float64 per GPU (512 KiB, held until the process
+ * ends): R x (blocks per replica: ceil(L*L / 2048), at most 64) must not
+ * exceed 65,536, else MC_ERR_VALUE before any launch.  Calls on one GPU share
+ * the array, so calls that overlap on one GPU (on different streams) race. */
+int mc_chain_energy_noalloc(void *chain, void *stream, double *out);
 int mc_chain_measure(void *chain, void *stream, const int32_t *temperature_indices,
                      const double *cached_energies, double *out);
 /* nsweeps sweeps from global sweep sweep0; after every sweep s with
