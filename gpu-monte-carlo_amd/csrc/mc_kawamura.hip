@@ -20,42 +20,29 @@ __device__ __forceinline__ R kaw_bonds(const R (&ntau)[4], const R (&dot)[4], co
 
 // One site visit (models/kawamura/update.py:26-65): theta move, tau flip,
 // director move, each against the energy left by the previous sub-move.  The
-// neighbours are read once; cos(theta - theta_n) and the dot products are
+// neighbours (nb[q] points at neighbour q's five reals: up, down, left,
+// right) are read once; cos(theta - theta_n) and the dot products are
 // computed once per value of theta and of the director and reused by the
 // later sub-moves (the same operations on the same operands as evaluating
-// every bond afresh, so the results are the same).
-template <class R>
-__global__ __launch_bounds__(256) void kawamura_color_kernel(R *__restrict__ st, uint64_t *__restrict__ rng,
-                                                             const R *__restrict__ betas, int L,
-                                                             int color, R J, R delta) {
-    const int h = L >> 1;
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= h * h) return;
-    const int r = blockIdx.y;
-    const int row = t / h, col = t - (t / h) * h;
-    const int i = 2 * row + (color >> 1), j = 2 * col + (color & 1);
-    R *b = st + (size_t)r * L * L * 5;
-    const size_t k = (size_t)r * L * L + (size_t)i * L + j;
-    const R one = R(1), two = R(2), zero = R(0);
+// every bond afresh, so the results are the same).  `s` holds the site's
+// five reals and is updated in place.  Both the per-colour kernel (global
+// memory) and the sweep kernel (LDS rows) call this, so they agree bit for bit.
+template <class R, class P>
+__device__ __forceinline__ void kawamura_visit(R (&s)[5], const P (&nb)[4], Rng &g, R beta, R J, R delta) {
+    const R one = R(1), two = R(2);
     const R pi = RealConst<R>::pi(), two_pi = RealConst<R>::two_pi();
-    const R beta = betas[r];
-    R *site = b + 5 * (i * L + j);
-    R theta = site[0], tau = site[1], dx = site[2], dy = site[3], dz = site[4];
-    const int ni[4] = {wrap_up(i, L), wrap_dn(i, L), i, i};
-    const int nj[4] = {j, j, wrap_up(j, L), wrap_dn(j, L)};
+    R theta = s[0], tau = s[1], dx = s[2], dy = s[3], dz = s[4];
     R nth[4], ntau[4], nx[4], ny[4], nz[4], dot[4], cs[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        const R *n = b + 5 * (ni[q] * L + nj[q]);
-        nth[q] = n[0];
-        ntau[q] = n[1];
-        nx[q] = n[2];
-        ny[q] = n[3];
-        nz[q] = n[4];
+        nth[q] = nb[q][0];
+        ntau[q] = nb[q][1];
+        nx[q] = nb[q][2];
+        ny[q] = nb[q][3];
+        nz[q] = nb[q][4];
         dot[q] = (dx * nx[q] + dy * ny[q]) + dz * nz[q];
         cs[q] = rcos(theta - nth[q]);
     }
-    Rng g = load_rng(rng, k);
     R energy = kaw_bonds(ntau, dot, cs, tau, J);
 
     R pt = theta + delta * (two * g.template uniform<R>() - one);
@@ -100,12 +87,106 @@ __global__ __launch_bounds__(256) void kawamura_color_kernel(R *__restrict__ st,
         dy = y;
         dz = z;
     }
+    s[0] = theta;
+    s[1] = tau;
+    s[2] = dx;
+    s[3] = dy;
+    s[4] = dz;
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void kawamura_color_kernel(R *__restrict__ st, uint64_t *__restrict__ rng,
+                                                             const R *__restrict__ betas, int L,
+                                                             int color, R J, R delta) {
+    const int h = L >> 1;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= h * h) return;
+    const int r = blockIdx.y;
+    const int row = t / h, col = t - (t / h) * h;
+    const int i = 2 * row + (color >> 1), j = 2 * col + (color & 1);
+    R *b = st + (size_t)r * L * L * 5;
+    const size_t k = (size_t)r * L * L + (size_t)i * L + j;
+    R *site = b + 5 * (i * L + j);
+    R s[5] = {site[0], site[1], site[2], site[3], site[4]};
+    const R *nb[4] = {b + 5 * (wrap_up(i, L) * L + j), b + 5 * (wrap_dn(i, L) * L + j),
+                      b + 5 * (i * L + wrap_up(j, L)), b + 5 * (i * L + wrap_dn(j, L))};
+    Rng g = load_rng(rng, k);
+    kawamura_visit(s, nb, g, betas[r], J, delta);
     store_rng(rng, k, g);
-    site[0] = theta;
-    site[1] = tau;
-    site[2] = dx;
-    site[3] = dy;
-    site[4] = dz;
+#pragma unroll
+    for (int c = 0; c < 5; c++) site[c] = s[c];
+}
+
+// Half a sweep with rows staged in LDS: half 0 runs colours 0 then 1 (the
+// sites of even rows), half 1 colours 2 then 3 (odd rows).  Colours 0 and 1
+// read only odd rows besides their own row, and colours 2 and 3 only even
+// rows, so a launch reads rows that no workgroup of the same launch writes:
+// workgroups need no ordering among themselves, and two launches make a
+// sweep.  A workgroup owns the B/2 rows i0 + half + 2k (k < B/2) of one
+// replica; LDS row lr holds lattice row (i0 + half - 1 + lr) mod L,
+// lr = 0..B, so its own rows are the odd lr and the rows they read the even
+// lr.  Rows are loaded whole and coalesced; the own rows go back whole (both
+// colours of a row change in the same launch), and thread (p, q) keeps the
+// streams of its two sites (row p, columns 2q and 2q + 1) in registers.
+template <class R>
+__global__ __launch_bounds__(512) void kawamura_rows_kernel(R *__restrict__ st, uint64_t *__restrict__ rng, const R *__restrict__ betas,
+                                     int L, int B, int half, R J, R delta) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    R *rows = reinterpret_cast<R *>(smem);
+    const int r = blockIdx.y, i0 = blockIdx.x * B, h = L >> 1;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    R *b = st + (size_t)r * L * L * 5;
+    const int row_reals = L * 5;
+    for (int lr = 0; lr <= B; lr++) {
+        const R *src = b + (size_t)((i0 + half - 1 + lr + L) % L) * row_reals;
+        R *dst = rows + lr * row_reals;
+        for (int x = tid; x < row_reals; x += nt) dst[x] = src[x];
+    }
+    const int p = tid / h, q = tid - (tid / h) * h;
+    const int gi = i0 + half + 2 * p, lr = 1 + 2 * p;
+    const size_t k0 = (size_t)r * L * L + (size_t)gi * L + 2 * q;
+    Rng g0 = load_rng(rng, k0), g1 = load_rng(rng, k0 + 1);
+    const R beta = betas[r];
+    __syncthreads();
+#pragma unroll
+    for (int cc = 0; cc < 2; cc++) {
+        const int j = 2 * q + cc;
+        R *site = rows + lr * row_reals + 5 * j;
+        R s[5] = {site[0], site[1], site[2], site[3], site[4]};
+        const R *nb[4] = {rows + (lr - 1) * row_reals + 5 * j, rows + (lr + 1) * row_reals + 5 * j,
+                          rows + lr * row_reals + 5 * wrap_up(j, L), rows + lr * row_reals + 5 * wrap_dn(j, L)};
+        kawamura_visit(s, nb, cc == 0 ? g0 : g1, beta, J, delta);
+#pragma unroll
+        for (int x = 0; x < 5; x++) site[x] = s[x];
+        __syncthreads();
+    }
+    store_rng(rng, k0, g0);
+    store_rng(rng, k0 + 1, g1);
+    for (int kk = 0; kk < B / 2; kk++) {
+        R *dst = b + (size_t)(i0 + half + 2 * kk) * row_reals;
+        const R *src = rows + (1 + 2 * kk) * row_reals;
+        for (int x = tid; x < row_reals; x += nt) dst[x] = src[x];
+    }
+}
+
+// Rows per workgroup of the row kernels: the even divisor B of L whose
+// workgroup of (B/2) * (L/2) threads comes closest to 512 without exceeding
+// it, with B + 1 rows of LDS within 64 KiB; 0 when none fits (the per-colour
+// kernels run instead).
+static int kawamura_rows_per_block(int L, size_t real_bytes) {
+    int best = 0, best_gap = 1 << 30;
+    for (int B = 2; B <= L; B += 2) {
+        if (L % B) continue;
+        const int threads = (B / 2) * (L / 2);
+        const size_t lds = (size_t)(B + 1) * L * 5 * real_bytes;
+        if (threads > 512 || lds > 64 * 1024) continue;
+        const int gap = threads > 512 ? threads - 512 : 512 - threads;
+        if (gap < best_gap) {
+            best = B;
+            best_gap = gap;
+        }
+    }
+    return best;
 }
 
 // Accumulated channels: e, chirality, px, py, xx, yy, zz, xy, xz, yz, cx, qx.
@@ -172,6 +253,27 @@ static int kawamura_update(void *stream, void *state, uint64_t *rng, const void 
     return MC_OK;
 }
 
+template <class R>
+static int kawamura_sweep(void *stream, void *state, uint64_t *rng, const void *betas, int R_, int L, double J,
+                          double delta) {
+    const int B = kawamura_rows_per_block(L, sizeof(R));
+    if (B == 0) {
+        for (int color = 0; color < 4; color++) {
+            const int rc = kawamura_update<R>(stream, state, rng, betas, R_, L, color, J, delta);
+            if (rc) return rc;
+        }
+        return MC_OK;
+    }
+    const int threads = (B / 2) * (L / 2);
+    const size_t lds = (size_t)(B + 1) * L * 5 * sizeof(R);
+    for (int half = 0; half < 2; half++) {
+        hipLaunchKernelGGL(kawamura_rows_kernel<R>, dim3((unsigned)(L / B), (unsigned)R_), dim3(threads), lds,
+                           as_stream(stream), (R *)state, rng, (const R *)betas, L, B, half, (R)J, (R)delta);
+        MC_CHECK_LAUNCH();
+    }
+    return MC_OK;
+}
+
 }  // namespace mc
 
 using namespace mc;
@@ -185,6 +287,16 @@ int mc_kawamura_update_color(void *stream, int dtype, void *state, uint64_t *rng
     MC_REQUIRE(color >= 0 && color < 4, "color must be in 0..3, got %d", color);
     if (dtype == MC_F32) return kawamura_update<float>(stream, state, rng, betas, R, L, color, J, delta);
     if (dtype == MC_F64) return kawamura_update<double>(stream, state, rng, betas, R, L, color, J, delta);
+    set_error(MC_ERR_TYPE, "unsupported Kawamura dtype code %d", dtype);
+    return MC_ERR_TYPE;
+}
+
+int mc_kawamura_sweep(void *stream, int dtype, void *state, uint64_t *rng, const void *betas, int R, int L,
+                      double J, double delta) {
+    MC_REQUIRE(L >= 2 && L % 2 == 0, "L must be even, got %d", L);
+    MC_REQUIRE(R >= 1, "need at least one replica");
+    if (dtype == MC_F32) return kawamura_sweep<float>(stream, state, rng, betas, R, L, J, delta);
+    if (dtype == MC_F64) return kawamura_sweep<double>(stream, state, rng, betas, R, L, J, delta);
     set_error(MC_ERR_TYPE, "unsupported Kawamura dtype code %d", dtype);
     return MC_ERR_TYPE;
 }

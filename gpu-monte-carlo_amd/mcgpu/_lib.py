@@ -70,6 +70,7 @@ SIGNATURES = {
     "mc_xy_measure": [P, INT, P, INT, INT, F64, P, P, P],
     "mc_xy_energy": [P, INT, P, INT, INT, F64, P],
     "mc_kawamura_update_color": [P, INT, P, P, P, INT, INT, INT, F64, F64],
+    "mc_kawamura_sweep": [P, INT, P, P, P, INT, INT, F64, F64],
     "mc_kawamura_measure": [P, INT, P, INT, INT, F64, P, P, P],
     "mc_kawamura_energy": [P, INT, P, INT, INT, F64, P],
     "mc_he3_update_color": [P, INT, P, P, P, INT, INT, INT, P, F64, F64, F64],

@@ -124,6 +124,12 @@ int mc_xy_energy(void *stream, int dtype, const void *theta, int R, int L, doubl
 int mc_kawamura_update_color(void *stream, int dtype, void *state, uint64_t *rng,
                              const void *betas, int R, int L, int color, double J,
                              double delta);
+/* One sweep: colours 0..3 of models/kawamura/update.py:21-65 in order, as two
+ * launches (colours 0+1 of the even rows, then 2+3 of the odd rows) that
+ * stage whole rows in LDS; bit-identical to four mc_kawamura_update_color
+ * calls. */
+int mc_kawamura_sweep(void *stream, int dtype, void *state, uint64_t *rng, const void *betas, int R,
+                      int L, double J, double delta);
 /* models/kawamura/measure.py:14-70 (energy, chirality, px, py, spin2,
  * twist_cos_x, twist_sin_x) */
 int mc_kawamura_measure(void *stream, int dtype, const void *state, int R, int L, double J,

@@ -139,3 +139,21 @@ def test_hamiltonian_change_rebinds_the_chain(gpu):
     for color in range(4):  # the per-colour path with the new coupling
         b.model.update_color(b.state, b.rng_states, b.betas, color, b.blocks_per_grid, b.threads_per_block)
     np.testing.assert_array_equal(a.snapshot(), b.snapshot())
+
+
+@pytest.mark.parametrize("L,R,dtype", [(192, 3, np.float32), (192, 2, np.float64), (12, 3, np.float32),
+                                       (6, 2, np.float64), (4, 2, np.float32)])
+def test_kawamura_row_sweeps_equal_four_colour_launches(gpu, L, R, dtype):
+    """The row-staged sweep kernels (two launches per sweep) leave what four
+    per-colour launches of update_color leave, bit for bit."""
+    from core.runtime import Runtime
+    from models.kawamura.model import KawamuraModel
+    temps = np.linspace(0.65, 0.8, R)
+    a = Runtime(KawamuraModel(start="random", dtype=dtype), L, temps, seed=31)
+    b = Runtime(KawamuraModel(start="random", dtype=dtype), L, temps, seed=31)
+    a.advance(5)
+    for _ in range(5):
+        for color in range(4):
+            b.model.update_color(b.state, b.rng_states, b.betas, color, b.blocks_per_grid, b.threads_per_block)
+    np.testing.assert_array_equal(a.snapshot(), b.snapshot())
+    np.testing.assert_array_equal(device_rng_as_pairs(a.rng_states), device_rng_as_pairs(b.rng_states))
