@@ -29,10 +29,17 @@ namespace mc {
 
 template <class R>
 __device__ __forceinline__ void sincos_r(R x, R &s, R &c);
-// sine and cosine from one shared argument reduction
+// float32: whole turns removed first (2 pi in two parts, so the remainder is
+// in [-pi, pi] up to the rounding of x itself), then the bounded sine and
+// cosine of mc_fastmath.h (Cody-Waite to a quarter turn and polynomials,
+// <= 1.5 ulp on [-2 pi, 2 pi]).  About a third of the instructions of the
+// full-range library sincosf, which these kernels' cost was mostly.
 template <>
 __device__ __forceinline__ void sincos_r<float>(float x, float &s, float &c) {
-    ::sincosf(x, &s, &c);
+    const float n = rintf(x * 0x1.45f306p-3f);      // x / (2 pi)
+    float r = fmaf(-n, 6.28318548202514648f, x);   // float32(2 pi)
+    r = fmaf(-n, -1.74845553e-07f, r);             // 2 pi - float32(2 pi)
+    fast_sincosf(r, s, c);
 }
 template <>
 __device__ __forceinline__ void sincos_r<double>(double x, double &s, double &c) {
@@ -127,6 +134,12 @@ __device__ __forceinline__ void bond_cs(R cp, R sp, R cq, R sq, R cd, R sd, R &c
     s = sa * cd - ca * sd;
 }
 
+// Unrotated bond (no twist): cos a only
+template <class R>
+__device__ __forceinline__ R bond_c(R cp, R sp, R cq, R sq) {
+    return cp * cq + sp * sq;
+}
+
 template <class R>
 __device__ __forceinline__ void twist_cs(const double *twist, int r, R &cd, R &sd) {
     sincos_r<R>((R)twist_of(twist, r), sd, cd);
@@ -157,7 +170,7 @@ __global__ __launch_bounds__(kTileThreads) void terms_tile_kernel(const R *__res
     const int tc = (L + kTileCols - 1) / kTileCols;
     R cd, sd;
     twist_cs<R>(twist, r, cd, sd);
-    double v[3] = {0.0, 0.0, 0.0};
+    double v[3] = {0.0, 0.0, 0.0};  // float64 across tiles; each tile's few sites are summed in R
     // the next tile's loads are in flight while this one's bonds are summed
     const int first = blockIdx.x * kTermsTiles, last = min(ntiles, first + kTermsTiles);
     TileRegs<R, COMP> g;
@@ -169,6 +182,7 @@ __global__ __launch_bounds__(kTileThreads) void terms_tile_kernel(const R *__res
         if (tile + 1 < last)
             g.load(b, L, ((tile + 1) / tc) * kTileRows, ((tile + 1) - ((tile + 1) / tc) * tc) * kTileCols);
         __syncthreads();
+        R tv[3] = {R(0), R(0), R(0)};
         for (int q = tid; q < kTileRows * kTileCols; q += kTileThreads) {
             const int li = 1 + q / kTileCols, lj = 1 + (q - (q / kTileCols) * kTileCols);
             if (i0 + li - 1 >= L || j0 + lj - 1 >= L) continue;
@@ -181,13 +195,15 @@ __global__ __launch_bounds__(kTileThreads) void terms_tile_kernel(const R *__res
                      ((t.x[1][li][lj] * t.x[1][li][lj + 1] + t.x[2][li][lj] * t.x[2][li][lj + 1]) +
                       t.x[3][li][lj] * t.x[3][li][lj + 1]);
             }
-            R cx, sx, cy, sy;
+            R cx, sx;
             bond_cs<R>(t.c[li][lj], t.s[li][lj], t.c[li + 1][lj], t.s[li + 1][lj], cd, sd, cx, sx);
-            bond_cs<R>(t.c[li][lj], t.s[li][lj], t.c[li][lj + 1], t.s[li][lj + 1], (R)1, (R)0, cy, sy);
-            v[0] += (double)(wx * cx);
-            v[1] += (double)(wy * cy);
-            v[2] += (double)(wx * sx);
+            const R cy = bond_c<R>(t.c[li][lj], t.s[li][lj], t.c[li][lj + 1], t.s[li][lj + 1]);
+            tv[0] += wx * cx;
+            tv[1] += wy * cy;
+            tv[2] += wx * sx;
         }
+#pragma unroll
+        for (int c = 0; c < 3; c++) v[c] += (double)tv[c];
     }
     block_partial<3, kTileThreads>(v, partials + ((size_t)r * nblk + blockIdx.x) * 3);
 }

@@ -167,3 +167,20 @@ def test_autograd_on_a_side_stream_alongside_drop_in_measurements(gpu):
     for m in measured:
         for k in alone:
             np.testing.assert_array_equal(m[k], alone[k])
+
+
+def test_float32_terms_accurate_for_angles_of_many_turns(gpu):
+    """The float32 kernels reduce whole turns before their bounded sine and
+    cosine: angles spread over +-1000 rad give the float64 energy and
+    gradient to float32 accuracy (the inputs' own rounding included)."""
+    from autograd import lattice_energy
+    g = np.random.default_rng(17)
+    host = g.uniform(-1000, 1000, (8, 64, 64)).astype(np.float32)
+    s32 = torch.tensor(host, device=_dev()).requires_grad_(True)
+    s64 = torch.tensor(host.astype(np.float64), device=_dev()).requires_grad_(True)
+    H32 = lattice_energy(s32, 1.0, 0.3, "xy")
+    H64 = lattice_energy(s64, 1.0, 0.3, "xy")
+    (g32,) = torch.autograd.grad(H32.sum(), s32)
+    (g64,) = torch.autograd.grad(H64.sum(), s64)
+    assert torch.allclose(H32, H64, rtol=0, atol=2e-2), (H32 - H64).abs().max()
+    assert (g32.double() - g64).abs().max() < 3e-4
