@@ -24,6 +24,7 @@ launches per sweep).  Then:
 
 import argparse
 import json
+import sys
 import time
 from pathlib import Path
 
@@ -52,7 +53,12 @@ def packed_runs(ising_sizes, xy_sizes, R, thermalization, measurement, interval,
     pr = PackedRun(ens, threads=512, task_sites=2048)
     t0 = time.perf_counter()
     runs = PackedSimulation(pr, thermalization, measurement, interval, chunk=chunk).run()
+    _progress(f"packed set: {len(ens)} ensembles in {time.perf_counter() - t0:.1f} s")
     return runs, time.perf_counter() - t0
+
+
+def _progress(msg):
+    print(f"[fss_study] {msg}", file=sys.stderr, flush=True)
 
 
 def plain_runs(ising_sizes, xy_sizes, R, thermalization, measurement, interval, seed, xy_seed_shift):
@@ -70,6 +76,7 @@ def plain_runs(ising_sizes, xy_sizes, R, thermalization, measurement, interval, 
         rt = Runtime(model, L, np.linspace(*T, R), seed=seed + L + shift)
         pt = ParallelTempering(rt, exchange_interval=10, seed=seed + 1 + L + shift)
         run = Simulation(pt, thermalization, measurement, interval).run()
+        _progress(f"plain {type(model).__name__} L={L}: {time.perf_counter() - t0:.1f} s so far")
         key = L
         while key in out:
             key = (key, len(out))
