@@ -47,7 +47,7 @@ class NativeChain:
 
     def __del__(self):
         h = getattr(self, "_handle", None)
-        if h is not None and h.value and not _lib.closing():
+        if h is not None and h.value and _lib is not None and not _lib.closing():
             _lib.load().mc_chain_destroy(h)
 
     @property

@@ -339,7 +339,7 @@ class BatchedRun(Ensemble):
 
     def __del__(self):
         s = getattr(self, "_stream", None)
-        if s is not None and not _lib.closing():
+        if s is not None and _lib is not None and not _lib.closing():
             _lib.load().mc_stream_destroy(s)
 
     # ------------------------------------------------------------- running
@@ -356,18 +356,30 @@ class BatchedRun(Ensemble):
             sw2t = device_array((n_samples, self.R), np.int32)
         else:  # no samples: the kernel takes no writes, so one placeholder pair serves every launch
             partials, sw2t = self._no_samples
-        _lib.call(
-            "mc_batch_run", self._stream, self.code, self.L, self.R, self.H, self.threads, as_pointer(self.state),
-            as_pointer(self.rng_states), as_pointer(self.betas), float(self.model.J),
-            float(getattr(self.model, "delta", 0.0)), as_pointer(self.sigmas), as_pointer(self._accepted),
-            nsweeps, sweep0, self.exchange_interval,
-            as_pointer(self._parity), as_pointer(self.energies), as_pointer(self.walker_to_temperature),
-            as_pointer(self.temperature_to_walker), as_pointer(self._pt_rng), as_pointer(self._swap_counts),
-            as_pointer(self._trips), int(sample_interval or 0), int(sample_first), n_samples,
-            as_pointer(partials), as_pointer(sw2t), as_pointer(self._halo), as_pointer(self._energy_words),
-            as_pointer(self._error))
+        head, pt, tail = self._launch_args()
+        _lib.call("mc_batch_run", *head, nsweeps, sweep0, self.exchange_interval, *pt, int(sample_interval or 0),
+                  int(sample_first), n_samples, as_pointer(partials), as_pointer(sw2t), *tail)
         self._advance(nsweeps)
         return (partials, sw2t, n_samples, sample_first, sample_interval)
+
+    def _launch_args(self):
+        """The arguments of mc_batch_run that do not change between launches, built once
+        (the arrays keep their buffers for the run's life; J and delta are read again)."""
+        arrays = (self.state, self.rng_states, self.betas, self.sigmas, self._accepted, self._parity, self.energies,
+                  self.walker_to_temperature, self.temperature_to_walker, self._pt_rng, self._swap_counts,
+                  self._trips, self._halo, self._energy_words, self._error)
+        key = (float(self.model.J), float(getattr(self.model, "delta", 0.0)), tuple(map(id, arrays)))
+        cached = getattr(self, "_args", None)
+        if cached is None or cached[0] != key:
+            head = (self._stream, self.code, self.L, self.R, self.H, self.threads, as_pointer(self.state),
+                    as_pointer(self.rng_states), as_pointer(self.betas), key[0], key[1], as_pointer(self.sigmas),
+                    as_pointer(self._accepted))
+            pt = (as_pointer(self._parity), as_pointer(self.energies), as_pointer(self.walker_to_temperature),
+                  as_pointer(self.temperature_to_walker), as_pointer(self._pt_rng), as_pointer(self._swap_counts),
+                  as_pointer(self._trips))
+            tail = (as_pointer(self._halo), as_pointer(self._energy_words), as_pointer(self._error))
+            self._args = cached = (key, (head, pt, tail))
+        return cached[1]
 
     def synchronize(self):
         if self._stream is not None:

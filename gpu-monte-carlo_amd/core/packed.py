@@ -172,7 +172,7 @@ class PackedRun:
             e._owner = weakref.ref(self)
 
     def __del__(self):
-        if _lib.closing() or _lib._lib is None:
+        if _lib is None or _lib.closing() or _lib._lib is None:
             return
         lib = _lib.load()
         if getattr(self, "_pack", None) is not None:
@@ -294,7 +294,7 @@ class PackedGraph:
         self._graph = g
 
     def __del__(self):
-        if getattr(self, "_graph", None) is not None and not _lib.closing():
+        if getattr(self, "_graph", None) is not None and _lib is not None and not _lib.closing():
             _lib.load().mc_graph_destroy(self._graph)
             self._graph = None
 

@@ -60,6 +60,8 @@
 //    draw per site (gauss_draw in mc_fastmath.h).
 // Accepted moves are counted per ladder temperature when `accepted` is given.
 #include <algorithm>
+#include <mutex>
+#include <vector>
 
 #include "mc_strip.h"
 
@@ -697,6 +699,17 @@ static int g_batch_test_missing = 0;
 // wait is bounded, so a grid that is not co-resident ends in the watchdog
 // error rather than a hang.
 static bool g_batch_cooperative = false;
+
+// co-resident workgroups per (device, kernel, workgroup size, dynamic LDS)
+struct Capacity {
+    int dev;
+    const void *kern;
+    int threads;
+    size_t lds;
+    long long capacity;
+};
+static std::mutex g_capacity_lock;
+static std::vector<Capacity> g_capacity;
 static unsigned g_batch_test_watchdog = 0;
 
 static size_t halo_words(int L, int R, int H) { return (size_t)R * (L / H) * 2 * 2 * L; }
@@ -854,13 +867,27 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, int threads, void
                                               as_stream(stream)));
     } else {
         // a plain dispatch of a grid the occupancy check admits: every
-        // workgroup is placed at once, as for the packed engine
-        int per_cu = 0, dev = 0, cus = 0;
-        MC_TRY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds));
+        // workgroup is placed at once, as for the packed engine.  The check's
+        // answer is kept per (device, kernel, workgroup size, LDS), so a
+        // launch costs no runtime queries after the first.
+        int dev = 0;
         MC_TRY_HIP(hipGetDevice(&dev));
-        MC_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        MC_REQUIRE((long long)R * S <= (long long)per_cu * cus,
-                   "%d workgroups cannot be co-resident (%d per CU x %d CUs)", R * S, per_cu, cus);
+        long long capacity = -1;
+        {
+            std::lock_guard<std::mutex> guard(g_capacity_lock);
+            for (const auto &e : g_capacity)
+                if (e.dev == dev && e.kern == kern && e.threads == threads && e.lds == lds) capacity = e.capacity;
+        }
+        if (capacity < 0) {
+            int per_cu = 0, cus = 0;
+            MC_TRY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds));
+            MC_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            capacity = (long long)per_cu * cus;
+            std::lock_guard<std::mutex> guard(g_capacity_lock);
+            g_capacity.push_back(Capacity{dev, kern, threads, lds, capacity});
+        }
+        MC_REQUIRE((long long)R * S <= capacity, "%d workgroups cannot be co-resident (%lld on the device at %zu "
+                   "bytes of LDS)", R * S, capacity, lds);
         MC_TRY_HIP(hipLaunchKernel(kern, dim3((unsigned)(R * S)), dim3(threads), kargs, lds, as_stream(stream)));
     }
     return MC_OK;

@@ -30,7 +30,8 @@ class _Allocation:
 
     def __del__(self):
         ptr, self.ptr = self.ptr, 0
-        if ptr and not _lib.closing():
+        # module globals are None once the interpreter tears modules down
+        if ptr and _lib is not None and not _lib.closing():
             _lib.load().mc_free(ctypes.c_void_p(ptr))
 
 

@@ -27,7 +27,7 @@ class EventTimer:
         return float(ms.value)
 
     def __del__(self):
-        if _lib.closing():
+        if _lib is None or _lib.closing():
             return
         _lib.load().mc_event_destroy(self._start)
         _lib.load().mc_event_destroy(self._stop)
