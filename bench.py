@@ -12,7 +12,10 @@ every 10 sweeps.  One step = 10 sweeps + 1 exchange.
   state already resident on the GPU.  A site-update is one visit of one site
   in one colour pass (L*L*R per sweep).
 * plain_fp64: the drop-in path (Runtime + ParallelTempering, float64 XY with
-  the same Gaussian widths, four colour launches per sweep from Python).
+  the same Gaussian widths, four colour launches per sweep, queued by one
+  native call per sweep); plain_fp64_python_launches: the same with the
+  model's update_color called from Python for each colour, the reference's
+  launch pattern.  speedup_vs_plain_fp64[_python_launches] = value / each.
 * uniform_window: the batched engine with the reference's uniform window
   (delta = 1.5) on the same lattice, for comparison.
 * roofline: algorithmic bytes of the batched kernel, 40 B per site-update
