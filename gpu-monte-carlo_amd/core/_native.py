@@ -10,6 +10,7 @@ the reference does (core/runtime.py:87-99).
 """
 
 import ctypes
+import sys
 from collections import namedtuple
 
 from mcgpu import _lib, as_pointer
@@ -45,9 +46,9 @@ class NativeChain:
         self._handle = handle
         self._exchange = None
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         h = getattr(self, "_handle", None)
-        if h is not None and h.value and _lib is not None and not _lib.closing():
+        if h is not None and h.value and not _finalizing() and not _lib.closing():
             _lib.load().mc_chain_destroy(h)
 
     @property

@@ -33,7 +33,7 @@ import time
 
 import numpy as np
 
-from mcgpu import (_lib, as_pointer, create_xoroshiro128p_states, device_array, device_zeros,
+from mcgpu import (PinnedArray, _lib, as_pointer, create_xoroshiro128p_states, device_array, device_zeros,
                    synchronize, to_device)
 
 MODEL_ISING, MODEL_XY_FAST, MODEL_XY_GAUSS = 0, 1, 2
@@ -326,6 +326,10 @@ class BatchedRun(Ensemble):
         self._halo = device_array(max(hw.value, 1), np.uint64)
         self._energy_words = device_array(max(ew.value, 1), np.uint64)
         self._error = device_zeros(2, np.int32)  # watchdog code, started workgroups
+        # tags and workgroup starts issued into the workspace since it was
+        # zeroed (0: zero it at the next launch); the launches keep it
+        self._clock = (ctypes.c_longlong * 2)()
+        self._error_host = PinnedArray(1, np.int32)  # the watchdog code after each launch
         self._no_samples = (device_array((1, self.R, self.strips, CHANNELS), np.float64),
                             device_array((1, self.R), np.int32))
         limit = max_workgroups(self.code, self.L, self.R, self.H, self.exchange_interval, self.threads)
@@ -337,9 +341,9 @@ class BatchedRun(Ensemble):
     def strips(self):
         return self.L // self.H
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         s = getattr(self, "_stream", None)
-        if s is not None and _lib is not None and not _lib.closing():
+        if s is not None and not _finalizing() and not _lib.closing():
             _lib.load().mc_stream_destroy(s)
 
     # ------------------------------------------------------------- running
@@ -377,7 +381,8 @@ class BatchedRun(Ensemble):
             pt = (as_pointer(self._parity), as_pointer(self.energies), as_pointer(self.walker_to_temperature),
                   as_pointer(self.temperature_to_walker), as_pointer(self._pt_rng), as_pointer(self._swap_counts),
                   as_pointer(self._trips))
-            tail = (as_pointer(self._halo), as_pointer(self._energy_words), as_pointer(self._error))
+            tail = (as_pointer(self._halo), as_pointer(self._energy_words), as_pointer(self._error),
+                    ctypes.cast(self._clock, ctypes.c_void_p), self._error_host.pointer)
             self._args = cached = (key, (head, pt, tail))
         return cached[1]
 
@@ -386,8 +391,10 @@ class BatchedRun(Ensemble):
             _lib.check(_lib.load().mc_stream_synchronize(self._stream))
         else:
             synchronize()
-        err = self._error[:1].copy_to_host()[0]
+        err = int(self._error_host.host[0])
         if err:
+            self._error_host.host[0] = 0
+            self._clock[0] = 0  # the next launch zeroes the workspace
             raise RuntimeError(f"batched launch stopped by its watchdog (code {err}): "
                                "a workgroup waited too long for a neighbour or the barrier")
 

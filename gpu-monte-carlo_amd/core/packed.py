@@ -171,8 +171,8 @@ class PackedRun:
         for e in self.ensembles:
             e._owner = weakref.ref(self)
 
-    def __del__(self):
-        if _lib is None or _lib.closing() or _lib._lib is None:
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing() or _lib.closing() or _lib._lib is None:
             return
         lib = _lib.load()
         if getattr(self, "_pack", None) is not None:
@@ -293,8 +293,8 @@ class PackedGraph:
                   self.n_max, pp, pw, ctypes.byref(g))
         self._graph = g
 
-    def __del__(self):
-        if getattr(self, "_graph", None) is not None and _lib is not None and not _lib.closing():
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if getattr(self, "_graph", None) is not None and not _finalizing() and not _lib.closing():
             _lib.load().mc_graph_destroy(self._graph)
             self._graph = None
 

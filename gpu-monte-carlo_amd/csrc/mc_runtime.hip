@@ -1,5 +1,6 @@
 // Runtime part of the C ABI: errors, device memory, streams, reduction
 // scratch and the parallel numba-compatible RNG initialisation.
+#include <cstring>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -223,6 +224,19 @@ int mc_malloc(void **ptr, size_t bytes) {
 
 int mc_free(void *ptr) {
     if (ptr) MC_TRY_HIP(hipFree(ptr));
+    return MC_OK;
+}
+
+int mc_host_alloc(void **ptr, size_t bytes) {
+    *ptr = nullptr;
+    if (bytes == 0) bytes = 16;
+    MC_TRY_HIP(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+    std::memset(*ptr, 0, bytes);
+    return MC_OK;
+}
+
+int mc_host_free(void *ptr) {
+    if (ptr) MC_TRY_HIP(hipHostFree(ptr));
     return MC_OK;
 }
 

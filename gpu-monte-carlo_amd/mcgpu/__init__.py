@@ -6,7 +6,7 @@ drop-in modules ``core/`` and ``models/`` are written on top of it.
 """
 
 from . import _lib
-from .device import (DeviceArray, as_pointer, device_array, device_zeros,
+from .device import (DeviceArray, PinnedArray, as_pointer, device_array, device_zeros,
                      is_available, synchronize, to_device)
 from .rng import (create_xoroshiro128p_states, init_xoroshiro128p_states,
                   xoroshiro128p_dtype)
@@ -15,7 +15,7 @@ MC_F32 = _lib.MC_F32
 MC_F64 = _lib.MC_F64
 
 __all__ = [
-    "DeviceArray", "as_pointer", "device_array", "device_zeros", "is_available",
+    "DeviceArray", "PinnedArray", "as_pointer", "device_array", "device_zeros", "is_available",
     "synchronize", "to_device", "create_xoroshiro128p_states",
     "init_xoroshiro128p_states", "xoroshiro128p_dtype", "MC_F32", "MC_F64",
     "dtype_code",

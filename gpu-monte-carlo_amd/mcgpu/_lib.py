@@ -48,6 +48,8 @@ SIGNATURES = {
     "mc_init": [INT],
     "mc_malloc": [ctypes.POINTER(P), SZ],
     "mc_free": [P],
+    "mc_host_alloc": [ctypes.POINTER(P), SZ],
+    "mc_host_free": [P],
     "mc_memcpy_htod": [P, P, SZ],
     "mc_memcpy_dtoh": [P, P, SZ],
     "mc_memcpy_dtod": [P, P, SZ],
@@ -100,7 +102,7 @@ SIGNATURES = {
     "mc_batch_test_watchdog": [INT, ctypes.c_uint],
     "mc_batch_set_cooperative": [INT],
     "mc_batch_run": [P, INT, INT, INT, INT, INT, P, P, P, F32, F32, P, P, INT, I64, INT, P, P, P, P, P, P, P,
-                     INT, I64, INT, P, P, P, P, P],
+                     INT, I64, INT, P, P, P, P, P, P, P],
     "mc_pack_lds_bytes": [INT, INT, INT, INT, INT, ctypes.POINTER(SZ)],
     "mc_pack_occupancy": [INT, SZ, ctypes.POINTER(INT)],
     "mc_pack_create": [INT, ctypes.POINTER(P)],

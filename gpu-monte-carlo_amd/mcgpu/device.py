@@ -8,6 +8,7 @@ and its callers use (core/runtime.py:109-119, core/parallel_tempering.py:151,
 
 import ctypes
 import operator
+import sys
 
 import numpy as np
 
@@ -28,10 +29,11 @@ class _Allocation:
         self.ptr = int(p.value or 0)
         self.nbytes = int(nbytes)
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         ptr, self.ptr = self.ptr, 0
-        # module globals are None once the interpreter tears modules down
-        if ptr and _lib is not None and not _lib.closing():
+        # at interpreter exit the runtime is already shut down and module
+        # globals may be None: the default argument survives that clearing
+        if ptr and not _finalizing() and not _lib.closing():
             _lib.load().mc_free(ctypes.c_void_p(ptr))
 
 
@@ -175,6 +177,32 @@ def as_pointer(ary):
     if not isinstance(ary, DeviceArray):
         raise TypeError(f"expected a DeviceArray, got {type(ary).__name__}")
     return ctypes.c_void_p(ary.ptr)
+
+
+class PinnedArray:
+    """A small zeroed host array in pinned memory (mc_host_alloc), the target of
+    asynchronous device-to-host copies; ``host`` is a NumPy view of it."""
+
+    def __init__(self, n, dtype):
+        self.ptr = 0
+        dtype = np.dtype(dtype)
+        _lib.ensure_device()
+        p = ctypes.c_void_p()
+        nbytes = max(int(n) * dtype.itemsize, 1)
+        _lib.check(_lib.load().mc_host_alloc(ctypes.byref(p), nbytes))
+        self.ptr = int(p.value or 0)
+        buf = (ctypes.c_char * nbytes).from_address(self.ptr)
+        self.host = np.frombuffer(buf, dtype=dtype, count=int(n))
+
+    @property
+    def pointer(self):
+        return ctypes.c_void_p(self.ptr)
+
+    def __del__(self, _finalizing=sys.is_finalizing):
+        ptr, self.ptr = getattr(self, "ptr", 0), 0
+        self.host = None
+        if ptr and not _finalizing() and not _lib.closing():
+            _lib.load().mc_host_free(ctypes.c_void_p(ptr))
 
 
 def synchronize():

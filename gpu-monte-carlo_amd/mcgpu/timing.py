@@ -1,6 +1,7 @@
 """HIP-event timing of work queued on one stream."""
 
 import ctypes
+import sys
 
 from . import _lib
 
@@ -26,8 +27,8 @@ class EventTimer:
         _lib.check(_lib.load().mc_event_elapsed_ms(self._start, self._stop, ctypes.byref(ms)))
         return float(ms.value)
 
-    def __del__(self):
-        if _lib is None or _lib.closing():
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing() or _lib.closing():
             return
         _lib.load().mc_event_destroy(self._start)
         _lib.load().mc_event_destroy(self._stop)

@@ -54,6 +54,10 @@ int mc_init(int device);
  * core/parallel_tempering.py:151,171-188). */
 int mc_malloc(void **ptr, size_t bytes);
 int mc_free(void *ptr);
+/* Pinned (page-locked) host memory, zeroed, for asynchronous copies from
+ * the device (mc_batch_run's error_host). */
+int mc_host_alloc(void **ptr, size_t bytes);
+int mc_host_free(void *ptr);
 int mc_memcpy_htod(void *dst, const void *src, size_t bytes);
 int mc_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int mc_memcpy_dtod(void *dst, const void *src, size_t bytes);
@@ -254,7 +258,7 @@ int mc_kawamura_twist_grad(void *stream, int dtype, const void *state, int R, in
  * No single reference function: this runs core/simulation.py:36-72's loop
  * (sweeps from core/runtime.py:87-99, exchanges from
  * core/parallel_tempering.py:245-269, samples every sample_interval) for
- * one lattice size inside one cooperative launch.  model 0 = Ising (the
+ * one lattice size inside one persistent launch.  model 0 = Ising (the
  * reference update, bit-identical to mc_ising_update_color), model 1 = XY
  * float32 fast update (cached cos/sin, float32 sincos/exp), model 2 = the
  * same with Gaussian proposals (sigmas: R entries per ladder temperature; z
@@ -278,12 +282,22 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, int threads, void
                  int32_t *temperature_to_walker, uint64_t *pt_rng, long long *counts,
                  long long *trips, int sample_interval, long long sample_first, int max_samples,
                  double *sample_partials, int32_t *sample_w2t, uint64_t *halo, uint64_t *energy,
-                 int *error);
-/* error points to two ints, zeroed by each launch: [0] the watchdog code
- * (nonzero when a bounded wait gave up), [1] a count of started workgroups.
- * The writers of the shared PT state (betas, maps, PT streams, parity) wait
- * for that count before writing, so every workgroup reads the state the
- * launch started from. */
+                 int *error, long long *clock, int *error_host);
+/* halo, energy and error form the launch's workspace (error: two ints, [0]
+ * the watchdog code, nonzero when a bounded wait gave up, [1] a count of
+ * started workgroups; the writers of the shared PT state -- betas, maps, PT
+ * streams, parity -- wait for that count before writing, so every workgroup
+ * reads the state the launch started from).
+ * clock (host memory, two long longs, or NULL) counts the tags and workgroup
+ * starts issued into the workspace since it was zeroed, so that launches
+ * need not clear it: zero it with a new workspace, and again after an error;
+ * a launch zeroes the workspace itself (one small kernel first) when clock is
+ * NULL or clock[0] is 0, and before a count would overflow, and advances it.
+ * One clock per workspace; launches on one workspace run in order (one
+ * stream).
+ * error_host (pinned host memory from mc_host_alloc, or NULL) receives
+ * error[0] through an asynchronous copy queued on the stream after the
+ * launch: read it after synchronising the stream. */
 /* Test hook, no counterpart in the reference: later launches make their
  * end-of-launch writers wait for `missing_workgroups` more workgroups than
  * the grid has, and every bounded wait give up after `polls` polls (0: the

@@ -63,6 +63,44 @@ def test_batched_ising_split_launches_continue_the_same_chain(gpu):
     np.testing.assert_array_equal(one.betas.copy_to_host(), two.betas.copy_to_host())
 
 
+def test_launches_reuse_the_workspace_without_clearing_it(gpu):
+    """Short launches in a row leave tagged halo and energy words behind; with
+    the workspace clock the next launch waits only for tags above them, so the
+    chain equals the one whose workspace is zeroed before every launch and the
+    one run in a single launch; also with tags near the top of their range, and
+    when a clock with no room left makes the launch zero the workspace first."""
+    from core.batched import BatchedRun
+    from models.xy.model import XYModel
+    temps = np.linspace(0.8, 1.1, 6)
+    chunks = (1, 1, 2, 3, 5, 10, 17, 1, 4)
+
+    def make():
+        model = XYModel(J=1.0, proposal="gaussian", sigma=1.0, start="random")
+        return BatchedRun(model, 32, temps, seed=11, exchange_interval=3, pt_seed=5, strip_rows=8, threads=256)
+
+    kept, cleared, high, single = make(), make(), make(), make()
+    for i, n in enumerate(chunks):
+        kept.run(n)
+        assert kept._clock[0] > 0
+        cleared._clock[0] = 0
+        cleared.run(n)
+        if i == 1:
+            high._clock[0] = 0x7fff0000  # later tags near the top of their range
+        if i == len(chunks) - 1:
+            high._clock[0] = 0x7fffffff - 1  # no room for this launch's tags
+        high.run(n)
+    assert high._clock[0] == chunks[-1] + 1  # zeroed, then advanced
+    single.run(sum(chunks))
+    for other in (cleared, high, single):
+        np.testing.assert_array_equal(kept.snapshot(), other.snapshot())
+        np.testing.assert_array_equal(device_rng_as_pairs(kept.rng_states), device_rng_as_pairs(other.rng_states))
+        np.testing.assert_array_equal(kept.betas.copy_to_host(), other.betas.copy_to_host())
+        d_k, d_o = kept.diagnostics(), other.diagnostics()
+        for key in ("attempts", "accepted", "walker_to_temperature", "walker_round_trips"):
+            np.testing.assert_array_equal(d_k[key], d_o[key], err_msg=key)
+    assert kept.diagnostics()["attempts"].sum() > 0
+
+
 def test_batched_ising_simulation_samples_equal_drop_in_simulation(gpu):
     from core.batched import BatchedRun, BatchedSimulation
     from core.simulation import Simulation
