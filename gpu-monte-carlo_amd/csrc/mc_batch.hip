@@ -502,17 +502,16 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
         __syncthreads();  // the observables read every row
 
         // ---- strip observables: right and down bonds (the row below row H is
-        // the ghost row H+1, current for this sweep); the energy alone when
-        // only an exchange needs it
+        // the ghost row H+1, current for this sweep)
         long long ef = 0;  // integer energy sum of the strip
         double v[kChannels];
         double nacc = 0.0;  // the workgroup's accepted moves travel with the sums
-        {
+        if (do_sample) {
             double m[4] = {0.0, 0.0, 0.0, 0.0};
             int oi = obs_i, oj = obs_j;
 #pragma unroll 2
             for (int idx = tid; idx < H * L; idx += NT) {
-                site_observables<MODEL>(view, 1 + oi, oj, L, do_sample, ef, m);
+                site_observables<MODEL>(view, 1 + oi, oj, L, true, ef, m);
                 oi += obs_di;
                 oj += obs_dj;
                 if (oj >= L) {
@@ -520,19 +519,38 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
                     oi += 1;
                 }
             }
-            if (do_sample) {
-                double w[5] = {m[0], m[1], m[2], m[3], (double)n_acc};
-                block_sum<5, NT>(ef, w, red_e, red);
+            double w[5] = {m[0], m[1], m[2], m[3], (double)n_acc};
+            block_sum<5, NT>(ef, w, red_e, red);
 #pragma unroll
-                for (int c = 0; c < 4; c++) v[c + 1] = w[c];
-                nacc = w[4];
-            } else {
-                double w[1] = {(double)n_acc};
-                block_sum<1, NT>(ef, w, red_e, red);
-                nacc = w[0];
-            }
-            v[0] = energy_from_fixed(MODEL, J, ef);
+            for (int c = 0; c < 4; c++) v[c + 1] = w[c];
+            nacc = w[4];
+        } else {
+            // The energy alone, for an exchange: each thread adds the bonds of
+            // the own sites of its update slots (every own site once), with
+            // the site's (cos, sin) from its registers and the neighbours'
+            // LDS positions the slots already hold -- the same per-site terms
+            // as site_observables, so the same integer sum.
+#pragma unroll
+            for (int p = 0; p < NP; p++)
+#pragma unroll
+                for (int m = 0; m < MAXP; m++) {
+                    if (!slot_own[p][m]) continue;
+                    const int self = slot_self[p][m], dn = self + L, rt = slot_rt[p][m];
+                    if (MODEL == MODEL_ISING) {
+                        const int sv = lsp[self];
+                        ef += sv * ((int)lsp[dn] + (int)lsp[rt]);
+                    } else {
+                        const float cs = slot_c[p][m], ss = slot_s[p][m];
+                        const float bxc = fmaf(ss, ls[dn], cs * lc[dn]);
+                        const float byc = fmaf(ss, ls[rt], cs * lc[rt]);
+                        ef += bond_fixed(bxc + byc);
+                    }
+                }
+            double w[1] = {(double)n_acc};
+            block_sum<1, NT>(ef, w, red_e, red);
+            nacc = w[0];
         }
+        v[0] = energy_from_fixed(MODEL, J, ef);
         // moves so far belong to the current temperature
         if (tid == 0 && a.accepted && nacc > 0.0) atomicAdd(&a.accepted[my_t], (unsigned long long)nacc);
         n_acc = 0;
