@@ -100,6 +100,48 @@ struct PackGroup {
     uint64_t *energy_words;    // [2 (exchange parity)][R][S][2 (lo, hi)] tagged words
 };
 
+// A value the same in every lane, moved to scalar registers
+template <class T>
+__device__ __forceinline__ T uniform(T v) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+    } else {
+        const unsigned long long x = __builtin_bit_cast(unsigned long long, v);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)x);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(x >> 32));
+        return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+    }
+}
+
+// The fields of the descriptor that the sweep loop uses, read once at entry
+// into scalar registers (the others -- PT state, counters, records -- are
+// read through the table where they are used, at entry, exchanges and exit).
+// Read through the table where each is used, the loads inside the loop could
+// be neither scalar nor hoisted (the loop's stores might alias the table):
+// each became a vector load waited for on the spot, several memory round
+// trips per streamed record.
+struct PackHot {
+    int L, R, H, S, K, wg_count, exchange_interval;
+    float J, delta;
+    void *state;
+    uint64_t *rng;
+    const float *sigmas;
+    uint64_t *halo, *ghost_rng;
+    void *ghost_state;
+    uint64_t *energy_words;
+};
+__device__ __forceinline__ PackHot load_hot(const PackGroup *gp) {
+    PackHot g;
+    g.L = uniform(gp->L); g.R = uniform(gp->R); g.H = uniform(gp->H); g.S = uniform(gp->S); g.K = uniform(gp->K);
+    g.wg_count = uniform(gp->wg_count); g.exchange_interval = uniform(gp->exchange_interval);
+    g.J = uniform(gp->J); g.delta = uniform(gp->delta);
+    g.state = uniform(gp->state); g.rng = uniform(gp->rng); g.sigmas = uniform(gp->sigmas);
+    g.halo = uniform(gp->halo); g.ghost_rng = uniform(gp->ghost_rng); g.ghost_state = uniform(gp->ghost_state);
+    g.energy_words = uniform(gp->energy_words);
+    return g;
+}
+
 struct PackWork {
     int group, task_first, task_count, pad;
 };
@@ -158,16 +200,15 @@ __host__ __device__ inline PackLds pack_lds(int model, int L, int R, int H, int 
     return o;
 }
 
-// The group descriptor is read through a pointer where each field is used
-// (scalar loads), rather than held in registers for the whole launch.
 template <int MODEL, int NT>
 __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork &w, unsigned char *smem,
                          long long *red_e, double (*red)[NT / 16], const float4 *q_tab) {
-    const PackGroup &g = *gp;
+    const PackHot g = load_hot(gp);
+    const PackGroup &gc = *gp;  // the fields used outside the sweep loop's hot path
     const int L = g.L, H = g.H, S = g.S, R = g.R, K = g.K;
     const int HH = H >> 1, half = L >> 1;
     const int tid = threadIdx.x;
-    const int lw = (int)blockIdx.x - g.wg_first;  // workgroup index within the group
+    const int lw = (int)blockIdx.x - gc.wg_first;  // workgroup index within the group
     const bool use_pt = g.exchange_interval > 0;
     const float J = g.J, delta = g.delta;
 
@@ -187,17 +228,17 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
     unsigned *pt_cnt = reinterpret_cast<unsigned *>(smem + lay.cnt);       // [3][R-1]
 
     for (int k = tid; k < R; k += NT) {
-        pt_beta[k] = g.betas[k];
-        pt_w2t[k] = use_pt ? g.w2t[k] : k;
-        pt_t2w[k] = use_pt ? g.t2w[k] : k;
+        pt_beta[k] = gc.betas[k];
+        pt_w2t[k] = use_pt ? gc.w2t[k] : k;
+        pt_t2w[k] = use_pt ? gc.t2w[k] : k;
         moves[k] = 0;
     }
     if (use_pt) {
-        for (int k = tid; k < 2 * (R - 1); k += NT) pt_g[k] = g.pt_rng[k];
-        for (int k = tid; k < 5 * R; k += NT) pt_trips[k] = g.trips[k];
+        for (int k = tid; k < 2 * (R - 1); k += NT) pt_g[k] = gc.pt_rng[k];
+        for (int k = tid; k < 5 * R; k += NT) pt_trips[k] = gc.trips[k];
         for (int k = tid; k < 3 * (R - 1); k += NT) pt_cnt[k] = 0;
     }
-    int parity = use_pt ? *g.parity : 0;
+    int parity = use_pt ? *gc.parity : 0;
     const long long sweep0 = a.clock[1];
     bool exchanged = false;
     __syncthreads();
@@ -220,10 +261,10 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
     };
     auto flush_moves = [&]() {
         __syncthreads();
-        if (g.accepted)
+        if (gc.accepted)
             for (int k = tid; k < R; k += NT)
                 if (moves[k]) {
-                    atomicAdd(&g.accepted[pt_w2t[k]], (unsigned long long)moves[k]);
+                    atomicAdd(&gc.accepted[pt_w2t[k]], (unsigned long long)moves[k]);
                     moves[k] = 0;
                 }
         __syncthreads();
@@ -264,34 +305,62 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
             }
             // ---- load rows 1..H (lattice) and H+1 (ghost buffer); thread
             // (lrow, lj) takes column lj of rows 1 + lrow, 1 + lrow + lrows, ...
-            if (lrow < lrows)
-                for (int k = 0; k < K; k++) {
-                    const StripView v = view_of(k);
-                    const size_t rep = (size_t)(r0 + k) * L * L + (size_t)row0 * L + lj;
-                    const auto rng_src = as_global<RngRec>(g.rng) + rep;
-                    const size_t gh = ((size_t)(r0 + k) * S + s) * L + lj;
-#pragma unroll 2
-                    for (int lr = 1 + lrow; lr <= H + 1; lr += lrows) {
-                        const int off = (lr - 1) * L;
-                        const bool ghost = lr > H;
-                        const ulonglong2 rec =
-                            load_rec(ghost ? as_global<RngRec>(g.ghost_rng) + gh : rng_src + off);
-                        v.rng[off + ljc] = rec;
-                        if (MODEL == MODEL_ISING) {
-                            const int8_t sp = ghost ? as_global<int8_t>(g.ghost_state)[gh]
-                                                    : as_global<int8_t>(g.state)[rep + off];
-                            v.sp[off + L + ljc] = sp;
-                        } else {
-                            const float th = ghost ? as_global<float>(g.ghost_state)[gh]
-                                                   : as_global<float>(g.state)[rep + off];
-                            float sv, cv;
-                            fast_sincosf(th, sv, cv);
-                            v.c[off + L + ljc] = cv;
-                            v.s[off + L + ljc] = sv;
-                            v.th[off + ljc] = th;
+            // of each strip.  The loads of kLoadBatch records are issued
+            // before any of them is stored to LDS, so a strip costs about one
+            // memory round trip rather than one per record.
+            if (lrow < lrows) {
+                constexpr int kLoadBatch = 4;
+                // the next record: strip k, local row lr (none when the
+                // thread's first row is past the strip: k = K)
+                int k = 1 + lrow <= H + 1 ? 0 : K, lr = 1 + lrow;
+                while (k < K) {
+                    ulonglong2 rec[kLoadBatch];
+                    float th[kLoadBatch];
+                    int8_t sp[kLoadBatch];
+                    int at[kLoadBatch];  // (strip << 16) | local row of record i, or -1
+#pragma unroll
+                    for (int i = 0; i < kLoadBatch; i++) {
+                        at[i] = -1;
+                        if (k < K) {
+                            const size_t rep = (size_t)(r0 + k) * L * L + (size_t)(row0 + lr - 1) * L + lj;
+                            const size_t gh = ((size_t)(r0 + k) * S + s) * L + lj;
+                            const bool ghost = lr > H;
+                            rec[i] = load_rec(ghost ? as_global<RngRec>(g.ghost_rng) + gh
+                                                    : as_global<RngRec>(g.rng) + rep);
+                            if (MODEL == MODEL_ISING)
+                                sp[i] = ghost ? as_global<int8_t>(g.ghost_state)[gh] : as_global<int8_t>(g.state)[rep];
+                            else
+                                th[i] = ghost ? as_global<float>(g.ghost_state)[gh] : as_global<float>(g.state)[rep];
+                            at[i] = (k << 16) | lr;
+                            lr += lrows;
+                            if (lr > H + 1) {
+                                lr = 1 + lrow;
+                                k++;
+                            }
                         }
                     }
+#pragma unroll
+                    for (int i = 0; i < kLoadBatch; i++) {
+                        if (at[i] < 0) continue;
+                        const StripView v = view_of(at[i] >> 16);
+                        const int off = ((at[i] & 0xffff) - 1) * L;
+                        v.rng[off + ljc] = rec[i];
+                        if (MODEL == MODEL_ISING) {
+                            v.sp[off + L + ljc] = sp[i];
+                        } else {
+                            float sv, cv;
+                            fast_sincosf(th[i], sv, cv);
+                            v.c[off + L + ljc] = cv;
+                            v.s[off + L + ljc] = sv;
+                            v.th[off + ljc] = th[i];
+                        }
+                    }
+                    // every load of the batch has been used: said explicitly, so
+                    // that the next batch's loads are not each held back by a
+                    // wait for the registers they reuse
+                    __builtin_amdgcn_s_waitcnt(0);
                 }
+            }
             // ---- rows 0 and H+2: the prefetched words (polled again only if
             // a neighbour was late), or rows H and 2 of this strip itself
             // when it is a whole replica (S == 1)
@@ -536,26 +605,26 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
     flush_moves();
     if (lw == 0 && use_pt) {
         for (int k = tid; k < R; k += NT) {
-            g.betas[k] = pt_beta[k];
-            g.t2w[k] = pt_t2w[k];
-            g.w2t[k] = pt_w2t[k];
-            if (exchanged) g.energies[k] = pt_e[k];
+            gc.betas[k] = pt_beta[k];
+            gc.t2w[k] = pt_t2w[k];
+            gc.w2t[k] = pt_w2t[k];
+            if (exchanged) gc.energies[k] = pt_e[k];
         }
-        for (int k = tid; k < 2 * (R - 1); k += NT) g.pt_rng[k] = pt_g[k];
-        if (tid == 0) *g.parity = parity;
+        for (int k = tid; k < 2 * (R - 1); k += NT) gc.pt_rng[k] = pt_g[k];
+        if (tid == 0) *gc.parity = parity;
     }
     if (use_pt) {  // the kept walkers' records and pairs' counters
         const int P = R - 1;
         for (int wk = lw + tid * g.wg_count; wk < R; wk += NT * g.wg_count)
-            for (int row = 0; row < 5; row++) g.trips[row * R + wk] = pt_trips[row * R + wk];
+            for (int row = 0; row < 5; row++) gc.trips[row * R + wk] = pt_trips[row * R + wk];
         for (int left = lw + tid * g.wg_count; left < P; left += NT * g.wg_count)
-            for (int row = 0; row < 3; row++) g.counts[row * P + left] += pt_cnt[row * P + left];
+            for (int row = 0; row < 3; row++) gc.counts[row * P + left] += pt_cnt[row * P + left];
     }
     if (blockIdx.x == 0 && tid == 0) a.clock[0] = sweep0 + a.nsweeps;
 }
 
 template <int NT>
-__global__ __launch_bounds__(NT) void pack_kernel(PackArgs a) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void pack_kernel(PackArgs a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ long long red_e[NT / 16];
     __shared__ double red[4][NT / 16];
