@@ -60,33 +60,59 @@ __device__ __forceinline__ R gradient(Cpx<R> dx_x, Cpx<R> dx_y, Cpx<R> dy_x, Cpx
            R(2) * ((conj(dx_x) * dy_y).re + (conj(dx_y) * dy_x).re);
 }
 
+// Change of the three gradient cells touched by site (i, j) for spin row
+// mu: the site's old (x, y) and proposed values (ex, ey are their
+// differences), and the neighbours right (ip, j), up (i, jp), left (im, j),
+// down (i, jm), left-up (im, jp) and right-down (ip, jm).
+template <class R>
+__device__ __forceinline__ R gradient_cells(Cpx<R> x, Cpx<R> y, Cpx<R> ex, Cpx<R> ey, Cpx<R> rx, Cpx<R> ry,
+                                            Cpx<R> ux, Cpx<R> uy, Cpx<R> lx, Cpx<R> ly, Cpx<R> dx, Cpx<R> dy,
+                                            Cpx<R> lux, Cpx<R> luy, Cpx<R> drx, Cpx<R> dry) {
+    const R three = R(3), two = R(2), eight = R(8), four = R(4);
+    const Cpx<R> a = rx - x, b = ry - y, c = ux - x, d = uy - y;
+    const Cpx<R> al = x - lx, bl = y - ly, cl = lux - lx, dl = luy - ly;
+    const Cpx<R> ad = drx - dx, bd = dry - dy, cd = x - dx, dd = y - dy;
+    const Cpx<R> t1 = ((scale(three, a) + b) + c) + d;
+    const Cpx<R> fx = (neg(t1) + (scale(three, al) + dl)) + (cd + bd);
+    const Cpx<R> t2 = ((a + b) + c) + scale(three, d);
+    const Cpx<R> fy = (neg(t2) + (bl + cl)) + (scale(three, dd) + ad);
+    const R linear = two * ((conj(ex) * fx).re + (conj(ey) * fy).re);
+    const R quadratic = eight * (norm2(ex) + norm2(ey)) + four * (conj(ex) * ey).re;
+    return linear + quadratic;
+}
+
 // exact change of the three gradient cells touched by site (i, j)
 template <class R>
 __device__ __forceinline__ R gradient_difference(const R *__restrict__ blk, int L, int i, int j,
                                                  const Cpx<R> *prop) {
     const int ip = wrap_dn(i, L), im = wrap_up(i, L), jp = wrap_dn(j, L), jm = wrap_up(j, L);
-    const R three = R(3), two = R(2), eight = R(8), four = R(4);
     R delta_g = R(0);
 #pragma unroll
     for (int mu = 0; mu < 3; mu++) {
         const Cpx<R> x = ld(blk, L, i, j, mu, 0), y = ld(blk, L, i, j, mu, 1);
-        const Cpx<R> ex = prop[2 * mu] - x, ey = prop[2 * mu + 1] - y;
-        const Cpx<R> rx = ld(blk, L, ip, j, mu, 0), ry = ld(blk, L, ip, j, mu, 1);
-        const Cpx<R> ux = ld(blk, L, i, jp, mu, 0), uy = ld(blk, L, i, jp, mu, 1);
-        const Cpx<R> lx = ld(blk, L, im, j, mu, 0), ly = ld(blk, L, im, j, mu, 1);
-        const Cpx<R> dx = ld(blk, L, i, jm, mu, 0), dy = ld(blk, L, i, jm, mu, 1);
-        const Cpx<R> lux = ld(blk, L, im, jp, mu, 0), luy = ld(blk, L, im, jp, mu, 1);
-        const Cpx<R> drx = ld(blk, L, ip, jm, mu, 0), dry = ld(blk, L, ip, jm, mu, 1);
-        const Cpx<R> a = rx - x, b = ry - y, c = ux - x, d = uy - y;
-        const Cpx<R> al = x - lx, bl = y - ly, cl = lux - lx, dl = luy - ly;
-        const Cpx<R> ad = drx - dx, bd = dry - dy, cd = x - dx, dd = y - dy;
-        const Cpx<R> t1 = ((scale(three, a) + b) + c) + d;
-        const Cpx<R> fx = (neg(t1) + (scale(three, al) + dl)) + (cd + bd);
-        const Cpx<R> t2 = ((a + b) + c) + scale(three, d);
-        const Cpx<R> fy = (neg(t2) + (bl + cl)) + (scale(three, dd) + ad);
-        const R linear = two * ((conj(ex) * fx).re + (conj(ey) * fy).re);
-        const R quadratic = eight * (norm2(ex) + norm2(ey)) + four * (conj(ex) * ey).re;
-        delta_g += linear + quadratic;
+        delta_g += gradient_cells(x, y, prop[2 * mu] - x, prop[2 * mu + 1] - y, ld(blk, L, ip, j, mu, 0),
+                                  ld(blk, L, ip, j, mu, 1), ld(blk, L, i, jp, mu, 0), ld(blk, L, i, jp, mu, 1),
+                                  ld(blk, L, im, j, mu, 0), ld(blk, L, im, j, mu, 1), ld(blk, L, i, jm, mu, 0),
+                                  ld(blk, L, i, jm, mu, 1), ld(blk, L, im, jp, mu, 0), ld(blk, L, im, jp, mu, 1),
+                                  ld(blk, L, ip, jm, mu, 0), ld(blk, L, ip, jm, mu, 1));
+    }
+    return delta_g;
+}
+
+// The site and the six neighbours its gradient cells read, in the order
+// self, right, up, left, down, left-up, right-down.
+constexpr int kStencil = 7;
+
+// the same from a stencil loaded beforehand (load_stencil)
+template <class R>
+__device__ __forceinline__ R gradient_difference(const Cpx<R> (&s)[kStencil][6], const Cpx<R> *prop) {
+    R delta_g = R(0);
+#pragma unroll
+    for (int mu = 0; mu < 3; mu++) {
+        const int a = 2 * mu, b = 2 * mu + 1;
+        delta_g += gradient_cells(s[0][a], s[0][b], prop[a] - s[0][a], prop[b] - s[0][b], s[1][a], s[1][b],
+                                  s[2][a], s[2][b], s[3][a], s[3][b], s[4][a], s[4][b], s[5][a], s[5][b],
+                                  s[6][a], s[6][b]);
     }
     return delta_g;
 }
@@ -95,6 +121,15 @@ template <class R>
 __device__ __forceinline__ void load_site(const R *__restrict__ blk, int L, int i, int j, Cpx<R> *psi) {
 #pragma unroll
     for (int m = 0; m < 6; m++) psi[m] = ld(blk, L, i, j, m >> 1, m & 1);
+}
+
+template <class R>
+__device__ __forceinline__ void load_stencil(const R *__restrict__ blk, int L, int i, int j,
+                                             Cpx<R> (&s)[kStencil][6]) {
+    const int ip = wrap_dn(i, L), im = wrap_up(i, L), jp = wrap_dn(j, L), jm = wrap_up(j, L);
+    const int si[kStencil] = {i, ip, i, im, i, im, ip}, sj[kStencil] = {j, j, jp, j, jm, jp, jm};
+#pragma unroll
+    for (int n = 0; n < kStencil; n++) load_site(blk, L, si[n], sj[n], s[n]);
 }
 
 template <class R>
@@ -120,9 +155,19 @@ __global__ __launch_bounds__(256) void he3_color_kernel(R *__restrict__ st, uint
     const int i = 2 * row + (color >> 1), j = 2 * col + (color & 1);
     R *blk = st + (size_t)r * L * L * 12;
     const size_t k = (size_t)r * L * L + (size_t)i * L + j;
+    // Every load is issued before any arithmetic, the stream first (the
+    // draws need only it) and the stencil after, so the site waits for one
+    // memory round trip rather than one per neighbour and component; at the
+    // small lattices of the reference's benchmark (L = 32, 100 replicas: less
+    // than one wave per SIMD) nothing else hides those waits.
     Rng g = load_rng(rng, k);
+    const R beta = betas[r];
+    Cpx<R> s[kStencil][6];
+    load_stencil(blk, L, i, j, s);
+    __builtin_amdgcn_sched_barrier(0);
     Cpx<R> psi[6];
-    load_site(blk, L, i, j, psi);
+#pragma unroll
+    for (int m = 0; m < 6; m++) psi[m] = s[0][m];
     const R old_potential = potential(psi, c);
 #pragma unroll
     for (int m = 0; m < 6; m++) {
@@ -131,11 +176,11 @@ __global__ __launch_bounds__(256) void he3_color_kernel(R *__restrict__ st, uint
         psi[m] = Cpx<R>{psi[m].re + re, psi[m].im + im};
     }
     const R dpot = potential(psi, c) - old_potential;
-    const R dgrad = gradient_difference(blk, L, i, j, psi);
+    const R dgrad = gradient_difference(s, psi);
     const R diff = dpot + c.K * dgrad;
     const R u = g.template uniform<R>();
     store_rng(rng, k, g);
-    if (metropolis_accept(diff, u, betas[r])) store_site(blk, L, i, j, psi);
+    if (metropolis_accept(diff, u, beta)) store_site(blk, L, i, j, psi);
 }
 
 // conditional field of the fixed neighbours (gaussian_fast.py:26-40)
