@@ -281,15 +281,19 @@ he3_gaussian_color_kernel(R *__restrict__ st, uint64_t *__restrict__ rng,
     const R log_ratio = -beta * dh + lq;
     const R u = g.template uniform<R>();
     store_rng(rng, k, g);
-    counts[k] += 1;
     // nonfinite: counted and rejected; else log r >= 0 or u < exp(log r),
     // evaluated without short-circuit control flow (see metropolis_accept)
     const bool finite = isfinite(log_ratio);
     const R x = log_ratio > R(0) ? R(0) : log_ratio;
     const bool accept = finite & ((log_ratio >= R(0)) | (u < rexp(x)));
-    if (!finite) counts[2 * N + k] += 1;
+    // The per-site counters by adds that return nothing (one thread per site,
+    // so the same counts as a read-modify-write, without waiting for a read
+    // of each counter at the end of every thread).
+    unsigned long long *cnt = reinterpret_cast<unsigned long long *>(counts);
+    atomicAdd(cnt + k, 1ull);
+    if (!finite) atomicAdd(cnt + 2 * N + k, 1ull);
     if (accept) {
-        counts[N + k] += 1;
+        atomicAdd(cnt + N + k, 1ull);
         store_site(blk, L, i, j, psi);
     }
 }
