@@ -325,10 +325,10 @@ class BatchedRun(Ensemble):
         _lib.call("mc_batch_workspace_size", self.code, self.L, self.R, self.H, ctypes.byref(hw), ctypes.byref(ew))
         self._halo = device_array(max(hw.value, 1), np.uint64)
         self._energy_words = device_array(max(ew.value, 1), np.uint64)
-        self._error = device_zeros(2, np.int32)  # watchdog code, started workgroups
-        # tags and workgroup starts issued into the workspace since it was
-        # zeroed (0: zero it at the next launch); the launches keep it
-        self._clock = (ctypes.c_longlong * 2)()
+        self._error = device_zeros(1, np.int32)  # the watchdog code
+        # tags issued into the workspace since it was zeroed (0: zero it at
+        # the next launch); the launches keep it
+        self._clock = (ctypes.c_longlong * 1)()
         self._error_host = PinnedArray(1, np.int32)  # the watchdog code after each launch
         self._no_samples = (device_array((1, self.R, self.strips, CHANNELS), np.float64),
                             device_array((1, self.R), np.int32))

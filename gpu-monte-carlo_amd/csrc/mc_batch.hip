@@ -42,9 +42,12 @@
 // later.  Each walker's first strip keeps its round-trip record and counts
 // the pairs it is the left member of; the PT streams go back from workgroup 0.
 // Every workgroup reads the shared PT state (betas, maps, PT streams, parity)
-// when it starts, and the writers of that state at the end of the launch
-// first wait until every workgroup has counted itself in (a start counter),
-// so no workgroup can read state of the launch's end, however late it starts.
+// when it starts.  The state at the end of the launch goes to a staging area
+// of the workspace, and a small kernel queued after the launch copies it in
+// place, so no workgroup can read state of the launch's end, however late it
+// starts.  (A start counter with an end-of-launch wait for it does the same
+// inside the launch, but its code alone -- never executed in a launch without
+// exchanges -- cost the sweep loop 5% on MI355X through register allocation.)
 //
 // Every wait is bounded: past a watchdog count a workgroup records an error
 // and leaves, so a broken invariant ends the launch instead of hanging it.
@@ -99,16 +102,38 @@ struct BatchArgs {
                                 //   (tag << 32 | payload)
     uint64_t *energy;           // [4 (exchange mod 4)][R (ladder temperature)][S][2 (lo, hi)] words
                                 //   (tag << 32 | half of the int64 energy partial)
-    int *error;                 // [0] error code, [1] workgroups started since the workspace was zeroed
+    int *error;                 // the watchdog code
     // Words written at the end of sweep n of this launch carry the tag
     // tag0 + n + 1.  tag0 is at least every tag an earlier launch wrote into
     // this workspace since it was zeroed, so a word left over from an earlier
     // launch never matches a tag this launch waits for, and the workspace
     // needs no clearing between launches.
     unsigned tag0;
-    int expect_started;         // start count the end-of-launch writers wait for
+    unsigned recv_tag0;         // tag0, or tags never published (test hook)
     unsigned watchdog;          // polls before a wait gives up
 };
+
+// The staging area of the end-of-launch PT state, after the energy words of
+// the workspace: PT streams (2 (R - 1) records), then walker betas, walker
+// to temperature and temperature to walker maps (R each) and the parity.
+struct BatchStage {
+    uint64_t *pt_rng;
+    float *betas;
+    int32_t *w2t, *t2w;
+    int *parity;
+};
+__host__ __device__ inline size_t batch_stage_words(int R) {
+    return (size_t)2 * (R > 1 ? R - 1 : 0) + ((size_t)3 * R + 1 + 1) / 2;
+}
+__host__ __device__ inline BatchStage batch_stage(uint64_t *energy, int R, int S) {
+    BatchStage st;
+    st.pt_rng = energy + (size_t)8 * R * S;
+    st.betas = reinterpret_cast<float *>(st.pt_rng + 2 * (R > 1 ? R - 1 : 0));
+    st.w2t = reinterpret_cast<int32_t *>(st.betas + R);
+    st.t2w = st.w2t + R;
+    st.parity = st.t2w + R;
+    return st;
+}
 
 // Item q of pass p (the five-pass schedule below): colour c and colour row
 // ri; false if the pass has fewer items.
@@ -265,9 +290,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             slot_pr[p][m] = slot_pc[p][m] = slot_ps[p][m] = slot_dc[p][m] = slot_ds[p][m] = 0.0f;
         }
     if (MODEL == MODEL_XY_GAUSS) normal_table_to_lds(q_tab);
-    __syncthreads();  // every thread's reads of the shared PT state have completed
-    if (use_pt && tid == 0)
-        __hip_atomic_fetch_add(a.error + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
     if (MODEL != MODEL_ISING) {
 #pragma unroll
         for (int p = 0; p < NP; p++)
@@ -369,9 +392,9 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             const int j = top ? k : k - L;
             const uint64_t *src = top ? halo_ptr(s_up, 1, par) : halo_ptr(s_dn, 0, par);
             uint32_t payload;
-            if (k == tid && (unsigned)(pre >> 32) == a.tag0 + (unsigned)n) {
+            if (k == tid && (unsigned)(pre >> 32) == a.recv_tag0 + (unsigned)n) {
                 payload = (uint32_t)pre;
-            } else if (!poll_word(src + j, a.tag0 + (unsigned)n, payload, a.error, a.watchdog)) {
+            } else if (!poll_word(src + j, a.recv_tag0 + (unsigned)n, payload, a.error, a.watchdog)) {
                 bad = 1;
                 break;
             }
@@ -666,24 +689,20 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
 
 done:
     flush_accepted();
-    if (use_pt && (s == 0 || blockIdx.x == 0)) {
-        // the shared PT state is overwritten only after every workgroup has read it
-        int bad = 0;
-        if (tid == 0 && !wait_count(a.error + 1, a.expect_started, a.error, a.watchdog)) bad = 1;
-        if (__syncthreads_or(bad)) {
-            if (tid == 0) atomicOr(a.error, BATCH_ERR_WATCHDOG);
-        }
-    }
-    if (use_pt) {  // the PT state after the last exchange
+    // the PT state after the last exchange: betas, maps, PT streams and parity
+    // (read by every workgroup at its start) to the staging area, the rest --
+    // read only by the workgroup that writes it -- in place
+    const BatchStage stage = batch_stage(a.energy, R, S);
+    if (use_pt) {
         if (s == 0 && tid == 0) {  // this walker's temperature, beta, energy and record
-            a.betas[r] = my_beta;
-            a.w2t[r] = my_t;
-            a.t2w[my_t] = r;
+            stage.betas[r] = my_beta;
+            stage.w2t[r] = my_t;
+            stage.t2w[my_t] = r;
             if (exchanged) a.energies[r] = my_e;
             for (int row = 0; row < 5; row++) a.trips[row * R + r] = my_trips[row];
         }
         if (blockIdx.x == 0)
-            for (int k = tid; k < 2 * (R - 1); k += NT) a.pt_rng[k] = pt_g[k];
+            for (int k = tid; k < 2 * (R - 1); k += NT) stage.pt_rng[k] = pt_g[k];
     }
     // ---- write back own rows' state and the streams of owned sites
     __syncthreads();
@@ -706,22 +725,35 @@ done:
                 a.rng[2 * gk] = slot_s0[p][m];
                 a.rng[2 * gk + 1] = slot_s1[p][m];
             }
-    if (blockIdx.x == 0 && tid == 0 && use_pt) *a.parity = parity;
+    if (blockIdx.x == 0 && tid == 0 && use_pt) *stage.parity = parity;
 }
 
-// Zeroes the workspace: halo and energy words, the error word and the start
-// count.  A kernel rather than memset nodes, so a launch can be captured into
-// a graph.
+// After a launch with exchanges (queued behind it on its stream): the staged
+// PT state in place.
+__global__ void batch_commit_kernel(const uint64_t *energy, int R, int S, float *betas, int32_t *w2t,
+                                    int32_t *t2w, uint64_t *pt_rng, int *parity) {
+    const BatchStage stage = batch_stage(const_cast<uint64_t *>(energy), R, S);
+    for (int k = threadIdx.x; k < R; k += blockDim.x) {
+        betas[k] = stage.betas[k];
+        w2t[k] = stage.w2t[k];
+        t2w[k] = stage.t2w[k];
+    }
+    for (int k = threadIdx.x; k < 2 * (R - 1); k += blockDim.x) pt_rng[k] = stage.pt_rng[k];
+    if (threadIdx.x == 0) *parity = *stage.parity;
+}
+
+// Zeroes the workspace's halo and energy words and the error word.  A kernel
+// rather than memset nodes, so a launch can be captured into a graph.
 __global__ void batch_reset_kernel(uint64_t *halo, size_t nhalo, uint64_t *energy, size_t nenergy, int *error) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < nhalo; k += stride) halo[k] = 0;
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < nenergy; k += stride) energy[k] = 0;
-    if (blockIdx.x == 0 && threadIdx.x < 2) error[threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *error = 0;
 }
 
-// Test hooks (mc_batch_test_watchdog): extra workgroups the end-of-launch
-// writers wait for, and a shorter watchdog; both 0 in normal use.
-static int g_batch_test_missing = 0;
+// Test hooks (mc_batch_test_watchdog): receives that wait for tags no
+// workgroup publishes, and a shorter watchdog; both 0 in normal use.
+static int g_batch_test_unreachable = 0;
 // Launch mode of mc_batch_run (mc_batch_set_cooperative): a plain dispatch
 // after an occupancy check on the host (the default: on MI355X it starts
 // 20-30 us sooner than a cooperative launch of the same grid, measured on
@@ -744,10 +776,11 @@ static std::vector<Capacity> g_capacity;
 static unsigned g_batch_test_watchdog = 0;
 
 static size_t halo_words(int L, int R, int H) { return (size_t)R * (L / H) * 2 * 2 * L; }
-// Bounds of the workspace clock (tags are 32-bit, the start count an int)
-static constexpr long long kMaxTag = 0x7fffffffLL;
-static constexpr long long kMaxStarts = 0x3fffffffLL;
-static size_t energy_words(int L, int R, int H) { return (size_t)4 * R * (L / H) * 2; }
+// Bound of the workspace clock (tags are 32-bit); receives of the test hook
+// wait for tags this far above any published one
+static constexpr long long kMaxTag = 0x3fffffffLL;
+static constexpr unsigned kUnreachableTag = 0x40000000u;
+static size_t energy_words(int L, int R, int H) { return (size_t)4 * R * (L / H) * 2 + batch_stage_words(R); }
 
 static size_t batch_lds(int model, int L, int R, int H, int exchange) {
     (void)exchange;
@@ -802,9 +835,8 @@ using namespace mc;
 
 extern "C" {
 
-int mc_batch_test_watchdog(int missing_workgroups, unsigned polls) {
-    MC_REQUIRE(missing_workgroups >= 0, "missing_workgroups must be nonnegative");
-    g_batch_test_missing = missing_workgroups;
+int mc_batch_test_watchdog(int unreachable, unsigned polls) {
+    g_batch_test_unreachable = unreachable != 0;
     g_batch_test_watchdog = polls;
     return MC_OK;
 }
@@ -886,16 +918,14 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, int threads, void
     a.halo = halo; a.energy = energy; a.error = error;
     a.watchdog = g_batch_test_watchdog ? g_batch_test_watchdog : kWatchdog;
 
-    // The workspace clock: tags and workgroup starts issued since the
-    // workspace was last zeroed.  Zeroed first when there is no clock, when it
-    // reads zero (a new workspace, or one after an error) and before either
-    // count could overflow.
-    const long long starts = (long long)R * S + g_batch_test_missing;
-    const bool reset = clock == nullptr || clock[0] <= 0 || clock[1] < 0 ||
-                       clock[0] + (long long)nsweeps + 1 > kMaxTag || clock[1] + starts > kMaxStarts;
-    const long long tag0 = reset ? 0 : clock[0], started0 = reset ? 0 : clock[1];
+    // The workspace clock: tags issued since the workspace was last zeroed.
+    // Zeroed first when there is no clock, when it reads zero (a new
+    // workspace, or one after an error) and before the count could overflow.
+    MC_REQUIRE((long long)nsweeps + 1 <= kMaxTag, "at most %lld sweeps per launch", kMaxTag - 1);
+    const bool reset = clock == nullptr || clock[0] <= 0 || clock[0] + (long long)nsweeps + 1 > kMaxTag;
+    const long long tag0 = reset ? 0 : clock[0];
     a.tag0 = (unsigned)tag0;
-    a.expect_started = (int)(started0 + starts);
+    a.recv_tag0 = a.tag0 + (g_batch_test_unreachable && S > 1 ? kUnreachableTag : 0u);
 
     const void *kern = kernel_ptr(model, threads, batch_items(L, H, threads));
     if (lds > 64 * 1024)
@@ -936,10 +966,12 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, int threads, void
                    "bytes of LDS)", R * S, capacity, lds);
         MC_TRY_HIP(hipLaunchKernel(kern, dim3((unsigned)(R * S)), dim3(threads), kargs, lds, as_stream(stream)));
     }
-    if (clock) {  // every tag of this launch is below tag0 + nsweeps + 1
-        clock[0] = tag0 + nsweeps + 1;
-        clock[1] = started0 + (long long)R * S;
+    if (exchange_interval > 0) {
+        hipLaunchKernelGGL(batch_commit_kernel, dim3(1), dim3(256), 0, as_stream(stream), energy, R, S, betas, w2t,
+                           t2w, pt_rng, parity);
+        MC_TRY_HIP(hipGetLastError());
     }
+    if (clock) clock[0] = tag0 + nsweeps + 1;  // every tag of this launch is below it
     if (error_host)
         MC_TRY_HIP(hipMemcpyAsync(error_host, error, sizeof(int), hipMemcpyDeviceToHost, as_stream(stream)));
     return MC_OK;

@@ -83,19 +83,6 @@ __device__ inline bool poll_word(const uint64_t *p, unsigned seq, uint32_t &payl
     }
 }
 
-// Bounded wait until the counter *p reaches `target`; false when the
-// watchdog expires or another workgroup has recorded an error.
-__device__ inline bool wait_count(const int *p, int target, const int *error, unsigned limit = kWatchdog) {
-    unsigned n = 0;
-    for (;;) {
-        if (__hip_atomic_load(const_cast<int *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
-        __builtin_amdgcn_s_sleep(1);
-        if ((++n & 1023u) == 0 &&
-            (n > limit || __hip_atomic_load(const_cast<int *>(error), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
-            return false;
-    }
-}
-
 __device__ __forceinline__ void publish_word(uint64_t *p, unsigned seq, uint32_t payload) {
     __hip_atomic_store(p, ((uint64_t)seq << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

@@ -283,27 +283,30 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, int threads, void
                  long long *trips, int sample_interval, long long sample_first, int max_samples,
                  double *sample_partials, int32_t *sample_w2t, uint64_t *halo, uint64_t *energy,
                  int *error, long long *clock, int *error_host);
-/* halo, energy and error form the launch's workspace (error: two ints, [0]
- * the watchdog code, nonzero when a bounded wait gave up, [1] a count of
- * started workgroups; the writers of the shared PT state -- betas, maps, PT
- * streams, parity -- wait for that count before writing, so every workgroup
- * reads the state the launch started from).
- * clock (host memory, two long longs, or NULL) counts the tags and workgroup
- * starts issued into the workspace since it was zeroed, so that launches
- * need not clear it: zero it with a new workspace, and again after an error;
- * a launch zeroes the workspace itself (one small kernel first) when clock is
- * NULL or clock[0] is 0, and before a count would overflow, and advances it.
- * One clock per workspace; launches on one workspace run in order (one
- * stream).
+/* halo, energy and error form the launch's workspace (error: one int, the
+ * watchdog code, nonzero when a bounded wait gave up).  The energy words end
+ * with a staging area for the PT state at the end of the launch (betas, maps,
+ * PT streams, parity); with exchanges, a small kernel queued after the launch
+ * copies it into betas, walker_to_temperature, temperature_to_walker, pt_rng
+ * and parity, so that no workgroup can read the state of the launch's end
+ * (every workgroup reads the shared PT state when it starts).
+ * clock (host memory, one long long, or NULL) counts the tags issued into the
+ * workspace since it was zeroed, so that launches need not clear it: zero it
+ * with a new workspace, and again after an error; a launch zeroes the
+ * workspace itself (one small kernel first) when clock is NULL or reads 0,
+ * and before the count could overflow (at most 2^30 - 2 sweeps per launch),
+ * and advances it.  One clock per workspace; launches on one workspace run in
+ * order (one stream).
  * error_host (pinned host memory from mc_host_alloc, or NULL) receives
  * error[0] through an asynchronous copy queued on the stream after the
  * launch: read it after synchronising the stream. */
-/* Test hook, no counterpart in the reference: later launches make their
- * end-of-launch writers wait for `missing_workgroups` more workgroups than
- * the grid has, and every bounded wait give up after `polls` polls (0: the
- * default, about a second).  (0, 0) restores normal launches.  Lets a test
- * force the watchdog path with every kernel still exiting normally. */
-int mc_batch_test_watchdog(int missing_workgroups, unsigned polls);
+/* Test hook, no counterpart in the reference: later launches make each
+ * strip's halo receives wait for tags that no workgroup publishes (when
+ * `unreachable` is nonzero and a replica has more than one strip), and every
+ * bounded wait give up after `polls` polls (0: the default, about a second).
+ * (0, 0) restores normal launches.  Lets a test force the watchdog path with
+ * every kernel still exiting normally. */
+int mc_batch_test_watchdog(int unreachable, unsigned polls);
 /* Launch mode of later mc_batch_run calls, no counterpart in the reference:
  * 0 (the default) a plain dispatch after a co-residency check (occupancy x
  * compute units) on the host, nonzero a cooperative launch. */

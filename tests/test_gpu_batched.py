@@ -85,9 +85,9 @@ def test_launches_reuse_the_workspace_without_clearing_it(gpu):
         cleared._clock[0] = 0
         cleared.run(n)
         if i == 1:
-            high._clock[0] = 0x7fff0000  # later tags near the top of their range
+            high._clock[0] = 0x3fff0000  # later tags near the top of their range
         if i == len(chunks) - 1:
-            high._clock[0] = 0x7fffffff - 1  # no room for this launch's tags
+            high._clock[0] = 0x3fffffff - 1  # no room for this launch's tags
         high.run(n)
     assert high._clock[0] == chunks[-1] + 1  # zeroed, then advanced
     single.run(sum(chunks))
@@ -219,9 +219,9 @@ def test_campaign_shape_serialises_launches_that_do_not_fit_together(gpu):
 
 
 def test_watchdog_expiry_raises_and_every_kernel_exits(gpu):
-    """Forced from the host: the end-of-launch writers wait for one workgroup
-    more than the grid has, with a short bounded wait; the launch ends, the
-    error reaches BatchedRun.synchronize, and the next launch runs normally."""
+    """Forced from the host: every strip's halo receives wait for tags that no
+    workgroup publishes, with a short bounded wait; the launch ends, the error
+    reaches BatchedRun.synchronize, and the next launch runs normally."""
     from core.batched import BatchedRun
     from mcgpu import _lib
     from models.xy.model import XYModel
