@@ -78,25 +78,32 @@ struct TileRegs {
     static constexpr int N = kHaloRows * kHaloCols, PER = (N + kTileThreads - 1) / kTileThreads;
     R v[PER][COMP];
 
+    // Every address first, then every load: loads issued one per branch, with
+    // each address computed into the registers of the previous one, were each
+    // held back until the previous load returned.  A lane past the halo (the
+    // last round of 256 threads covers only part of it) reads the last halo
+    // site again and store() drops it.
     __device__ __forceinline__ void load(const R *__restrict__ b, int L, int i0, int j0) {
+        const R *p[PER];
 #pragma unroll
         for (int k = 0; k < PER; k++) {
-            const int q = threadIdx.x + k * kTileThreads;
-            if (q < N) {
-                const int li = q / kHaloCols, lj = q - (q / kHaloCols) * kHaloCols;
-                int gi = i0 + li - 1, gj = j0 + lj - 1;
-                if (L >= kHaloRows) {  // gi, gj in [-1, 2L): one wrap
-                    gi = gi < 0 ? gi + L : (gi >= L ? gi - L : gi);
-                    gj = gj < 0 ? gj + L : (gj >= L ? gj - L : gj);
-                } else {
-                    gi = (gi % L + L) % L;
-                    gj = (gj % L + L) % L;
-                }
-                const R *p = b + ((size_t)gi * L + gj) * COMP;
-#pragma unroll
-                for (int m = 0; m < COMP; m++) v[k][m] = p[m];
+            const int q0 = threadIdx.x + k * kTileThreads;
+            const int q = q0 < N ? q0 : N - 1;
+            const int li = q / kHaloCols, lj = q - (q / kHaloCols) * kHaloCols;
+            int gi = i0 + li - 1, gj = j0 + lj - 1;
+            if (L >= kHaloRows) {  // gi, gj in [-1, 2L): one wrap
+                gi = gi < 0 ? gi + L : (gi >= L ? gi - L : gi);
+                gj = gj < 0 ? gj + L : (gj >= L ? gj - L : gj);
+            } else {
+                gi = (gi % L + L) % L;
+                gj = (gj % L + L) % L;
             }
+            p[k] = b + ((size_t)gi * L + gj) * COMP;
         }
+#pragma unroll
+        for (int k = 0; k < PER; k++)
+#pragma unroll
+            for (int m = 0; m < COMP; m++) v[k][m] = p[k][m];
     }
 
     __device__ __forceinline__ void store(Tile<R, COMP> &t) const {
