@@ -117,6 +117,8 @@ static int chain_color(const Chain &c, void *stream, int color) {
 static int chain_sweep(const Chain &c, void *stream) {
     if (c.kind == CHAIN_KAWAMURA && c.code == MC_F32)
         return mc_kawamura_sweep(stream, c.code, c.state, c.rng, c.betas, c.R, c.L, c.J, c.delta);
+    if (c.kind == CHAIN_HE3)
+        return mc_he3_sweep(stream, c.code, c.state, c.rng, c.betas, c.R, c.L, c.coef, c.K, c.delta, c.g_d);
     for (int color = 0; color < 4; color++) {
         const int rc = chain_color(c, stream, color);
         if (rc) return rc;

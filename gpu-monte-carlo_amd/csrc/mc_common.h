@@ -164,4 +164,67 @@ template <class R> __device__ __forceinline__ R norm2(Cpx<R> z) { return z.re * 
 __device__ __forceinline__ int wrap_up(int i, int L) { return i == 0 ? L - 1 : i - 1; }
 __device__ __forceinline__ int wrap_dn(int i, int L) { return i == L - 1 ? 0 : i + 1; }
 
+// Row staging of the row-staged sweep kernels: LDS rows 0..nrows-1 from the
+// global rows (first + lr) mod L of replica block b (row_reals reals per
+// row).  A thread takes columns tid, tid + nt, ... of every row; the loads of
+// kStageRows rows x kStageCols columns are issued (addresses first, an index
+// past the end clamped to a valid one) before any of them is stored, so the
+// staging costs a few memory round trips instead of one per value.
+constexpr int kStageRows = 3, kStageCols = 4;
+template <class R>
+__device__ __forceinline__ void stage_rows(R *rows, const R *b, int L, int row_reals, int nrows, int first,
+                                           int tid, int nt) {
+    const int ncols = (row_reals + nt - 1) / nt;
+    for (int lr0 = 0; lr0 < nrows; lr0 += kStageRows)
+        for (int c0 = 0; c0 < ncols; c0 += kStageCols) {
+            const R *src[kStageRows][kStageCols];
+#pragma unroll
+            for (int a = 0; a < kStageRows; a++) {
+                const int row = lr0 + a < nrows ? lr0 + a : nrows - 1;
+                const R *base = b + (size_t)((first + row + L) % L) * row_reals;
+#pragma unroll
+                for (int c = 0; c < kStageCols; c++) {
+                    const int x = tid + (c0 + c) * nt;
+                    src[a][c] = base + (x < row_reals ? x : row_reals - 1);
+                }
+            }
+            R v[kStageRows][kStageCols];
+#pragma unroll
+            for (int a = 0; a < kStageRows; a++)
+#pragma unroll
+                for (int c = 0; c < kStageCols; c++) v[a][c] = *src[a][c];
+#pragma unroll
+            for (int a = 0; a < kStageRows; a++)
+#pragma unroll
+                for (int c = 0; c < kStageCols; c++) {
+                    const int x = tid + (c0 + c) * nt;
+                    if (lr0 + a < nrows && x < row_reals) rows[(lr0 + a) * row_reals + x] = v[a][c];
+                }
+        }
+}
+
+// The own rows of a row kernel back: LDS rows 1, 3, ..., 2 n - 1 to global
+// rows first + 1, first + 3, ... (no wrap: they lie inside the block).
+template <class R>
+__device__ __forceinline__ void write_rows(R *b, const R *rows, int row_reals, int n, int first, int tid, int nt) {
+    const int ncols = (row_reals + nt - 1) / nt;
+    for (int kk = 0; kk < n; kk++) {
+        R *dst = b + (size_t)(first + 1 + 2 * kk) * row_reals;
+        const R *src = rows + (1 + 2 * kk) * row_reals;
+        for (int c0 = 0; c0 < ncols; c0 += kStageCols) {
+            R v[kStageCols];
+#pragma unroll
+            for (int c = 0; c < kStageCols; c++) {
+                const int x = tid + (c0 + c) * nt;
+                v[c] = src[x < row_reals ? x : row_reals - 1];
+            }
+#pragma unroll
+            for (int c = 0; c < kStageCols; c++) {
+                const int x = tid + (c0 + c) * nt;
+                if (x < row_reals) dst[x] = v[c];
+            }
+        }
+    }
+}
+
 }  // namespace mc

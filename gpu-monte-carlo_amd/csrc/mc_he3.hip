@@ -142,7 +142,29 @@ __device__ __forceinline__ void store_site(R *__restrict__ blk, int L, int i, in
     }
 }
 
-// Cartesian box proposal: 12 draws (re, im per component), then one accept draw.
+// One Cartesian Metropolis visit of a site (models/he3/update.py:186-227)
+// from its stencil (load_stencil's order) and its stream: the box proposal
+// (12 draws, re and im per component) into psi, then one acceptance draw;
+// returns the decision.
+template <class R>
+__device__ __forceinline__ bool he3_visit(const Cpx<R> (&s)[kStencil][6], Rng &g, R beta, const He3Coef<R> &c,
+                                          R delta, Cpx<R> (&psi)[6]) {
+#pragma unroll
+    for (int m = 0; m < 6; m++) psi[m] = s[0][m];
+    const R old_potential = potential(psi, c);
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        const R re = delta * (R(2) * g.template uniform<R>() - R(1));
+        const R im = delta * (R(2) * g.template uniform<R>() - R(1));
+        psi[m] = Cpx<R>{psi[m].re + re, psi[m].im + im};
+    }
+    const R dpot = potential(psi, c) - old_potential;
+    const R dgrad = gradient_difference(s, psi);
+    const R diff = dpot + c.K * dgrad;
+    const R u = g.template uniform<R>();
+    return metropolis_accept(diff, u, beta);
+}
+
 template <class R>
 __global__ __launch_bounds__(256) void he3_color_kernel(R *__restrict__ st, uint64_t *__restrict__ rng,
                                                         const R *__restrict__ betas, int L, int color,
@@ -166,21 +188,61 @@ __global__ __launch_bounds__(256) void he3_color_kernel(R *__restrict__ st, uint
     load_stencil(blk, L, i, j, s);
     __builtin_amdgcn_sched_barrier(0);
     Cpx<R> psi[6];
-#pragma unroll
-    for (int m = 0; m < 6; m++) psi[m] = s[0][m];
-    const R old_potential = potential(psi, c);
-#pragma unroll
-    for (int m = 0; m < 6; m++) {
-        const R re = delta * (R(2) * g.template uniform<R>() - R(1));
-        const R im = delta * (R(2) * g.template uniform<R>() - R(1));
-        psi[m] = Cpx<R>{psi[m].re + re, psi[m].im + im};
-    }
-    const R dpot = potential(psi, c) - old_potential;
-    const R dgrad = gradient_difference(s, psi);
-    const R diff = dpot + c.K * dgrad;
-    const R u = g.template uniform<R>();
+    const bool accept = he3_visit(s, g, beta, c, delta, psi);
     store_rng(rng, k, g);
-    if (metropolis_accept(diff, u, beta)) store_site(blk, L, i, j, psi);
+    if (accept) store_site(blk, L, i, j, psi);
+}
+
+// Half a sweep of the Cartesian update in one launch: colours 2 half then
+// 2 half + 1 of the rows of parity `half`, a workgroup owning whole rows.  A
+// site's stencil reaches rows i - 1 and i + 1 only, which have the other
+// parity and do not change in this launch, and its own row's sites of the
+// other colour, which the workgroup's first pass updates before the barrier
+// (a workgroup-scope release and acquire: the first pass's stores are
+// complete and visible to the workgroup's second-pass loads).  So two launches
+// per sweep leave what four colour launches leave, bit for bit: the visit is
+// the colour kernel's, each site loads its stream and stencil in one round
+// trip.  Thread (p, q) of the block of rows from i0 visits row
+// i0 + half + 2 p, columns 2 q and 2 q + 1.
+template <class R>
+__global__ __launch_bounds__(512) void he3_rows_kernel(R *__restrict__ st, uint64_t *__restrict__ rng,
+                                                       const R *__restrict__ betas, int L, int B, int half,
+                                                       He3Coef<R> c, R delta) {
+    const int r = blockIdx.y, h = L >> 1;
+    const int p = threadIdx.x / h, q = threadIdx.x - (threadIdx.x / h) * h;
+    const int i = blockIdx.x * B + half + 2 * p;
+    R *blk = st + (size_t)r * L * L * 12;
+    const size_t k0 = (size_t)r * L * L + (size_t)i * L + 2 * q;
+    const R beta = betas[r];
+#pragma unroll
+    for (int cc = 0; cc < 2; cc++) {
+        const int j = 2 * q + cc;
+        Rng g = load_rng(rng, k0 + cc);
+        Cpx<R> s[kStencil][6];
+        load_stencil(blk, L, i, j, s);
+        __builtin_amdgcn_sched_barrier(0);
+        Cpx<R> psi[6];
+        const bool accept = he3_visit(s, g, beta, c, delta, psi);
+        store_rng(rng, k0 + cc, g);
+        if (accept) store_site(blk, L, i, j, psi);
+        if (cc == 0) __syncthreads();
+    }
+}
+
+// Rows per workgroup of the He3 row kernel: among the even divisors B of L
+// whose workgroup of (B/2) * (L/2) threads stays within 512, the largest that
+// still gives every compute unit a workgroup (R * L / B >= cus), else the
+// smallest; 0 when none fits (the colour kernels run instead).  At L = 64,
+// R = 100: B = 16, 400 workgroups of 256 threads over 256 CUs, rather than
+// 200 workgroups of 512 threads on 200 of them.
+static int he3_rows_per_block(int L, int R, int cus) {
+    int spread = 0, smallest = 0;
+    for (int B = 2; B <= L; B += 2) {
+        if (L % B || (B / 2) * (L / 2) > 512) continue;
+        if (!smallest) smallest = B;
+        if ((long long)R * (L / B) >= cus) spread = B;
+    }
+    return spread ? spread : smallest;
 }
 
 // conditional field of the fixed neighbours (gaussian_fast.py:26-40)
@@ -362,6 +424,34 @@ static He3Coef<R> make_coef(const double *coef, double K, double g_d) {
     return c;
 }
 
+template <class R>
+static int he3_sweep(void *stream, void *state, uint64_t *rng, const void *betas, int R_, int L,
+                     const He3Coef<R> &c, R delta) {
+    static int cus[64] = {0};
+    int dev = 0;
+    MC_TRY_HIP(hipGetDevice(&dev));
+    if (dev >= 0 && dev < 64 && cus[dev] == 0)
+        MC_TRY_HIP(hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev));
+    const int B = he3_rows_per_block(L, R_, dev >= 0 && dev < 64 ? cus[dev] : 256);
+    const int h = L / 2;
+    if (B == 0) {
+        dim3 grid((unsigned)((h * h + 255) / 256), (unsigned)R_);
+        for (int color = 0; color < 4; color++) {
+            hipLaunchKernelGGL(he3_color_kernel<R>, grid, dim3(256), 0, as_stream(stream), (R *)state, rng,
+                               (const R *)betas, L, color, c, delta);
+            MC_CHECK_LAUNCH();
+        }
+        return MC_OK;
+    }
+    const int threads = (B / 2) * h;
+    for (int half = 0; half < 2; half++) {
+        hipLaunchKernelGGL(he3_rows_kernel<R>, dim3((unsigned)(L / B), (unsigned)R_), dim3(threads), 0,
+                           as_stream(stream), (R *)state, rng, (const R *)betas, L, B, half, c, delta);
+        MC_CHECK_LAUNCH();
+    }
+    return MC_OK;
+}
+
 }  // namespace mc
 
 using namespace mc;
@@ -391,6 +481,19 @@ int mc_he3_update_color(void *stream, int dtype, void *state, uint64_t *rng, con
     }
     MC_CHECK_LAUNCH();
     return MC_OK;
+}
+
+int mc_he3_sweep(void *stream, int dtype, void *state, uint64_t *rng, const void *betas, int R, int L,
+                 const double *coef, double K, double delta, double g_d) {
+    MC_REQUIRE(L >= 2 && L % 2 == 0, "L must be even, got %d", L);
+    MC_REQUIRE(R >= 1, "need at least one replica");
+    MC_REQUIRE(coef != nullptr, "He3 coefficients are required");
+    if (dtype == MC_F32)
+        return he3_sweep<float>(stream, state, rng, betas, R, L, make_coef<float>(coef, K, g_d), (float)delta);
+    if (dtype == MC_F64)
+        return he3_sweep<double>(stream, state, rng, betas, R, L, make_coef<double>(coef, K, g_d), delta);
+    set_error(MC_ERR_TYPE, "unsupported He3 dtype code %d", dtype);
+    return MC_ERR_TYPE;
 }
 
 int mc_he3_gaussian_update_color(void *stream, int dtype, void *state, uint64_t *rng, const void *betas,

@@ -128,8 +128,6 @@ __global__ __launch_bounds__(256) void kawamura_color_kernel(R *__restrict__ st,
 // lr.  Rows are loaded whole and coalesced; the own rows go back whole (both
 // colours of a row change in the same launch), and thread (p, q) keeps the
 // streams of its two sites (row p, columns 2q and 2q + 1) in registers.
-constexpr int kStageRows = 3, kStageCols = 4;  // values a thread stages per batch
-
 template <class R>
 __global__ __launch_bounds__(512) void kawamura_rows_kernel(R *__restrict__ st, uint64_t *__restrict__ rng, const R *__restrict__ betas,
                                      int L, int B, int half, R J, R delta) {
@@ -145,38 +143,8 @@ __global__ __launch_bounds__(512) void kawamura_rows_kernel(R *__restrict__ st, 
     // the streams first: in flight while the rows are staged
     Rng g0 = load_rng(rng, k0), g1 = load_rng(rng, k0 + 1);
     const R beta = betas[r];
-    // Stage rows 0..B (global rows i0 + half - 1 + lr, mod L).  A thread takes
-    // columns tid, tid + nt, ... of every row; the loads of kStageRows rows x
-    // kStageCols columns are issued (addresses first, an index past the end
-    // clamped to a valid one) before any of them is stored to LDS, so staging
-    // costs a few memory round trips instead of one per loaded value.
-    const int ncols = (row_reals + nt - 1) / nt;
-    for (int lr0 = 0; lr0 <= B; lr0 += kStageRows)
-        for (int c0 = 0; c0 < ncols; c0 += kStageCols) {
-            const R *src[kStageRows][kStageCols];
-#pragma unroll
-            for (int a = 0; a < kStageRows; a++) {
-                const int row = lr0 + a <= B ? lr0 + a : B;
-                const R *base = b + (size_t)((i0 + half - 1 + row + L) % L) * row_reals;
-#pragma unroll
-                for (int c = 0; c < kStageCols; c++) {
-                    const int x = tid + (c0 + c) * nt;
-                    src[a][c] = base + (x < row_reals ? x : row_reals - 1);
-                }
-            }
-            R v[kStageRows][kStageCols];
-#pragma unroll
-            for (int a = 0; a < kStageRows; a++)
-#pragma unroll
-                for (int c = 0; c < kStageCols; c++) v[a][c] = *src[a][c];
-#pragma unroll
-            for (int a = 0; a < kStageRows; a++)
-#pragma unroll
-                for (int c = 0; c < kStageCols; c++) {
-                    const int x = tid + (c0 + c) * nt;
-                    if (lr0 + a <= B && x < row_reals) rows[(lr0 + a) * row_reals + x] = v[a][c];
-                }
-        }
+    // rows 0..B: global rows i0 + half - 1 + lr (mod L)
+    stage_rows(rows, b, L, row_reals, B + 1, i0 + half - 1, tid, nt);
     __syncthreads();
 #pragma unroll
     for (int cc = 0; cc < 2; cc++) {
@@ -192,23 +160,7 @@ __global__ __launch_bounds__(512) void kawamura_rows_kernel(R *__restrict__ st, 
     }
     store_rng(rng, k0, g0);
     store_rng(rng, k0 + 1, g1);
-    for (int kk = 0; kk < B / 2; kk++) {
-        R *dst = b + (size_t)(i0 + half + 2 * kk) * row_reals;
-        const R *src = rows + (1 + 2 * kk) * row_reals;
-        for (int c0 = 0; c0 < ncols; c0 += kStageCols) {
-            R v[kStageCols];
-#pragma unroll
-            for (int c = 0; c < kStageCols; c++) {
-                const int x = tid + (c0 + c) * nt;
-                v[c] = src[x < row_reals ? x : row_reals - 1];
-            }
-#pragma unroll
-            for (int c = 0; c < kStageCols; c++) {
-                const int x = tid + (c0 + c) * nt;
-                if (x < row_reals) dst[x] = v[c];
-            }
-        }
-    }
+    write_rows(b, rows, row_reals, B / 2, i0 + half - 1, tid, nt);
 }
 
 // Rows per workgroup of the row kernels: the even divisor B of L whose

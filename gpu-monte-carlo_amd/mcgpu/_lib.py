@@ -76,6 +76,7 @@ SIGNATURES = {
     "mc_kawamura_measure": [P, INT, P, INT, INT, F64, P, P, P],
     "mc_kawamura_energy": [P, INT, P, INT, INT, F64, P],
     "mc_he3_update_color": [P, INT, P, P, P, INT, INT, INT, P, F64, F64, F64],
+    "mc_he3_sweep": [P, INT, P, P, P, INT, INT, P, F64, F64, F64],
     "mc_he3_gaussian_update_color": [P, INT, P, P, P, INT, INT, INT, P, F64, F64, F64, P, F64],
     "mc_he3_measure": [P, INT, P, INT, INT, P, F64, F64, P, P, P],
     "mc_he3_energy": [P, INT, P, INT, INT, P, F64, F64, P],

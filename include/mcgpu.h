@@ -150,6 +150,12 @@ int mc_kawamura_energy(void *stream, int dtype, const void *state, int R, int L,
 int mc_he3_update_color(void *stream, int dtype, void *state, uint64_t *rng, const void *betas,
                         int R, int L, int color, const double *coef, double K, double delta,
                         double g_d);
+/* One sweep of the Cartesian update: core/runtime.py:87-99 (colours 0..3 of
+ * models/he3/update.py:176-227) in two launches (colours 0+1 of the even rows,
+ * then 2+3 of the odd rows, a workgroup owning whole rows); bit-identical to
+ * four mc_he3_update_color calls. */
+int mc_he3_sweep(void *stream, int dtype, void *state, uint64_t *rng, const void *betas, int R, int L,
+                 const double *coef, double K, double delta, double g_d);
 /* models/he3/gaussian_fast.py:91-119 (conditional-Gaussian MH); counts is
  * int64 (3, R, L, L) */
 int mc_he3_gaussian_update_color(void *stream, int dtype, void *state, uint64_t *rng,

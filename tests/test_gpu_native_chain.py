@@ -157,3 +157,28 @@ def test_kawamura_row_sweeps_equal_four_colour_launches(gpu, L, R, dtype):
             b.model.update_color(b.state, b.rng_states, b.betas, color, b.blocks_per_grid, b.threads_per_block)
     np.testing.assert_array_equal(a.snapshot(), b.snapshot())
     np.testing.assert_array_equal(device_rng_as_pairs(a.rng_states), device_rng_as_pairs(b.rng_states))
+
+
+@pytest.mark.parametrize("L,R,dtype", [(32, 3, np.complex64), (64, 2, np.complex64), (80, 2, np.complex64),
+                                       (48, 2, np.complex128), (12, 3, np.complex128), (6, 2, np.complex64),
+                                       (4, 2, np.complex64)])
+def test_he3_row_sweeps_equal_four_colour_launches(gpu, L, R, dtype):
+    """The He3 row-staged sweep (two launches per sweep, rows in LDS) leaves
+    what four per-colour launches of update_color leave, bit for bit,
+    including the g_d dipole term."""
+    from core.runtime import Runtime
+    from models.he3.model import He3Model
+    temps = np.linspace(0.2, 0.4, R)
+
+    def model():
+        return He3Model(K=1.0, delta=0.15, pressure=20, reduced_temperature=1.0, g_d=0.3, dtype=dtype)
+
+    a = Runtime(model(), L, temps, seed=31)
+    b = Runtime(model(), L, temps, seed=31)
+    a.advance(6)
+    for _ in range(6):
+        for color in range(4):
+            b.model.update_color(b.state, b.rng_states, b.betas, color, b.blocks_per_grid, b.threads_per_block)
+    np.testing.assert_array_equal(a.snapshot(), b.snapshot())
+    np.testing.assert_array_equal(device_rng_as_pairs(a.rng_states), device_rng_as_pairs(b.rng_states))
+    assert not np.array_equal(a.snapshot(), model().initialize_state(L, R, 31))  # the sweeps moved sites
