@@ -197,13 +197,23 @@ __global__ __launch_bounds__(256) void he3_color_kernel(R *__restrict__ st, uint
 // 2 half + 1 of the rows of parity `half`, a workgroup owning whole rows.  A
 // site's stencil reaches rows i - 1 and i + 1 only, which have the other
 // parity and do not change in this launch, and its own row's sites of the
-// other colour, which the workgroup's first pass updates before the barrier
-// (a workgroup-scope release and acquire: the first pass's stores are
-// complete and visible to the workgroup's second-pass loads).  So two launches
+// other colour, which the workgroup's first pass updates.  So two launches
 // per sweep leave what four colour launches leave, bit for bit: the visit is
-// the colour kernel's, each site loads its stream and stencil in one round
-// trip.  Thread (p, q) of the block of rows from i0 visits row
-// i0 + half + 2 p, columns 2 q and 2 q + 1.
+// the colour kernel's.  Thread (p, q) of the block of rows from i0 visits
+// row i = i0 + half + 2 p, column 2 q (first pass) and 2 q + 1 (second).
+// The second site's stencil is
+//   self (i, 2q+1), left (i-1, 2q+1) and right-down (i+1, 2q): in the
+//       first site's stencil (its up, left-up and right neighbours);
+//   right (i+1, 2q+1) and left-up (i-1, 2q+2): loaded with the first
+//       stencil, in the same memory round trip;
+//   down (i, 2q): the thread's own first site after its visit;
+//   up (i, 2q+2): thread (p, q+1 mod L/2)'s first site after its visit,
+//       through LDS behind the workgroup barrier;
+// so the second pass waits for no memory round trip.  float64 keeps the
+// first form, the second stencil loaded after the barrier (a workgroup-scope
+// release and acquire orders the first pass's stores before those loads):
+// holding three of the first stencil's sites across the first visit spills
+// its registers.
 template <class R>
 __global__ __launch_bounds__(512) void he3_rows_kernel(R *__restrict__ st, uint64_t *__restrict__ rng,
                                                        const R *__restrict__ betas, int L, int B, int half,
@@ -214,18 +224,62 @@ __global__ __launch_bounds__(512) void he3_rows_kernel(R *__restrict__ st, uint6
     R *blk = st + (size_t)r * L * L * 12;
     const size_t k0 = (size_t)r * L * L + (size_t)i * L + 2 * q;
     const R beta = betas[r];
+    if constexpr (sizeof(R) == 8) {
 #pragma unroll
-    for (int cc = 0; cc < 2; cc++) {
-        const int j = 2 * q + cc;
-        Rng g = load_rng(rng, k0 + cc);
-        Cpx<R> s[kStencil][6];
-        load_stencil(blk, L, i, j, s);
+        for (int cc = 0; cc < 2; cc++) {
+            const int j = 2 * q + cc;
+            Rng g = load_rng(rng, k0 + cc);
+            Cpx<R> s[kStencil][6];
+            load_stencil(blk, L, i, j, s);
+            __builtin_amdgcn_sched_barrier(0);
+            Cpx<R> psi[6];
+            const bool accept = he3_visit(s, g, beta, c, delta, psi);
+            store_rng(rng, k0 + cc, g);
+            if (accept) store_site(blk, L, i, j, psi);
+            if (cc == 0) __syncthreads();
+        }
+    } else {
+        __shared__ Cpx<R> first[512][6];
+        const int ip = wrap_dn(i, L), im = wrap_up(i, L), jb = 2 * q + 1, jn = wrap_dn(jb, L);
+        Rng ga = load_rng(rng, k0), gb = load_rng(rng, k0 + 1);
+        Cpx<R> s[kStencil][6], right_b[6], left_up_b[6];
+        load_stencil(blk, L, i, 2 * q, s);
+        load_site(blk, L, ip, jb, right_b);
+        load_site(blk, L, im, jn, left_up_b);
         __builtin_amdgcn_sched_barrier(0);
+        // the first site's value after its visit, in LDS: the old value now,
+        // the proposal if accepted (a select of the two would put both arrays
+        // in scratch memory)
+#pragma unroll
+        for (int m = 0; m < 6; m++) first[threadIdx.x][m] = s[0][m];
         Cpx<R> psi[6];
-        const bool accept = he3_visit(s, g, beta, c, delta, psi);
-        store_rng(rng, k0 + cc, g);
-        if (accept) store_site(blk, L, i, j, psi);
-        if (cc == 0) __syncthreads();
+        const bool accept_a = he3_visit(s, ga, beta, c, delta, psi);
+        // every load has long arrived; waiting for them here keeps the second
+        // visit from waiting on this visit's stores (vmcnt counts both, in
+        // order)
+        __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
+        store_rng(rng, k0, ga);
+        if (accept_a) {
+            store_site(blk, L, i, 2 * q, psi);
+#pragma unroll
+            for (int m = 0; m < 6; m++) first[threadIdx.x][m] = psi[m];
+        }
+        __syncthreads();
+        const int up_thread = p * h + (q + 1 == h ? 0 : q + 1);
+        Cpx<R> t[kStencil][6];
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+            t[0][m] = s[2][m];
+            t[1][m] = right_b[m];
+            t[2][m] = first[up_thread][m];
+            t[3][m] = s[5][m];
+            t[4][m] = first[threadIdx.x][m];
+            t[5][m] = left_up_b[m];
+            t[6][m] = s[1][m];
+        }
+        const bool accept_b = he3_visit(t, gb, beta, c, delta, psi);
+        store_rng(rng, k0 + 1, gb);
+        if (accept_b) store_site(blk, L, i, jb, psi);
     }
 }
 

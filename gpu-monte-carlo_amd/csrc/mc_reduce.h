@@ -17,10 +17,13 @@ namespace mc {
 
 constexpr int kReduceThreads = 256;
 
-// Blocks per replica: at least 8 sites per thread, at most 64 blocks.
+// Blocks per replica: at least 2 sites per thread, at most 64 blocks (a
+// function of L alone, so the order of the additions does not depend on the
+// device).  With 8 sites per thread the He3 energy of L = 64 and 100
+// replicas ran as 200 workgroups, fewer than the compute units, at 12.5 us.
 inline int reduce_blocks(int L) {
     const long sites = (long)L * L;
-    long b = (sites + kReduceThreads * 8 - 1) / (kReduceThreads * 8);
+    long b = (sites + kReduceThreads * 2 - 1) / (kReduceThreads * 2);
     if (b < 1) b = 1;
     if (b > 64) b = 64;
     return (int)b;

@@ -160,12 +160,15 @@ def test_kawamura_row_sweeps_equal_four_colour_launches(gpu, L, R, dtype):
 
 
 @pytest.mark.parametrize("L,R,dtype", [(32, 3, np.complex64), (64, 2, np.complex64), (80, 2, np.complex64),
-                                       (48, 2, np.complex128), (12, 3, np.complex128), (6, 2, np.complex64),
-                                       (4, 2, np.complex64)])
+                                       (48, 2, np.complex64), (256, 1, np.complex64), (48, 2, np.complex128),
+                                       (12, 3, np.complex128), (6, 2, np.complex64), (4, 2, np.complex64),
+                                       (4, 3, np.complex128)])
 def test_he3_row_sweeps_equal_four_colour_launches(gpu, L, R, dtype):
-    """The He3 row-staged sweep (two launches per sweep, rows in LDS) leaves
-    what four per-colour launches of update_color leave, bit for bit,
-    including the g_d dipole term."""
+    """The He3 row sweep (two launches per sweep, a workgroup owning whole
+    rows; float32: the second pass's stencil from the first pass's registers
+    and a neighbour through LDS) leaves what four per-colour launches of
+    update_color leave, bit for bit, including the g_d dipole term, down to
+    L = 4, where a row's two threads are each other's neighbours."""
     from core.runtime import Runtime
     from models.he3.model import He3Model
     temps = np.linspace(0.2, 0.4, R)
