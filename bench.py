@@ -568,6 +568,10 @@ def main():
     ap.add_argument("--no-ising", action="store_true", help="skip the Ising L=32 R=8 whole-run comparison")
     ap.add_argument("--no-gradients", action="store_true", help="skip the differentiable-energy line")
     ap.add_argument("--no-he3-ref", action="store_true", help="skip the reference's He3 benchmark protocol")
+    ap.add_argument("--he3-sizes", default="32,48,64,80", help="lattice sizes of the He3 protocol (comma list)")
+    ap.add_argument("--he3-warmup", type=int, default=1000, help="warm-up sweeps of the He3 protocol")
+    ap.add_argument("--he3-sweeps", type=int, default=3000, help="sweeps per timed window of the He3 protocol")
+    ap.add_argument("--he3-repeats", type=int, default=5, help="timed windows per mode of the He3 protocol")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device once, so a counter run holds a "
                          "plain copy kernel of known size to scale its byte counters by")
@@ -680,7 +684,9 @@ def main():
                 films[f"{kind}_{r['dtype']}"] = r
         out["films"] = films
     if not args.no_he3_ref:
-        out["he3_reference_protocol"] = bench_he3_reference()
+        out["he3_reference_protocol"] = bench_he3_reference(
+            sizes=tuple(int(x) for x in args.he3_sizes.split(",")), warmup=args.he3_warmup,
+            sweeps=args.he3_sweeps, repeats=args.he3_repeats)
     if not args.no_ising:
         out["ising_config1"] = bench_ising_config1()
     if not args.no_gradients:
