@@ -193,7 +193,8 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     uint32_t *pt_part = reinterpret_cast<uint32_t *>(sig_tab + R);  // 2 (walker, partner) x S x (lo, hi)
     __shared__ long long my_trips[5];  // this walker's round-trip record (strip 0)
     __shared__ int x_acc;              // the exchange decision of this walker's pair
-    __shared__ double x_u, x_logu;     // its PT draw and log (when u > 0)
+    __shared__ double x_e;             // this walker's energy at the exchange
+    __shared__ int s_err;              // a wait of this workgroup expired
     __shared__ double red[kChannels][NT / 64];
     __shared__ long long red_e[NT / 64];
     __shared__ float thr[5];
@@ -290,6 +291,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             slot_pr[p][m] = slot_pc[p][m] = slot_ps[p][m] = slot_dc[p][m] = slot_ds[p][m] = 0.0f;
         }
     if (MODEL == MODEL_XY_GAUSS) normal_table_to_lds(q_tab);
+    if (tid == 0) s_err = 0;
     __syncthreads();
     if (MODEL != MODEL_ISING) {
 #pragma unroll
@@ -356,7 +358,12 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     // Rows 0 and H+2 for sweep n: the bottom row of strip s-1 and row 2 of
     // strip s+1 as of the end of sweep n-1 (tag tag0 + n).  With one strip they are
     // this strip's rows H and 2, which still hold sweep n-1 when this runs.
-    // Ends with a workgroup barrier; false if the watchdog expired.
+    // Ends with a workgroup barrier.  A thread whose wait expired records the
+    // watchdog error and sets s_err before the barrier; every thread reads
+    // s_err into err_seen after it, and the sweep loop leaves at the end of
+    // the sweep (the rest of the sweep only touches LDS and publishes rows
+    // that no valid launch will read: the error ends every workgroup's wait).
+    // This costs one barrier, where a block-wide OR takes three.
     uint64_t pre = 0;  // this thread's halo word (k = tid), loaded ahead of the receive
     auto prefetch = [&](int n) {
         if (S == 1 || n == 0 || tid >= 2 * L) return;
@@ -365,7 +372,8 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
         const uint64_t *src = top ? halo_ptr(s_up, 1, (n - 1) & 1) : halo_ptr(s_dn, 0, (n - 1) & 1);
         pre = __hip_atomic_load(const_cast<uint64_t *>(src + j), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     };
-    auto receive = [&](int n) -> bool {
+    int err_seen = 0;
+    auto receive = [&](int n) {
         if (S == 1) {
             for (int k = tid; k < 2 * L; k += NT) {
                 const bool top = k < L;
@@ -379,14 +387,13 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
                 }
             }
             __syncthreads();
-            return true;
+            return;
         }
         if (n == 0) {  // the launch loaded them
             __syncthreads();
-            return true;
+            return;
         }
         const int par = (n - 1) & 1;
-        int bad = 0;
         for (int k = tid; k < 2 * L; k += NT) {
             const bool top = k < L;
             const int j = top ? k : k - L;
@@ -395,16 +402,14 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             if (k == tid && (unsigned)(pre >> 32) == a.recv_tag0 + (unsigned)n) {
                 payload = (uint32_t)pre;
             } else if (!poll_word(src + j, a.recv_tag0 + (unsigned)n, payload, a.error, a.watchdog)) {
-                bad = 1;
+                atomicOr(a.error, BATCH_ERR_WATCHDOG);
+                s_err = 1;
                 break;
             }
             put_row(top ? 0 : H + 2, j, payload);
         }
-        if (__syncthreads_or(bad)) {
-            if (tid == 0) atomicOr(a.error, BATCH_ERR_WATCHDOG);
-            return false;
-        }
-        return true;
+        __syncthreads();
+        err_seen = s_err;
     };
     // After the colour-3 pass of sweep n: publish rows 2 and H (tag tag0 + n + 1).
     // Mirrors that pass's thread map, so each thread publishes the colour-3
@@ -511,13 +516,14 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
         apply(0, beta);
         __syncthreads();
         apply(1, beta);
-        if (!receive(n)) goto done;
+        receive(n);
         apply(2, beta);
         __syncthreads();
         apply(3, beta);
         __syncthreads();
         apply(4, beta);
         publish(n);
+        if (err_seen) goto done;
 
         const bool do_exchange = n == x_next;
         const bool do_sample = n == s_next;
@@ -569,9 +575,25 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
                         ef += bond_fixed(bxc + byc);
                     }
                 }
-            double w[1] = {(double)n_acc};
-            block_sum<1, NT>(ef, w, red_e, red);
-            nacc = w[0];
+            // the workgroup's totals in thread 0: wavefront sums by DPP, then
+            // the NT / 64 wavefront totals (integers: exact in any order)
+            const long long we = wave_sum_i64(ef), wa = wave_sum_i64((long long)n_acc);
+            if ((tid & 63) == 0) {
+                red_e[tid >> 6] = we;
+                red_i[tid >> 6] = (int)wa;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                long long te = 0;
+                int ta = 0;
+#pragma unroll
+                for (int w = 0; w < NT / 64; w++) {
+                    te += red_e[w];
+                    ta += red_i[w];
+                }
+                ef = te;
+                nacc = (double)ta;
+            }
         }
         v[0] = energy_from_fixed(MODEL, J, ef);
         // moves so far belong to the current temperature
@@ -597,53 +619,77 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             const int partner = (rel >= 0 && !(rel & 1) && my_t < R - 1) ? my_t + 1
                                 : (rel > 0 && (rel & 1)) ? my_t - 1 : -1;
             const int my_left = partner < 0 ? -1 : (partner < my_t ? partner : my_t);
-            // every attempted pair's stream takes one draw (energies are
-            // integer sums here, always finite).  The draws do not depend on
-            // the energies, so they are taken while the energy words are in
-            // flight, by the threads of the second wavefront (the first one
-            // polls); the thread of this walker's pair keeps u and log u.
+            // Every attempted pair's stream takes one draw (energies are
+            // integer sums here, always finite).  The threads of the second
+            // wavefront take the draws of the other pairs while the third
+            // wavefront (the deciding one) takes this pair's draw, gathers
+            // the strips of this walker and of its partner, adds them with
+            // DPP sums and decides.  The draws do not depend on the energies,
+            // so they run while the energy words are in flight.
             for (int left = parity + 2 * ((tid + NT - 64) % NT); left < R - 1; left += 2 * NT) {
+                if (left == my_left) continue;
                 Rng g;
                 g.s0 = pt_g[2 * left];
                 g.s1 = pt_g[2 * left + 1];
-                const double u = g.uniform_f64();
+                (void)g.next();
                 pt_g[2 * left] = g.s0;
                 pt_g[2 * left + 1] = g.s1;
-                if (left == my_left) {
-                    x_u = u;
-                    x_logu = u == 0.0 ? 0.0 : ::log(u);
-                }
             }
-            // gather the strips of this walker and of its partner only
-            int bad = 0;
-            const int nw = partner >= 0 ? 4 * S : 2 * S;
-            for (int k = tid; k < nw; k += NT) {
-                const int t = k < 2 * S ? my_t : partner;
-                uint32_t payload;
-                if (!poll_word(ring + (size_t)t * S * 2 + (k % (2 * S)), tag, payload, a.error, a.watchdog)) {
-                    bad = 1;
-                    break;
-                }
-                pt_part[k] = payload;
-            }
-            if (__syncthreads_or(bad)) {
-                if (tid == 0) atomicOr(a.error, BATCH_ERR_WATCHDOG);
-                goto done;
-            }
-            // this walker's pair decides (core/parallel_tempering.py:44-69)
-            if (tid == 0) {
-                const double e0 = energy_from_fixed(MODEL, J, strip_total(pt_part, S));
-                my_e = e0;
-                x_acc = 0;
+            if ((tid >> 6) == 2) {
+                const int lane = tid & 63;
+                double u = 0.0, logu = 0.0;
                 if (my_left >= 0) {
-                    const double e1 = energy_from_fixed(MODEL, J, strip_total(pt_part + 2 * S, S));
-                    const double ea = my_t == my_left ? e0 : e1, eb = my_t == my_left ? e1 : e0;
-                    const double lr = ((double)ladder[my_left] - (double)ladder[my_left + 1]) * (ea - eb);
-                    x_acc = (lr >= 0.0 || x_u == 0.0 || x_logu < lr) ? 1 : 0;
+                    Rng g;
+                    g.s0 = pt_g[2 * my_left];
+                    g.s1 = pt_g[2 * my_left + 1];
+                    u = g.uniform_f64();
+                    logu = u == 0.0 ? 0.0 : ::log(u);
+                    if (lane == 0) {
+                        pt_g[2 * my_left] = g.s0;
+                        pt_g[2 * my_left + 1] = g.s1;
+                    }
+                }
+                // word k: strip k / 2 (low half for even k) of this walker
+                // (k < 2 S), then of the partner
+                const int nw = partner >= 0 ? 4 * S : 2 * S;
+                int bad = 0;
+                long long e_own = 0, e_par = 0;
+                for (int k0 = 0; k0 < nw; k0 += 64) {
+                    const int k = k0 + lane;
+                    uint32_t payload = 0;
+                    if (k < nw) {
+                        const int t = k < 2 * S ? my_t : partner;
+                        if (!poll_word(ring + (size_t)t * S * 2 + (k % (2 * S)), tag, payload, a.error, a.watchdog))
+                            bad = 1;
+                    }
+                    // the high half from the next lane (quad permutation 1, 1, 3, 3)
+                    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)payload, 0xF5, 0xf, 0xf, false);
+                    const long long w = (k & 1) ? 0 : (long long)(((unsigned long long)hi << 32) | payload);
+                    e_own += wave_sum_i64(k < 2 * S ? w : 0);
+                    e_par += wave_sum_i64(k < 2 * S ? 0 : w);
+                }
+                if (lane == 0) {
+                    // this walker's pair decides (core/parallel_tempering.py:44-69)
+                    const double e0 = energy_from_fixed(MODEL, J, e_own);
+                    int acc = 0;
+                    if (my_left >= 0) {
+                        const double e1 = energy_from_fixed(MODEL, J, e_par);
+                        const double ea = my_t == my_left ? e0 : e1, eb = my_t == my_left ? e1 : e0;
+                        const double lr = ((double)ladder[my_left] - (double)ladder[my_left + 1]) * (ea - eb);
+                        acc = (lr >= 0.0 || u == 0.0 || logu < lr) ? 1 : 0;
+                    }
+                    x_acc = acc;
+                    x_e = e0;
+                }
+                if (__any(bad)) {
+                    if (lane == 0) atomicOr(a.error, BATCH_ERR_WATCHDOG);
+                    s_err = 1;
                 }
             }
             __syncthreads();
+            if (s_err) goto done;
             const bool acc = x_acc != 0;
+            if (tid == 0) my_e = x_e;
             if (s == 0 && tid == 0 && my_left == my_t) {  // the left walker's strip 0 counts the pair
                 atomicAdd(reinterpret_cast<unsigned long long *>(a.counts) + my_left, 1ull);
                 if (acc) atomicAdd(reinterpret_cast<unsigned long long *>(a.counts) + (R - 1) + my_left, 1ull);

@@ -87,6 +87,36 @@ __device__ __forceinline__ void publish_word(uint64_t *p, unsigned seq, uint32_t
     __hip_atomic_store(p, ((uint64_t)seq << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// x of the lane SH below in the same row of 16 lanes (0 for the first SH
+// lanes of a row): a DPP row shift of each 32-bit half.
+template <int SH>
+__device__ __forceinline__ long long dpp_row_shr_i64(long long x) {
+    const unsigned long long u = (unsigned long long)x;
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, 0x110 | SH, 0xf, 0xf, true);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), 0x110 | SH, 0xf, 0xf, true);
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// Sum of x over the 64 lanes of a wavefront, in every lane; every lane must
+// be active.  Inclusive sums within rows of 16 lanes by DPP row shifts, then
+// the four row totals from lanes 15, 31, 47 and 63.  A shuffle reduction
+// (ds_bpermute) takes six dependent LDS round trips instead.
+__device__ __forceinline__ long long wave_sum_i64(long long x) {
+    x += dpp_row_shr_i64<1>(x);
+    x += dpp_row_shr_i64<2>(x);
+    x += dpp_row_shr_i64<4>(x);
+    x += dpp_row_shr_i64<8>(x);
+    const unsigned long long u = (unsigned long long)x;
+    long long t = 0;
+#pragma unroll
+    for (int r = 15; r < 64; r += 16) {
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, r);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), r);
+        t += (long long)(((unsigned long long)hi << 32) | lo);
+    }
+    return t;
+}
+
 // Ising acceptance threshold for 2 s sum_nn = v: always flip when dE <= 0,
 // else u < exp(-beta dE) in float32 through the float64 exponential.
 __device__ __forceinline__ float ising_threshold(int v, float beta, float J) {
