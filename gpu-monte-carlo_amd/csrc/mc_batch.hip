@@ -20,8 +20,9 @@
 // of strip s needs from below.  Its update repeats the owner's operations on
 // the same inputs and streams, so both copies agree bit for bit.  So a strip
 // only waits for data of the previous sweep -- the bottom row of s-1 and row 2
-// of s+1 -- published at the end of that sweep and received after two colour
-// passes of interior rows, which hide the hand-off latency.  A strip can run
+// of s+1 -- published at the end of that sweep and used by the boundary sites
+// of the third and fourth colour passes, after two passes of interior rows
+// that hide the hand-off latency.  A strip can run
 // at most one sweep ahead of its neighbours, so halo slots are double-buffered
 // by sweep parity.  Each halo word carries its payload (angle or spin) and the
 // sweep it belongs to, so the receiver polls the data itself: one L2 round
@@ -190,7 +191,6 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     uint64_t *pt_g = reinterpret_cast<uint64_t *>(pt_base);        // 2 (R-1): PT streams
     float *ladder = reinterpret_cast<float *>(pt_g + 2 * (R - 1));  // R: beta of each ladder temperature
     float *sig_tab = ladder + R;                                    // R: proposal width of each temperature
-    uint32_t *pt_part = reinterpret_cast<uint32_t *>(sig_tab + R);  // 2 (walker, partner) x S x (lo, hi)
     __shared__ long long my_trips[5];  // this walker's round-trip record (strip 0)
     __shared__ int x_acc;              // the exchange decision of this walker's pair
     __shared__ double x_e;             // this walker's energy at the exchange
@@ -259,6 +259,15 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     constexpr int kPrefetchPass = 3;
     int slot_self[NP][MAXP], slot_lf[NP][MAXP], slot_rt[NP][MAXP];
     bool slot_on[NP][MAXP], slot_own[NP][MAXP];
+    // Colours 0 and 1 of local row 1 read row 0 (the strip above), and of
+    // the ghost row H+1 read row H+2 (the strip below); these boundary sites
+    // are items 0 and 1 of passes 2 and 3.  Each takes its one received word
+    // itself (halo word offset slot_hw, without the parity term), loaded
+    // ahead into slot_pre: the received rows are read by no other site, so
+    // they never go through LDS and need no barrier of their own.
+    int slot_bnd[2][MAXP];     // 0: none, 1: up from the strip above, 2: down from the strip below
+    int slot_hw[2][MAXP];
+    uint64_t slot_pre[2][MAXP];
     uint64_t slot_s0[NP][MAXP], slot_s1[NP][MAXP];
     SiteDraw slot_draw[NP][MAXP];
     // XY: the site's angle and (cos, sin), which only this thread changes, and
@@ -289,6 +298,14 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             }
             slot_draw[p][m] = SiteDraw{0.0f, 0.0f};
             slot_pr[p][m] = slot_pc[p][m] = slot_ps[p][m] = slot_dc[p][m] = slot_ds[p][m] = 0.0f;
+            if (p == 2 || p == 3) {
+                const int bnd = (on && c < 2 && ri == 0) ? 1 : (on && c < 2 && ri == HH) ? 2 : 0;
+                slot_bnd[p - 2][m] = bnd;
+                const int j = 2 * tcol + (c & 1);
+                slot_hw[p - 2][m] = bnd == 1 ? (int)(((((size_t)r * S + s_up) * 2 + 1) * 2) * L) + j
+                                             : (int)(((((size_t)r * S + s_dn) * 2 + 0) * 2) * L) + j;
+                slot_pre[p - 2][m] = 0;
+            }
         }
     if (MODEL == MODEL_XY_GAUSS) normal_table_to_lds(q_tab);
     if (tid == 0) s_err = 0;
@@ -305,15 +322,45 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             }
     }
 
-    // The sites of pass p with this sweep's draws
-    auto apply = [&](const int p, const float beta) {
+    int err_seen = 0;
+    // The received word of boundary slot (p, m) for sweep n (tag tag0 + n):
+    // the early load when it carries the tag, else polled.  A wait that
+    // expires records the watchdog error and sets s_err; the sweep loop
+    // leaves at the end of the sweep (see the sweep schedule below).
+    auto received = [&](const int p, const int m, const int n) -> uint32_t {
+        const unsigned tag = a.recv_tag0 + (unsigned)n;
+        const uint64_t w = slot_pre[p - 2][m];
+        uint32_t payload = (uint32_t)w;
+        if ((unsigned)(w >> 32) != tag &&
+            !poll_word(a.halo + slot_hw[p - 2][m] + ((n - 1) & 1) * L, tag, payload, a.error, a.watchdog)) {
+            atomicOr(a.error, BATCH_ERR_WATCHDOG);
+            s_err = 1;
+            payload = 0;
+        }
+        return payload;
+    };
+    // The sites of pass p with this sweep's draws; `recv`: boundary sites
+    // take their row-0 or row-(H+2) neighbour from the received word (S > 1,
+    // after the launch's first sweep), else from LDS.
+    auto apply = [&](const int p, const float beta, const bool recv, const int n) {
 #pragma unroll
         for (int m = 0; m < MAXP; m++) {
             if (!slot_on[p][m]) continue;
+            const bool bnd = (p == 2 || p == 3) && recv && slot_bnd[p == 3 ? 1 : 0][m] != 0;
+            const bool bnd_up = bnd && slot_bnd[p == 3 ? 1 : 0][m] == 1;
             if (MODEL == MODEL_ISING) {
-                if (site_apply<MODEL>(view, slot_self[p][m], slot_lf[p][m], slot_rt[p][m], L, beta, sigma, J, delta,
-                                      thr, slot_draw[p][m]))
+                const int self = slot_self[p][m], up = self - L, dn = self + L;
+                const int sv = lsp[self];
+                int s_up_v = lsp[up], s_dn_v = lsp[dn];
+                if (bnd) {
+                    const int rv = (int)(int8_t)(uint8_t)received(p, m, n);
+                    if (bnd_up) s_up_v = rv; else s_dn_v = rv;
+                }
+                const int nb = s_up_v + s_dn_v + (int)lsp[slot_lf[p][m]] + (int)lsp[slot_rt[p][m]];
+                if (slot_draw[p][m].a < thr[(2 * sv * nb + 8) >> 2]) {  // site_apply's Ising rule
+                    lsp[self] = (int8_t)(-sv);
                     n_acc += slot_own[p][m];
+                }
             } else {
                 const int self = slot_self[p][m], lf = slot_lf[p][m], rt = slot_rt[p][m];
                 const int up = self - L, dn = self + L;
@@ -321,8 +368,21 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
                 // left to itself the scheduler interleaves each read with a
                 // wait and an add, seven LDS round trips in a row per pass
                 // (measured on MI355X: 124.5 -> 144 site-updates/ns).
-                const float cu = lc[up], cd = lc[dn], cl = lc[lf], cr = lc[rt];
-                const float su = ls[up], sd = ls[dn], sl = ls[lf], sr = ls[rt];
+                float cu = lc[up], cd = lc[dn];
+                const float cl = lc[lf], cr = lc[rt];
+                float su = ls[up], sd = ls[dn];
+                const float sl = ls[lf], sr = ls[rt];
+                if (bnd) {
+                    float rs, rc;
+                    fast_sincosf(__uint_as_float(received(p, m, n)), rs, rc);
+                    if (bnd_up) {
+                        cu = rc;
+                        su = rs;
+                    } else {
+                        cd = rc;
+                        sd = rs;
+                    }
+                }
                 __builtin_amdgcn_sched_barrier(0);
                 const float Cs = ((cu + cd) + cl) + cr;
                 const float Ss = ((su + sd) + sl) + sr;
@@ -345,71 +405,37 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     auto halo_ptr = [&](int ss, int which, int par) {
         return a.halo + ((((size_t)r * S + ss) * 2 + which) * 2 + par) * L;
     };
-    auto put_row = [&](int lds_row, int j, uint32_t payload) {
-        if (MODEL == MODEL_ISING) {
-            lsp[lds_index(lds_row, j, L)] = (int8_t)(uint8_t)payload;
-        } else {
-            float sv, cv;
-            fast_sincosf(__uint_as_float(payload), sv, cv);
-            lc[lds_index(lds_row, j, L)] = cv;
-            ls[lds_index(lds_row, j, L)] = sv;
-        }
-    };
-    // Rows 0 and H+2 for sweep n: the bottom row of strip s-1 and row 2 of
-    // strip s+1 as of the end of sweep n-1 (tag tag0 + n).  With one strip they are
-    // this strip's rows H and 2, which still hold sweep n-1 when this runs.
-    // Ends with a workgroup barrier.  A thread whose wait expired records the
-    // watchdog error and sets s_err before the barrier; every thread reads
-    // s_err into err_seen after it, and the sweep loop leaves at the end of
-    // the sweep (the rest of the sweep only touches LDS and publishes rows
-    // that no valid launch will read: the error ends every workgroup's wait).
-    // This costs one barrier, where a block-wide OR takes three.
-    uint64_t pre = 0;  // this thread's halo word (k = tid), loaded ahead of the receive
+    // Rows 0 and H+2 for sweep n are the bottom row of strip s-1 and row 2 of
+    // strip s+1 as of the end of sweep n-1 (tag tag0 + n).  The boundary
+    // slots load their words early, in the draw phase (after the draws of
+    // the first kPrefetchPass passes' sites), and use them in passes 2 and 3;
+    // only a word that had not arrived yet is polled again.
     auto prefetch = [&](int n) {
-        if (S == 1 || n == 0 || tid >= 2 * L) return;
-        const bool top = tid < L;
-        const int j = top ? tid : tid - L;
-        const uint64_t *src = top ? halo_ptr(s_up, 1, (n - 1) & 1) : halo_ptr(s_dn, 0, (n - 1) & 1);
-        pre = __hip_atomic_load(const_cast<uint64_t *>(src + j), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (S == 1 || n == 0) return;
+#pragma unroll
+        for (int p = 2; p < 4; p++)
+#pragma unroll
+            for (int m = 0; m < MAXP; m++)
+                if (slot_bnd[p - 2][m])
+                    slot_pre[p - 2][m] = __hip_atomic_load(a.halo + slot_hw[p - 2][m] + ((n - 1) & 1) * L,
+                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     };
-    int err_seen = 0;
-    auto receive = [&](int n) {
-        if (S == 1) {
-            for (int k = tid; k < 2 * L; k += NT) {
-                const bool top = k < L;
-                const int j = top ? k : k - L;
-                const int dst = lds_index(top ? 0 : H + 2, j, L), src = lds_index(top ? H : 2, j, L);
-                if (MODEL == MODEL_ISING) {
-                    lsp[dst] = lsp[src];
-                } else {
-                    lc[dst] = lc[src];
-                    ls[dst] = ls[src];
-                }
-            }
-            __syncthreads();
-            return;
-        }
-        if (n == 0) {  // the launch loaded them
-            __syncthreads();
-            return;
-        }
-        const int par = (n - 1) & 1;
+    // With one strip, rows 0 and H+2 are this strip's rows H and 2, which
+    // still hold sweep n-1 before pass 2 (their colour-2 and colour-3 sites
+    // are updated in passes 2 to 4): copied in LDS before pass 2.
+    auto copy_single = [&]() {
         for (int k = tid; k < 2 * L; k += NT) {
             const bool top = k < L;
             const int j = top ? k : k - L;
-            const uint64_t *src = top ? halo_ptr(s_up, 1, par) : halo_ptr(s_dn, 0, par);
-            uint32_t payload;
-            if (k == tid && (unsigned)(pre >> 32) == a.recv_tag0 + (unsigned)n) {
-                payload = (uint32_t)pre;
-            } else if (!poll_word(src + j, a.recv_tag0 + (unsigned)n, payload, a.error, a.watchdog)) {
-                atomicOr(a.error, BATCH_ERR_WATCHDOG);
-                s_err = 1;
-                break;
+            const int dst = lds_index(top ? 0 : H + 2, j, L), src = lds_index(top ? H : 2, j, L);
+            if (MODEL == MODEL_ISING) {
+                lsp[dst] = lsp[src];
+            } else {
+                lc[dst] = lc[src];
+                ls[dst] = ls[src];
             }
-            put_row(top ? 0 : H + 2, j, payload);
         }
         __syncthreads();
-        err_seen = s_err;
     };
     // After the colour-3 pass of sweep n: publish rows 2 and H (tag tag0 + n + 1).
     // Mirrors that pass's thread map, so each thread publishes the colour-3
@@ -460,12 +486,15 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     // gives identical results.  Per sweep n:
     //   D   every site's draws for the sweep, from its stream (registers only)
     //   P1  c0 of interior 0/1 rows (local 3..H-1)
-    //   P2  c1 of the same rows; receive rows 0 and H+2 (sweep n-1)
-    //   P3  c0 of rows 1 and H+1 (ghost); c2 of local rows 4..H-2
-    //   P4  c1 of rows 1 and H+1; c2 of rows 2 and H
+    //   P2  c1 of the same rows
+    //   P3  c0 of rows 1 and H+1 (ghost), with rows 0 and H+2 (sweep n-1)
+    //       from the received words; c2 of local rows 4..H-2
+    //   P4  c1 of rows 1 and H+1, likewise; c2 of rows 2 and H
     //   P5  c3 of all 2/3 rows; publish rows 2 and H
-    // P5 of sweep n and P1 of sweep n+1 touch disjoint sites and P2 starts
-    // after a barrier, so P5 needs no barrier of its own.
+    // P2 and P3 read nothing the other writes (P3 reads colours 1 and 3 of
+    // sweep n-1 and colour 0 of P1), so no barrier separates them.  P5 of
+    // sweep n and P1 of sweep n+1 touch disjoint sites and P2 starts after a
+    // barrier, so P5 needs no barrier of its own: three barriers per sweep.
     long long x_next = a.x_next, x_index = a.x_index, s_next = a.s_next, s_index = a.s_index;
     // Every load of the set-up (the RNG streams above all) completes here.
     // Otherwise the compiler may wait for them inside the sweep loop, before
@@ -513,15 +542,17 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
                     }
                 }
         }
-        apply(0, beta);
+        const bool recv = S > 1 && n > 0;
+        apply(0, beta, false, n);
         __syncthreads();
-        apply(1, beta);
-        receive(n);
-        apply(2, beta);
+        apply(1, beta, false, n);
+        if (S == 1) copy_single();
+        apply(2, beta, recv, n);
         __syncthreads();
-        apply(3, beta);
+        apply(3, beta, recv, n);
         __syncthreads();
-        apply(4, beta);
+        err_seen = s_err;
+        apply(4, beta, false, n);
         publish(n);
         if (err_seen) goto done;
 
@@ -830,7 +861,7 @@ static size_t energy_words(int L, int R, int H) { return (size_t)4 * R * (L / H)
 
 static size_t batch_lds(int model, int L, int R, int H, int exchange) {
     (void)exchange;
-    const size_t pt_bytes = (size_t)16 * (R - 1) + (size_t)8 * R + (size_t)16 * (L / H);
+    const size_t pt_bytes = (size_t)16 * (R - 1) + (size_t)8 * R;
     return ((strip_bytes(model, L, H, false) + 15) & ~(size_t)15) + pt_bytes + 64;
 }
 
