@@ -205,20 +205,38 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     const int row0 = s * H;  // first global row of the strip
     const bool use_pt = a.exchange_interval > 0;
 
-    // ---- load the strip's state: rows 0..H+2 (global rows row0-1 .. row0+H+1)
-    for (int idx = tid; idx < (H + 3) * L; idx += NT) {
-        const int lr = idx / L, j = idx - lr * L;
-        const int gi = (row0 + lr - 1 + L) % L;
-        const size_t gk = rep + (size_t)gi * L + j;
-        if (MODEL == MODEL_ISING) {
-            lsp[lds_index(lr, j, L)] = reinterpret_cast<const int8_t *>(a.state)[gk];
-        } else {
-            const float th = reinterpret_cast<const float *>(a.state)[gk];
-            float sv, cv;
-            fast_sincosf(th, sv, cv);
-            lc[lds_index(lr, j, L)] = cv;
-            ls[lds_index(lr, j, L)] = sv;
-            if (lr >= 1 && lr <= H + 1) lth[lds_index(lr - 1, j, L)] = th;
+    // ---- load the strip's state: rows 0..H+2 (global rows row0-1 .. row0+H+1),
+    // the loads of kLoadBatch sites per thread issued before any is used
+    constexpr int kLoadBatch = 8;
+    for (int base = tid; base < (H + 3) * L; base += NT * kLoadBatch) {
+        float thv[kLoadBatch];
+        int spv[kLoadBatch];
+#pragma unroll
+        for (int b = 0; b < kLoadBatch; b++) {
+            const int idx = base + b * NT;
+            if (idx >= (H + 3) * L) continue;
+            const int lr = idx / L, j = idx - lr * L;
+            const size_t gk = rep + (size_t)((row0 + lr - 1 + L) % L) * L + j;
+            if (MODEL == MODEL_ISING)
+                spv[b] = reinterpret_cast<const int8_t *>(a.state)[gk];
+            else
+                thv[b] = reinterpret_cast<const float *>(a.state)[gk];
+        }
+#pragma unroll
+        for (int b = 0; b < kLoadBatch; b++) {
+            const int idx = base + b * NT;
+            if (idx >= (H + 3) * L) continue;
+            const int lr = idx / L, j = idx - lr * L;
+            if (MODEL == MODEL_ISING) {
+                lsp[lds_index(lr, j, L)] = (int8_t)spv[b];
+            } else {
+                const float th = thv[b];
+                float sv, cv;
+                fast_sincosf(th, sv, cv);
+                lc[lds_index(lr, j, L)] = cv;
+                ls[lds_index(lr, j, L)] = sv;
+                if (lr >= 1 && lr <= H + 1) lth[lds_index(lr - 1, j, L)] = th;
+            }
         }
     }
     if (use_pt) {

@@ -202,7 +202,7 @@ __host__ __device__ inline PackLds pack_lds(int model, int L, int R, int H, int 
 
 template <int MODEL, int NT>
 __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork &w, unsigned char *smem,
-                         long long *red_e, double (*red)[NT / 16], const float4 *q_tab) {
+                         long long *red_e, double (*red)[NT / 16], const float4 *q_tab, int &s_err) {
     const PackHot g = load_hot(gp);
     const PackGroup &gc = *gp;  // the fields used outside the sweep loop's hot path
     const int L = g.L, H = g.H, S = g.S, R = g.R, K = g.K;
@@ -241,6 +241,9 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
     int parity = use_pt ? *gc.parity : 0;
     const long long sweep0 = a.clock[1];
     bool exchanged = false;
+    // A wait that expired sets s_err before a barrier; every thread reads it
+    // after the barrier (one barrier, where a block-wide OR takes three).
+    if (tid == 0) s_err = 0;
     __syncthreads();
 
     // pass map: colour rows of the K strips spread over the threads
@@ -389,23 +392,27 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
                     if ((unsigned)(hw1 >> 32) != tag && !poll_word(halo_src(hq1), tag, payload, a.error)) bad = 1;
                     put(hq1, payload);
                 }
-                if (__syncthreads_or(bad)) {
+                if (bad) s_err = 1;
+                __syncthreads();
+                if (S == 1) {
+                    if (lrow < lrows)
+                        for (int qr = lrow; qr < 2 * K; qr += lrows) {
+                            const StripView v = view_of(qr >> 1);
+                            const int dst = (qr & 1) ? H + 2 : 0, src = (qr & 1) ? 2 : H;
+                            if (MODEL == MODEL_ISING) {
+                                v.sp[lds_index(dst, lj, L)] = v.sp[lds_index(src, lj, L)];
+                            } else {
+                                v.c[lds_index(dst, lj, L)] = v.c[lds_index(src, lj, L)];
+                                v.s[lds_index(dst, lj, L)] = v.s[lds_index(src, lj, L)];
+                            }
+                        }
+                    __syncthreads();
+                }
+                if (s_err) {
                     if (tid == 0) atomicOr(a.error, BATCH_ERR_WATCHDOG);
                     failed = true;
                     break;
                 }
-                if (S == 1 && lrow < lrows)
-                    for (int qr = lrow; qr < 2 * K; qr += lrows) {
-                        const StripView v = view_of(qr >> 1);
-                        const int dst = (qr & 1) ? H + 2 : 0, src = (qr & 1) ? 2 : H;
-                        if (MODEL == MODEL_ISING) {
-                            v.sp[lds_index(dst, lj, L)] = v.sp[lds_index(src, lj, L)];
-                        } else {
-                            v.c[lds_index(dst, lj, L)] = v.c[lds_index(src, lj, L)];
-                            v.s[lds_index(dst, lj, L)] = v.s[lds_index(src, lj, L)];
-                        }
-                    }
-                __syncthreads();
             }
 
             // ---- colours 0, 1, 2, 3 in the reference order; colours 0/1
@@ -538,7 +545,9 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
                 atomicAdd(reinterpret_cast<unsigned long long *>(psum + k / S),
                           ((unsigned long long)hi << 32) | lo);
             }
-            if (__syncthreads_or(bad)) {
+            if (bad) s_err = 1;
+            __syncthreads();
+            if (s_err) {
                 if (tid == 0) atomicOr(a.error, BATCH_ERR_WATCHDOG);
                 failed = true;
                 break;
@@ -629,16 +638,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void pa
     __shared__ long long red_e[NT / 16];
     __shared__ double red[4][NT / 16];
     __shared__ float4 q_tab[kNormalCells];  // Gaussian XY (ordered by pack_run's first barrier)
+    __shared__ int s_err;
     const PackWork w = a.work[blockIdx.x];
     const PackGroup *g = a.groups + w.group;
     const int model = g->model;
     if (model == MODEL_ISING) {
-        pack_run<MODEL_ISING, NT>(a, g, w, smem, red_e, red, q_tab);
+        pack_run<MODEL_ISING, NT>(a, g, w, smem, red_e, red, q_tab, s_err);
     } else if (model == MODEL_XY_FAST) {
-        pack_run<MODEL_XY_FAST, NT>(a, g, w, smem, red_e, red, q_tab);
+        pack_run<MODEL_XY_FAST, NT>(a, g, w, smem, red_e, red, q_tab, s_err);
     } else {
         normal_table_to_lds(q_tab);
-        pack_run<MODEL_XY_GAUSS, NT>(a, g, w, smem, red_e, red, q_tab);
+        pack_run<MODEL_XY_GAUSS, NT>(a, g, w, smem, red_e, red, q_tab, s_err);
     }
 }
 
