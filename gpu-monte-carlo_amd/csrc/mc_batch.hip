@@ -608,22 +608,44 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             // the site's (cos, sin) from its registers and the neighbours'
             // LDS positions the slots already hold -- the same per-site terms
             // as site_observables, so the same integer sum.
+            // Every slot's neighbour reads are issued before the first sum
+            // (the positions of a slot that is off are valid LDS positions
+            // too): with a branch per slot each slot took two LDS round trips
+            // of its own.
+            if (MODEL == MODEL_ISING) {
 #pragma unroll
-            for (int p = 0; p < NP; p++)
+                for (int p = 0; p < NP; p++)
 #pragma unroll
-                for (int m = 0; m < MAXP; m++) {
-                    if (!slot_own[p][m]) continue;
-                    const int self = slot_self[p][m], dn = self + L, rt = slot_rt[p][m];
-                    if (MODEL == MODEL_ISING) {
+                    for (int m = 0; m < MAXP; m++) {
+                        if (!slot_own[p][m]) continue;
+                        const int self = slot_self[p][m], dn = self + L, rt = slot_rt[p][m];
                         const int sv = lsp[self];
                         ef += sv * ((int)lsp[dn] + (int)lsp[rt]);
-                    } else {
-                        const float cs = slot_c[p][m], ss = slot_s[p][m];
-                        const float bxc = fmaf(ss, ls[dn], cs * lc[dn]);
-                        const float byc = fmaf(ss, ls[rt], cs * lc[rt]);
-                        ef += bond_fixed(bxc + byc);
                     }
-                }
+            } else {
+                float dc[NP][MAXP], dsn[NP][MAXP], rc[NP][MAXP], rsn[NP][MAXP];
+#pragma unroll
+                for (int p = 0; p < NP; p++)
+#pragma unroll
+                    for (int m = 0; m < MAXP; m++) {
+                        const int dn = slot_self[p][m] + L, rt = slot_rt[p][m];
+                        dc[p][m] = lc[dn];
+                        dsn[p][m] = ls[dn];
+                        rc[p][m] = lc[rt];
+                        rsn[p][m] = ls[rt];
+                    }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int p = 0; p < NP; p++)
+#pragma unroll
+                    for (int m = 0; m < MAXP; m++) {
+                        const float cs = slot_c[p][m], ss = slot_s[p][m];
+                        const float bxc = fmaf(ss, dsn[p][m], cs * dc[p][m]);
+                        const float byc = fmaf(ss, rsn[p][m], cs * rc[p][m]);
+                        const long long b = bond_fixed(bxc + byc);
+                        ef += slot_own[p][m] ? b : 0;
+                    }
+            }
             // the workgroup's totals in thread 0: wavefront sums by DPP, then
             // the NT / 64 wavefront totals (integers: exact in any order)
             const long long we = wave_sum_i64(ef), wa = wave_sum_i64((long long)n_acc);
