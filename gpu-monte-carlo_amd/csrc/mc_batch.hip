@@ -466,14 +466,16 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             if (q != 0 && q != HH - 1) continue;
             const int lr = 2 + 2 * q;
             uint64_t *dst = halo_ptr(s, q == 0 ? 0 : 1, par);
+            // both columns read before the first store
+            uint32_t payload[2];
 #pragma unroll
             for (int pp = 0; pp < 2; pp++) {
                 const int j = 2 * tcol + pp;
-                const uint32_t payload = MODEL == MODEL_ISING
-                                             ? (uint32_t)(uint8_t)lsp[lds_index(lr, j, L)]
-                                             : __float_as_uint(lth[lds_index(lr - 1, j, L)]);
-                publish_word(dst + j, a.tag0 + (unsigned)(n + 1), payload);
+                payload[pp] = MODEL == MODEL_ISING ? (uint32_t)(uint8_t)lsp[lds_index(lr, j, L)]
+                                                   : __float_as_uint(lth[lds_index(lr - 1, j, L)]);
             }
+#pragma unroll
+            for (int pp = 0; pp < 2; pp++) publish_word(dst + 2 * tcol + pp, a.tag0 + (unsigned)(n + 1), payload[pp]);
         }
     };
 
