@@ -61,7 +61,7 @@
 //  * MODEL_XY_GAUSS: the same with a Gaussian proposal theta + sigma_t z,
 //    sigma_t per ladder temperature (it follows the walker's temperature
 //    through exchanges); z and the acceptance uniform come from one 64-bit
-//    draw per site (gauss_draw in mc_fastmath.h).
+//    draw per site (gauss_draw_k in mc_fastmath.h).
 // Accepted moves are counted per ladder temperature when `accepted` is given.
 #include <algorithm>
 #include <mutex>
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     __shared__ long long red_e[NT / 64];
     __shared__ float thr[5];
     __shared__ int red_i[NT / 64];
-    __shared__ float4 q_tab[MODEL == MODEL_XY_GAUSS ? kNormalCells : 1];  // normal quantile table of gauss_draw
+    __shared__ float4 q_tab[MODEL == MODEL_XY_GAUSS ? kNormalCells : 1];  // normal quantile table of gauss_draw_k
 
     const size_t rep = (size_t)r * L * L;
     const int row0 = s * H;  // first global row of the strip
@@ -325,7 +325,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
                 slot_pre[p - 2][m] = 0;
             }
         }
-    if (MODEL == MODEL_XY_GAUSS) normal_table_to_lds(q_tab);
+    if (MODEL == MODEL_XY_GAUSS) normal_table_k_to_lds(q_tab);
     if (tid == 0) s_err = 0;
     __syncthreads();
     if (MODEL != MODEL_ISING) {
@@ -407,7 +407,8 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
                 float t = slot_dc[p][m] * Cs;
                 t = fmaf(slot_ds[p][m], Ss, t);
                 const float dE = -J * t;
-                if (slot_draw[p][m].b < accept_weight(-beta * dE)) {
+                const float w = MODEL == MODEL_XY_GAUSS ? accept_weight_k21(-beta * dE) : accept_weight(-beta * dE);
+                if (slot_draw[p][m].b < w) {
                     const float pr = slot_pr[p][m];
                     lth[up] = pr;
                     lc[self] = slot_pc[p][m];
@@ -535,7 +536,14 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
                     Rng g;
                     g.s0 = slot_s0[p][m];
                     g.s1 = slot_s1[p][m];
-                    const SiteDraw d = site_draw<MODEL>(g, q_tab);
+                    SiteDraw d;
+                    if (MODEL == MODEL_XY_GAUSS) {
+                        const GaussDraw gd = gauss_draw_k(g.next(), q_tab);  // z; v as k3 (see accept_weight_k21)
+                        d.a = gd.z;
+                        d.b = gd.v;
+                    } else {
+                        d = site_draw<MODEL>(g, q_tab);
+                    }
                     slot_draw[p][m] = d;
                     slot_s0[p][m] = g.s0;
                     slot_s1[p][m] = g.s1;

@@ -60,8 +60,13 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 
 // ------------------------------------------------------------- RNG
 // numba.cuda.random xoroshiro128+ (Blackman & Vigna); record = {s0, s1}.
+// Left rotation by k (32 < k < 64) as a right rotation by 64 - k < 32: each
+// half is a funnel shift of both halves (v_alignbit), two instructions where
+// the shift-and-or form takes three.
 __device__ __forceinline__ uint64_t rotl64(uint64_t x, unsigned k) {
-    return (x << k) | (x >> (64u - k));
+    const unsigned lo = (unsigned)x, hi = (unsigned)(x >> 32), r = 64u - k;
+    const unsigned nlo = __builtin_amdgcn_alignbit(hi, lo, r), nhi = __builtin_amdgcn_alignbit(lo, hi, r);
+    return ((uint64_t)nhi << 32) | nlo;
 }
 
 struct Rng {

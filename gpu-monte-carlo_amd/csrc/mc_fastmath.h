@@ -76,6 +76,25 @@ __device__ __forceinline__ float accept_weight(float x) {
     return p * __int_as_float(((int)k + 127) << 23);
 }
 
+// accept_weight(x) 2^21, exactly: the scale 2^k of the result built as
+// 2^(k+21) (accept_weight's value is at least exp(-87), a normal float, so
+// the product by 2^k is exact either way).  v < accept_weight(x) is then
+// k3 < accept_weight_k21(x) for v = k3 2^-21.
+__device__ __forceinline__ float accept_weight_k21(float x) {
+    x = fminf(fmaxf(x, -87.0f), 0.0f);
+    const float k = rintf(x * 0x1.715476p+0f);
+    float r = fmaf(-k, 0x1.62e430p-1f, x);
+    r = fmaf(-k, -0x1.05c610p-29f, r);
+    float p = fmaf(r, 0x1.a01a02p-13f, 0x1.6c16c2p-10f);
+    p = fmaf(p, r, 0x1.111112p-7f);
+    p = fmaf(p, r, 0x1.555556p-5f);
+    p = fmaf(p, r, 0x1.555556p-3f);
+    p = fmaf(p, r, 0x1p-1f);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
+    return p * __int_as_float(((int)k + 127 + 21) << 23);
+}
+
 // log for positive normal x (the engine calls it on uniforms in (0, 1])
 __device__ __forceinline__ float fast_logf(float x) {
     const unsigned b = __float_as_uint(x);
@@ -293,6 +312,20 @@ __device__ __forceinline__ void normal_table_to_lds(float4 *lds) {
     for (int k = threadIdx.x; k < kNormalCells; k += blockDim.x) lds[k] = kNormalQ[k];
 }
 
+// The table for gauss_draw_k: each cell's cubic in the 20 mantissa bits m of
+// w themselves (coefficients of t, t^2, t^3 scaled by 2^-20, 2^-40, 2^-60,
+// exact).  Every step of the Horner evaluation is then the original's scaled
+// by a power of two, and rounding commutes with such scaling while no value
+// comes near the subnormal range, so z is the same bit for bit (checked over
+// all 2^21 inputs: the smallest scaled coefficient is 1.5e-23, the smallest
+// scaled partial sum 1.5e-16).
+__device__ __forceinline__ void normal_table_k_to_lds(float4 *lds) {
+    for (int k = threadIdx.x; k < kNormalCells; k += blockDim.x) {
+        const float4 c = kNormalQ[k];
+        lds[k] = make_float4(c.x, c.y * 0x1p-20f, c.z * 0x1p-40f, c.w * 0x1p-60f);
+    }
+}
+
 // Gaussian angle step of the batched fast path from ONE 64-bit draw x:
 //   bits 63..43 -> k, z = (-1)^s Q((2k + 1) 2^-23) (inverse-CDF sampling)
 //   bit 42      -> the sign s, so the proposal is exactly symmetric
@@ -310,6 +343,22 @@ __device__ __forceinline__ GaussDraw gauss_draw(uint64_t x, const float4 *__rest
     GaussDraw d;
     d.z = (hi & 0x400u) ? -z : z;
     d.v = (float)((lo >> 1) & 0x1fffffu) * 0x1p-21f;
+    return d;
+}
+
+// gauss_draw with the table of normal_table_k_to_lds (no scaling of t) and
+// the acceptance uniform as the integer k3 of v = k3 2^-21 (exact in float),
+// to be compared with accept_weight_k21: the same z and the same decisions,
+// two instructions fewer per draw.
+__device__ __forceinline__ GaussDraw gauss_draw_k(uint64_t x, const float4 *__restrict__ tab) {
+    const unsigned hi = (unsigned)(x >> 32), lo = (unsigned)x;
+    const unsigned b = __float_as_uint((float)(2u * (hi >> 11) + 1u));
+    const float4 c = tab[(b >> 20) - 1016u];
+    const float t = (float)(b & 0xfffffu);
+    const float z = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
+    GaussDraw d;
+    d.z = (hi & 0x400u) ? -z : z;
+    d.v = (float)((lo >> 1) & 0x1fffffu);
     return d;
 }
 
