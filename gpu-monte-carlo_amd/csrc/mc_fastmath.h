@@ -27,11 +27,13 @@ __device__ __forceinline__ void fast_sincosf(float x, float &sn, float &cs) {
     pc = fmaf(r2, pc, 0x1.555556p-5f);
     pc = fmaf(r2, pc, -0x1p-1f);
     const float c = fmaf(r2, pc, 1.0f);
-    const int q = ((int)k) & 3;
+    const int q = (int)k;  // the quadrant is q mod 4
     const float s1 = (q & 1) ? c : s;
     const float c1 = (q & 1) ? s : c;
-    sn = (q & 2) ? -s1 : s1;
-    cs = ((q + 1) & 2) ? -c1 : c1;
+    // negation flips the sign bit: adding 2^31 does it (the carry leaves the
+    // word), one shift-and-add instead of a compare and a select
+    sn = __uint_as_float(__float_as_uint(s1) + ((unsigned)(q & 2) << 30));
+    cs = __uint_as_float(__float_as_uint(c1) + ((unsigned)((q + 1) & 2) << 30));
 }
 
 // Branch-free: the arguments at or below -104 (and NaN) take the select at
@@ -357,7 +359,10 @@ __device__ __forceinline__ GaussDraw gauss_draw_k(uint64_t x, const float4 *__re
     const float t = (float)(b & 0xfffffu);
     const float z = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
     GaussDraw d;
-    d.z = (hi & 0x400u) ? -z : z;
+    // Every table value is positive (the smallest, at k = 2^21 - 1, is
+    // 2.9e-7), so negating is setting the sign bit: bit 10 of hi moved to
+    // bit 31 and or-ed in, two instructions instead of a compare and a select.
+    d.z = __uint_as_float(__float_as_uint(z) | ((hi << 21) & 0x80000000u));
     d.v = (float)((lo >> 1) & 0x1fffffu);
     return d;
 }
