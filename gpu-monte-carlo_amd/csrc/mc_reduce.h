@@ -29,11 +29,7 @@ inline int reduce_blocks(int L) {
     return (int)b;
 }
 
-__device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-    return x;
-}
+// wave_sum (lane 0's sum of a wavefront) is in mc_common.h.
 
 // One block's C channel sums (v per thread) into partials[0..C-1] (thread 0).
 template <int C, int NT>

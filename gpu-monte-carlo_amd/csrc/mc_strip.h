@@ -130,14 +130,11 @@ __device__ __forceinline__ float ising_threshold(int v, float beta, float J) {
 template <int C, int NT>
 __device__ void block_sum(long long &e, double (&v)[C], long long *she, double (*sh)[NT / 64]) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
+    e = wave_sum_i64(e);  // every lane (integers: any order)
     if (lane == 0) she[wave] = e;
 #pragma unroll
     for (int c = 0; c < C; c++) {
-        double x = v[c];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        const double x = wave_sum(v[c]);  // lane 0: the shuffle-down tree's sum
         if (lane == 0) sh[c][wave] = x;
     }
     __syncthreads();

@@ -184,3 +184,32 @@ def test_float32_terms_accurate_for_angles_of_many_turns(gpu):
     (g64,) = torch.autograd.grad(H64.sum(), s64)
     assert torch.allclose(H32, H64, rtol=0, atol=2e-2), (H32 - H64).abs().max()
     assert (g32.double() - g64).abs().max() < 3e-4
+
+
+@pytest.mark.parametrize("L", [64, 128])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_whole_row_xy_kernels_against_closed_forms(gpu, L, dtype):
+    """The whole-row XY kernels (L a power of two from 64 to 1024: a workgroup per
+    16-row segment, rows by shuffles and LDS across wavefront edges and the
+    periodic wrap) against the bond sums and their gradient evaluated in numpy
+    float64, with a twist per replica."""
+    from autograd import lattice_energy
+    R = 3
+    th64 = _xy(R, L, torch.float64, 91 + L)
+    tw = torch.tensor([0.0, 0.3, -1.1], dtype=torch.float64, device=_dev())
+    J = 0.8
+    th = th64.to(dtype).requires_grad_(True)
+    H = lattice_energy(th, J, tw, "xy")
+    (g,) = torch.autograd.grad(H.sum(), th)
+    t = th.detach().double().cpu().numpy()
+    d = tw.cpu().numpy()[:, None, None]
+    ax = t - np.roll(t, -1, axis=1) - d  # (i, j) - (i+1, j), twisted
+    ay = t - np.roll(t, -1, axis=2)      # (i, j) - (i, j+1)
+    H_ref = -J * (np.cos(ax).sum((1, 2)) + np.cos(ay).sum((1, 2)))
+    g_ref = -J * (-np.sin(ax) + np.roll(np.sin(ax), 1, axis=1) - np.sin(ay) + np.roll(np.sin(ay), 1, axis=2))
+    if dtype == torch.float64:
+        np.testing.assert_allclose(H.detach().cpu().numpy(), H_ref, rtol=1e-12)
+        np.testing.assert_allclose(g.cpu().numpy(), g_ref, rtol=0, atol=1e-12)
+    else:
+        np.testing.assert_allclose(H.detach().cpu().numpy(), H_ref, rtol=0, atol=1e-3 * L)
+        np.testing.assert_allclose(g.cpu().numpy(), g_ref, rtol=0, atol=2e-5)
