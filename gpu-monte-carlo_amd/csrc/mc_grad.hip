@@ -369,7 +369,7 @@ __global__ __launch_bounds__(kTileThreads) void kawamura_grad_kernel(const R *__
 // j holding column j of the segment (plus the rows above and below it) in
 // registers.  A site's bonds along the column are then in the thread's own
 // registers and its bonds along the row come from the neighbouring lanes by
-// shuffles, with each wavefront's first and last lanes exchanged through LDS
+// DPP wave shifts, with each wavefront's first and last lanes exchanged through LDS
 // (the wavefronts of a workgroup cover a whole row, so the row's periodic
 // wrap is the last wavefront's neighbour being the first).  A site costs
 // (kSegRows + 1) / kSegRows sincos (terms) or (kSegRows + 2) / kSegRows
@@ -381,10 +381,25 @@ __global__ __launch_bounds__(kTileThreads) void kawamura_grad_kernel(const R *__
 // rounding differs from rotating each bond).
 constexpr int kSegRows = 16;
 
+// Lane i takes lane i + 1's value (shfl_dn1) or lane i - 1's (shfl_up1): a
+// DPP wave shift (wave_shl:1 / wave_shr:1, one VALU instruction per 32 bits,
+// checked on MI355X), the wavefront's last or first lane receiving 0 (the
+// callers take those from LDS).
+template <int CTRL>
+__device__ __forceinline__ float wave_shift(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double wave_shift(double x) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xf, 0xf, true);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xf, 0xf, true);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 template <class R>
-__device__ __forceinline__ R shfl_dn1(R x) { return __shfl_down(x, 1, 64); }
+__device__ __forceinline__ R shfl_dn1(R x) { return wave_shift<0x130>(x); }
 template <class R>
-__device__ __forceinline__ R shfl_up1(R x) { return __shfl_up(x, 1, 64); }
+__device__ __forceinline__ R shfl_up1(R x) { return wave_shift<0x138>(x); }
 
 template <class R, int NT>
 __global__ __launch_bounds__(NT) void xy_terms_rows_kernel(const R *__restrict__ th, int nseg, const double *twist,
