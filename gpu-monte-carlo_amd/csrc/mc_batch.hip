@@ -855,18 +855,24 @@ done:
     if (blockIdx.x == 0 && tid == 0 && use_pt) *stage.parity = parity;
 }
 
-// After a launch with exchanges (queued behind it on its stream): the staged
-// PT state in place.
+// Queued behind a launch on its stream: the staged PT state in place (after a
+// launch with exchanges), and the watchdog code into the caller's pinned host
+// word (error_host, device-accessible; NULL: none) -- a store of this kernel
+// instead of a copy operation of its own.
 __global__ void batch_commit_kernel(const uint64_t *energy, int R, int S, float *betas, int32_t *w2t,
-                                    int32_t *t2w, uint64_t *pt_rng, int *parity) {
-    const BatchStage stage = batch_stage(const_cast<uint64_t *>(energy), R, S);
-    for (int k = threadIdx.x; k < R; k += blockDim.x) {
-        betas[k] = stage.betas[k];
-        w2t[k] = stage.w2t[k];
-        t2w[k] = stage.t2w[k];
+                                    int32_t *t2w, uint64_t *pt_rng, int *parity, int exchanges, const int *error,
+                                    int *error_host) {
+    if (exchanges) {
+        const BatchStage stage = batch_stage(const_cast<uint64_t *>(energy), R, S);
+        for (int k = threadIdx.x; k < R; k += blockDim.x) {
+            betas[k] = stage.betas[k];
+            w2t[k] = stage.w2t[k];
+            t2w[k] = stage.t2w[k];
+        }
+        for (int k = threadIdx.x; k < 2 * (R - 1); k += blockDim.x) pt_rng[k] = stage.pt_rng[k];
+        if (threadIdx.x == 0) *parity = *stage.parity;
     }
-    for (int k = threadIdx.x; k < 2 * (R - 1); k += blockDim.x) pt_rng[k] = stage.pt_rng[k];
-    if (threadIdx.x == 0) *parity = *stage.parity;
+    if (error_host && threadIdx.x == 0) *error_host = *error;
 }
 
 // Zeroes the workspace's halo and energy words and the error word.  A kernel
@@ -1093,14 +1099,12 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, int threads, void
                    "bytes of LDS)", R * S, capacity, lds);
         MC_TRY_HIP(hipLaunchKernel(kern, dim3((unsigned)(R * S)), dim3(threads), kargs, lds, as_stream(stream)));
     }
-    if (exchange_interval > 0) {
+    if (exchange_interval > 0 || error_host) {
         hipLaunchKernelGGL(batch_commit_kernel, dim3(1), dim3(256), 0, as_stream(stream), energy, R, S, betas, w2t,
-                           t2w, pt_rng, parity);
+                           t2w, pt_rng, parity, exchange_interval > 0 ? 1 : 0, error, error_host);
         MC_TRY_HIP(hipGetLastError());
     }
     if (clock) clock[0] = tag0 + nsweeps + 1;  // every tag of this launch is below it
-    if (error_host)
-        MC_TRY_HIP(hipMemcpyAsync(error_host, error, sizeof(int), hipMemcpyDeviceToHost, as_stream(stream)));
     return MC_OK;
 }
 

@@ -54,8 +54,8 @@ int mc_init(int device);
  * core/parallel_tempering.py:151,171-188). */
 int mc_malloc(void **ptr, size_t bytes);
 int mc_free(void *ptr);
-/* Pinned (page-locked) host memory, zeroed, for asynchronous copies from
- * the device (mc_batch_run's error_host). */
+/* Pinned (page-locked) host memory, zeroed, device-accessible: targets of
+ * asynchronous copies and of device stores (mc_batch_run's error_host). */
 int mc_host_alloc(void **ptr, size_t bytes);
 int mc_host_free(void *ptr);
 int mc_memcpy_htod(void *dst, const void *src, size_t bytes);
@@ -304,8 +304,9 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, int threads, void
  * and advances it.  One clock per workspace; launches on one workspace run in
  * order (one stream).
  * error_host (pinned host memory from mc_host_alloc, or NULL) receives
- * error[0] through an asynchronous copy queued on the stream after the
- * launch: read it after synchronising the stream. */
+ * error[0] from the small kernel queued on the stream after the launch (a
+ * device store into the pinned word): read it after synchronising the
+ * stream. */
 /* Test hook, no counterpart in the reference: later launches make each
  * strip's halo receives wait for tags that no workgroup publishes (when
  * `unreachable` is nonzero and a replica has more than one strip), and every
