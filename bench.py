@@ -602,6 +602,8 @@ def main():
         # torch's HIP runtime must be the process's only one: load it before
         # libmcgpu binds to a runtime (the gradients line runs through torch)
         import torch  # noqa: F401
+        if args.copy_probe_mib > 0:
+            copy_probe(args.copy_probe_mib)
         print(json.dumps(bench_gradients()), flush=True)
         return
     # Python's full garbage collections traverse every live object; with
