@@ -5,8 +5,10 @@ colour kernels per sweep from Python, like the reference.  ``BatchedRun``
 executes the same schedule -- sweeps (core/runtime.py:87-99), an exchange
 every ``exchange_interval`` sweeps (core/parallel_tempering.py:245-269) and a
 sample every ``sample_interval`` sweeps (core/simulation.py:36-72) -- inside
-one launch of ``mc_batch_run``: each workgroup keeps one strip of H rows of
-one replica on chip for the whole launch.
+one persistent launch (``mc_batch_run``; a run makes a launch plan once with
+``mc_batch_plan_create`` and queues each launch with ``mc_batch_plan_launch``):
+each workgroup keeps one strip of H rows of one replica on chip for the whole
+launch.
 
 Models:
 * ``IsingModel``: the reference update; trajectories, exchanges and samples
@@ -342,8 +344,13 @@ class BatchedRun(Ensemble):
         return self.L // self.H
 
     def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing() or _lib.closing():
+            return
+        plan = getattr(self, "_plan", None)
+        if plan is not None:
+            _lib.load().mc_batch_plan_destroy(plan[1])
         s = getattr(self, "_stream", None)
-        if s is not None and not _finalizing() and not _lib.closing():
+        if s is not None:
             _lib.load().mc_stream_destroy(s)
 
     # ------------------------------------------------------------- running
@@ -360,31 +367,38 @@ class BatchedRun(Ensemble):
             sw2t = device_array((n_samples, self.R), np.int32)
         else:  # no samples: the kernel takes no writes, so one placeholder pair serves every launch
             partials, sw2t = self._no_samples
-        head, pt, tail = self._launch_args()
-        _lib.call("mc_batch_run", *head, nsweeps, sweep0, self.exchange_interval, *pt, int(sample_interval or 0),
-                  int(sample_first), n_samples, as_pointer(partials), as_pointer(sw2t), *tail)
+        _lib.check(_lib.load().mc_batch_plan_launch(self._launch_plan(), nsweeps, sweep0, int(sample_interval or 0),
+                                                    int(sample_first), n_samples, as_pointer(partials),
+                                                    as_pointer(sw2t)))
         self._advance(nsweeps)
         return (partials, sw2t, n_samples, sample_first, sample_interval)
 
-    def _launch_args(self):
-        """The arguments of mc_batch_run that do not change between launches, built once
-        (the arrays keep their buffers for the run's life; J and delta are read again)."""
+    def _launch_plan(self):
+        """The native launch plan (mc_batch_plan_create): the arguments that do not
+        change between launches, checked once.  Made again when J, delta or one of
+        the exchange interval or one of the arrays changes (the arrays keep their
+        buffers for the run's life)."""
         arrays = (self.state, self.rng_states, self.betas, self.sigmas, self._accepted, self._parity, self.energies,
                   self.walker_to_temperature, self.temperature_to_walker, self._pt_rng, self._swap_counts,
                   self._trips, self._halo, self._energy_words, self._error)
-        key = (float(self.model.J), float(getattr(self.model, "delta", 0.0)), tuple(map(id, arrays)))
-        cached = getattr(self, "_args", None)
-        if cached is None or cached[0] != key:
-            head = (self._stream, self.code, self.L, self.R, self.H, self.threads, as_pointer(self.state),
-                    as_pointer(self.rng_states), as_pointer(self.betas), key[0], key[1], as_pointer(self.sigmas),
-                    as_pointer(self._accepted))
-            pt = (as_pointer(self._parity), as_pointer(self.energies), as_pointer(self.walker_to_temperature),
+        key = (float(self.model.J), float(getattr(self.model, "delta", 0.0)), self.exchange_interval,
+               tuple(map(id, arrays)))
+        cached = getattr(self, "_plan", None)
+        if cached is not None and cached[0] == key:
+            return cached[1]
+        plan = ctypes.c_void_p()
+        _lib.call("mc_batch_plan_create", self._stream, self.code, self.L, self.R, self.H, self.threads,
+                  as_pointer(self.state), as_pointer(self.rng_states), as_pointer(self.betas), key[0], key[1],
+                  as_pointer(self.sigmas), as_pointer(self._accepted), self.exchange_interval,
+                  as_pointer(self._parity), as_pointer(self.energies), as_pointer(self.walker_to_temperature),
                   as_pointer(self.temperature_to_walker), as_pointer(self._pt_rng), as_pointer(self._swap_counts),
-                  as_pointer(self._trips))
-            tail = (as_pointer(self._halo), as_pointer(self._energy_words), as_pointer(self._error),
-                    ctypes.cast(self._clock, ctypes.c_void_p), self._error_host.pointer)
-            self._args = cached = (key, (head, pt, tail))
-        return cached[1]
+                  as_pointer(self._trips), as_pointer(self._halo), as_pointer(self._energy_words),
+                  as_pointer(self._error), ctypes.cast(self._clock, ctypes.c_void_p), self._error_host.pointer,
+                  ctypes.byref(plan))
+        if cached is not None:
+            _lib.load().mc_batch_plan_destroy(cached[1])
+        self._plan = (key, plan)
+        return plan
 
     def synchronize(self):
         if self._stream is not None:

@@ -65,6 +65,7 @@
 // Accepted moves are counted per ladder temperature when `accepted` is given.
 #include <algorithm>
 #include <mutex>
+#include <new>
 #include <vector>
 
 #include "mc_strip.h"
@@ -962,6 +963,133 @@ static int check_shape(int model, int L, int R, int H, int threads) {
     return MC_OK;
 }
 
+// The arguments of a run that stay the same from one launch to the next,
+// checked once, with the kernel, its LDS and the co-residency check resolved.
+struct BatchPlan {
+    void *stream = nullptr;
+    BatchArgs a{};  // every field but the per-launch ones
+    const void *kern = nullptr;
+    size_t lds = 0;
+    int threads = 0;
+    long long *clock = nullptr;
+    int *error_host = nullptr;
+};
+
+static int batch_prepare(BatchPlan &p, void *stream, int model, int L, int R, int H, int threads, void *state,
+                         uint64_t *rng, float *betas, float J, float delta, const float *sigmas,
+                         unsigned long long *accepted, int exchange_interval, int *parity, double *energies,
+                         int32_t *w2t, int32_t *t2w, uint64_t *pt_rng, long long *counts, long long *trips,
+                         uint64_t *halo, uint64_t *energy, int *error, long long *clock, int *error_host) {
+    int rc = check_shape(model, L, R, H, threads);
+    if (rc) return rc;
+    MC_REQUIRE(exchange_interval >= 0, "exchange_interval must be nonnegative");
+    MC_REQUIRE(exchange_interval == 0 || R >= 2, "exchanges need at least two replicas");
+    MC_REQUIRE(model != MODEL_XY_GAUSS || sigmas, "the Gaussian XY model needs per-temperature sigmas");
+    const int S = L / H;
+    const size_t lds = batch_lds(model, L, R, H, exchange_interval > 0);
+    MC_REQUIRE(lds <= 160 * 1024, "batch workgroup needs %zu bytes of LDS", lds);
+
+    BatchArgs &a = p.a;
+    a.model = model; a.L = L; a.R = R; a.H = H; a.S = S;
+    a.state = state; a.rng = rng; a.betas = betas; a.J = J; a.delta = delta;
+    a.sigmas = sigmas; a.accepted = accepted;
+    a.exchange_interval = exchange_interval; a.parity = parity; a.energies = energies;
+    a.w2t = w2t; a.t2w = t2w; a.pt_rng = pt_rng; a.counts = counts; a.trips = trips;
+    a.halo = halo; a.energy = energy; a.error = error;
+
+    const void *kern = kernel_ptr(model, threads, batch_items(L, H, threads));
+    if (lds > 64 * 1024) MC_TRY_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    // a plain dispatch places every workgroup at once only for a grid the
+    // occupancy check admits (a cooperative launch checks the same)
+    int dev = 0;
+    MC_TRY_HIP(hipGetDevice(&dev));
+    long long capacity = -1;
+    {
+        std::lock_guard<std::mutex> guard(g_capacity_lock);
+        for (const auto &e : g_capacity)
+            if (e.dev == dev && e.kern == kern && e.threads == threads && e.lds == lds) capacity = e.capacity;
+    }
+    if (capacity < 0) {
+        int per_cu = 0, cus = 0;
+        MC_TRY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds));
+        MC_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        capacity = (long long)per_cu * cus;
+        std::lock_guard<std::mutex> guard(g_capacity_lock);
+        g_capacity.push_back(Capacity{dev, kern, threads, lds, capacity});
+    }
+    MC_REQUIRE((long long)R * S <= capacity, "%d workgroups cannot be co-resident (%lld on the device at %zu "
+               "bytes of LDS)", R * S, capacity, lds);
+    p.stream = stream;
+    p.kern = kern;
+    p.lds = lds;
+    p.threads = threads;
+    p.clock = clock;
+    p.error_host = error_host;
+    return MC_OK;
+}
+
+static int batch_launch(const BatchPlan &p, int nsweeps, long long sweep0, int sample_interval,
+                        long long sample_first, int max_samples, double *sample_partials, int32_t *sample_w2t) {
+    MC_REQUIRE(nsweeps >= 0, "nsweeps must be nonnegative");
+    MC_REQUIRE(sample_interval >= 0 && max_samples >= 0, "invalid sampling arguments");
+    if (nsweeps == 0) return MC_OK;
+    BatchArgs a = p.a;
+    const int L = a.L, R = a.R, H = a.H, S = a.S, exchange_interval = a.exchange_interval;
+    a.nsweeps = nsweeps; a.sweep0 = sweep0;
+    a.sample_interval = sample_interval; a.sample_first = sample_first; a.max_samples = max_samples;
+    a.sample_partials = sample_partials; a.sample_w2t = sample_w2t;
+    // first exchange sweep: the first global sweep g >= sweep0 + 1 with
+    // g % exchange_interval == 0; first sample sweep: the first
+    // g >= max(sweep0 + 1, sample_first) with (g - sample_first) % sample_interval == 0
+    const long long g1 = sweep0 + 1;
+    a.x_next = a.s_next = nsweeps;
+    a.x_index = a.s_index = 0;
+    if (exchange_interval > 0) {
+        const long long first = g1 + (exchange_interval - g1 % exchange_interval) % exchange_interval;
+        a.x_next = std::min<long long>(first - g1, nsweeps);
+        a.x_index = first / exchange_interval;
+    }
+    if (sample_interval > 0) {
+        const long long first =
+            g1 <= sample_first ? sample_first
+                               : g1 + (sample_interval - (g1 - sample_first) % sample_interval) % sample_interval;
+        a.s_next = std::min<long long>(first - g1, nsweeps);
+        a.s_index = (first - sample_first) / sample_interval;
+    }
+    a.watchdog = g_batch_test_watchdog ? g_batch_test_watchdog : kWatchdog;
+
+    // The workspace clock: tags issued since the workspace was last zeroed.
+    // Zeroed first when there is no clock, when it reads zero (a new
+    // workspace, or one after an error) and before the count could overflow.
+    long long *clock = p.clock;
+    MC_REQUIRE((long long)nsweeps + 1 <= kMaxTag, "at most %lld sweeps per launch", kMaxTag - 1);
+    const bool reset = clock == nullptr || clock[0] <= 0 || clock[0] + (long long)nsweeps + 1 > kMaxTag;
+    const long long tag0 = reset ? 0 : clock[0];
+    a.tag0 = (unsigned)tag0;
+    a.recv_tag0 = a.tag0 + (g_batch_test_unreachable && S > 1 ? kUnreachableTag : 0u);
+
+    hipStream_t st = as_stream(p.stream);
+    if (reset) {
+        const size_t nh = halo_words(L, R, H);
+        const unsigned reset_blocks = (unsigned)std::min<size_t>((nh + 255) / 256, 1024);
+        hipLaunchKernelGGL(batch_reset_kernel, dim3(reset_blocks), dim3(256), 0, st, a.halo, nh, a.energy,
+                           energy_words(L, R, H), a.error);
+        MC_TRY_HIP(hipGetLastError());
+    }
+    void *kargs[] = {&a};
+    if (g_batch_cooperative)
+        MC_TRY_HIP(hipLaunchCooperativeKernel(p.kern, dim3((unsigned)(R * S)), dim3(p.threads), kargs, p.lds, st));
+    else
+        MC_TRY_HIP(hipLaunchKernel(p.kern, dim3((unsigned)(R * S)), dim3(p.threads), kargs, p.lds, st));
+    if (exchange_interval > 0 || p.error_host) {
+        hipLaunchKernelGGL(batch_commit_kernel, dim3(1), dim3(256), 0, st, a.energy, R, S, a.betas, a.w2t, a.t2w,
+                           a.pt_rng, a.parity, exchange_interval > 0 ? 1 : 0, a.error, p.error_host);
+        MC_TRY_HIP(hipGetLastError());
+    }
+    if (clock) clock[0] = tag0 + nsweeps + 1;  // every tag of this launch is below it
+    return MC_OK;
+}
+
 }  // namespace mc
 
 using namespace mc;
@@ -1009,102 +1137,44 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, int threads, void
                  long long *trips, int sample_interval, long long sample_first, int max_samples,
                  double *sample_partials, int32_t *sample_w2t, uint64_t *halo, uint64_t *energy,
                  int *error, long long *clock, int *error_host) {
-    int rc = check_shape(model, L, R, H, threads);
+    BatchPlan plan;
+    const int rc = batch_prepare(plan, stream, model, L, R, H, threads, state, rng, betas, J, delta, sigmas,
+                                 accepted, exchange_interval, parity, energies, w2t, t2w, pt_rng, counts, trips,
+                                 halo, energy, error, clock, error_host);
     if (rc) return rc;
-    MC_REQUIRE(nsweeps >= 0, "nsweeps must be nonnegative");
-    MC_REQUIRE(exchange_interval >= 0, "exchange_interval must be nonnegative");
-    MC_REQUIRE(exchange_interval == 0 || R >= 2, "exchanges need at least two replicas");
-    MC_REQUIRE(sample_interval >= 0 && max_samples >= 0, "invalid sampling arguments");
-    MC_REQUIRE(model != MODEL_XY_GAUSS || sigmas, "the Gaussian XY model needs per-temperature sigmas");
-    const int S = L / H;
-    const size_t lds = batch_lds(model, L, R, H, exchange_interval > 0);
-    MC_REQUIRE(lds <= 160 * 1024, "batch workgroup needs %zu bytes of LDS", lds);
-    if (nsweeps == 0) return MC_OK;
+    return batch_launch(plan, nsweeps, sweep0, sample_interval, sample_first, max_samples, sample_partials,
+                        sample_w2t);
+}
 
-    BatchArgs a;
-    a.model = model; a.L = L; a.R = R; a.H = H; a.S = S;
-    a.state = state; a.rng = rng; a.betas = betas; a.J = J; a.delta = delta;
-    a.sigmas = sigmas; a.accepted = accepted;
-    a.nsweeps = nsweeps; a.sweep0 = sweep0;
-    a.exchange_interval = exchange_interval; a.parity = parity; a.energies = energies;
-    a.w2t = w2t; a.t2w = t2w; a.pt_rng = pt_rng; a.counts = counts; a.trips = trips;
-    a.sample_interval = sample_interval; a.sample_first = sample_first; a.max_samples = max_samples;
-    a.sample_partials = sample_partials; a.sample_w2t = sample_w2t;
-    // first exchange sweep: the first global sweep g >= sweep0 + 1 with
-    // g % exchange_interval == 0; first sample sweep: the first
-    // g >= max(sweep0 + 1, sample_first) with (g - sample_first) % sample_interval == 0
-    const long long g1 = sweep0 + 1;
-    a.x_next = a.s_next = nsweeps;
-    a.x_index = a.s_index = 0;
-    if (exchange_interval > 0) {
-        const long long first = g1 + (exchange_interval - g1 % exchange_interval) % exchange_interval;
-        a.x_next = std::min<long long>(first - g1, nsweeps);
-        a.x_index = first / exchange_interval;
+int mc_batch_plan_create(void *stream, int model, int L, int R, int H, int threads, void *state, uint64_t *rng,
+                         float *betas, float J, float delta, const float *sigmas, unsigned long long *accepted,
+                         int exchange_interval, int *parity, double *energies, int32_t *w2t, int32_t *t2w,
+                         uint64_t *pt_rng, long long *counts, long long *trips, uint64_t *halo, uint64_t *energy,
+                         int *error, long long *clock, int *error_host, void **plan) {
+    MC_REQUIRE(plan, "plan must not be NULL");
+    *plan = nullptr;
+    BatchPlan *p = new (std::nothrow) BatchPlan;
+    MC_REQUIRE(p, "out of host memory");
+    const int rc = batch_prepare(*p, stream, model, L, R, H, threads, state, rng, betas, J, delta, sigmas, accepted,
+                                 exchange_interval, parity, energies, w2t, t2w, pt_rng, counts, trips, halo, energy,
+                                 error, clock, error_host);
+    if (rc) {
+        delete p;
+        return rc;
     }
-    if (sample_interval > 0) {
-        const long long first =
-            g1 <= sample_first ? sample_first
-                               : g1 + (sample_interval - (g1 - sample_first) % sample_interval) % sample_interval;
-        a.s_next = std::min<long long>(first - g1, nsweeps);
-        a.s_index = (first - sample_first) / sample_interval;
-    }
-    a.halo = halo; a.energy = energy; a.error = error;
-    a.watchdog = g_batch_test_watchdog ? g_batch_test_watchdog : kWatchdog;
+    *plan = p;
+    return MC_OK;
+}
 
-    // The workspace clock: tags issued since the workspace was last zeroed.
-    // Zeroed first when there is no clock, when it reads zero (a new
-    // workspace, or one after an error) and before the count could overflow.
-    MC_REQUIRE((long long)nsweeps + 1 <= kMaxTag, "at most %lld sweeps per launch", kMaxTag - 1);
-    const bool reset = clock == nullptr || clock[0] <= 0 || clock[0] + (long long)nsweeps + 1 > kMaxTag;
-    const long long tag0 = reset ? 0 : clock[0];
-    a.tag0 = (unsigned)tag0;
-    a.recv_tag0 = a.tag0 + (g_batch_test_unreachable && S > 1 ? kUnreachableTag : 0u);
+int mc_batch_plan_launch(void *plan, int nsweeps, long long sweep0, int sample_interval, long long sample_first,
+                         int max_samples, double *sample_partials, int32_t *sample_w2t) {
+    MC_REQUIRE(plan, "plan must not be NULL");
+    return batch_launch(*static_cast<const BatchPlan *>(plan), nsweeps, sweep0, sample_interval, sample_first,
+                        max_samples, sample_partials, sample_w2t);
+}
 
-    const void *kern = kernel_ptr(model, threads, batch_items(L, H, threads));
-    if (lds > 64 * 1024)
-        MC_TRY_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    if (reset) {
-        const size_t nh = halo_words(L, R, H);
-        const unsigned reset_blocks = (unsigned)std::min<size_t>((nh + 255) / 256, 1024);
-        hipLaunchKernelGGL(batch_reset_kernel, dim3(reset_blocks), dim3(256), 0, as_stream(stream), halo, nh,
-                           energy, energy_words(L, R, H), error);
-        MC_TRY_HIP(hipGetLastError());
-    }
-    void *kargs[] = {&a};
-    if (g_batch_cooperative) {
-        MC_TRY_HIP(hipLaunchCooperativeKernel(kern, dim3((unsigned)(R * S)), dim3(threads), kargs, lds,
-                                              as_stream(stream)));
-    } else {
-        // a plain dispatch of a grid the occupancy check admits: every
-        // workgroup is placed at once, as for the packed engine.  The check's
-        // answer is kept per (device, kernel, workgroup size, LDS), so a
-        // launch costs no runtime queries after the first.
-        int dev = 0;
-        MC_TRY_HIP(hipGetDevice(&dev));
-        long long capacity = -1;
-        {
-            std::lock_guard<std::mutex> guard(g_capacity_lock);
-            for (const auto &e : g_capacity)
-                if (e.dev == dev && e.kern == kern && e.threads == threads && e.lds == lds) capacity = e.capacity;
-        }
-        if (capacity < 0) {
-            int per_cu = 0, cus = 0;
-            MC_TRY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds));
-            MC_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-            capacity = (long long)per_cu * cus;
-            std::lock_guard<std::mutex> guard(g_capacity_lock);
-            g_capacity.push_back(Capacity{dev, kern, threads, lds, capacity});
-        }
-        MC_REQUIRE((long long)R * S <= capacity, "%d workgroups cannot be co-resident (%lld on the device at %zu "
-                   "bytes of LDS)", R * S, capacity, lds);
-        MC_TRY_HIP(hipLaunchKernel(kern, dim3((unsigned)(R * S)), dim3(threads), kargs, lds, as_stream(stream)));
-    }
-    if (exchange_interval > 0 || error_host) {
-        hipLaunchKernelGGL(batch_commit_kernel, dim3(1), dim3(256), 0, as_stream(stream), energy, R, S, betas, w2t,
-                           t2w, pt_rng, parity, exchange_interval > 0 ? 1 : 0, error, error_host);
-        MC_TRY_HIP(hipGetLastError());
-    }
-    if (clock) clock[0] = tag0 + nsweeps + 1;  // every tag of this launch is below it
+int mc_batch_plan_destroy(void *plan) {
+    delete static_cast<BatchPlan *>(plan);
     return MC_OK;
 }
 

@@ -104,6 +104,10 @@ SIGNATURES = {
     "mc_batch_set_cooperative": [INT],
     "mc_batch_run": [P, INT, INT, INT, INT, INT, P, P, P, F32, F32, P, P, INT, I64, INT, P, P, P, P, P, P, P,
                      INT, I64, INT, P, P, P, P, P, P, P],
+    "mc_batch_plan_create": [P, INT, INT, INT, INT, INT, P, P, P, F32, F32, P, P, INT, P, P, P, P, P, P, P, P, P,
+                             P, P, P, ctypes.POINTER(P)],
+    "mc_batch_plan_launch": [P, INT, I64, INT, I64, INT, P, P],
+    "mc_batch_plan_destroy": [P],
     "mc_pack_lds_bytes": [INT, INT, INT, INT, INT, ctypes.POINTER(SZ)],
     "mc_pack_occupancy": [INT, SZ, ctypes.POINTER(INT)],
     "mc_pack_create": [INT, ctypes.POINTER(P)],

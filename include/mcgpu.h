@@ -307,6 +307,27 @@ int mc_batch_run(void *stream, int model, int L, int R, int H, int threads, void
  * error[0] from the small kernel queued on the stream after the launch (a
  * device store into the pinned word): read it after synchronising the
  * stream. */
+/* The same launch in two calls, no single counterpart in the reference (the
+ * loop of core/simulation.py:36-72 calls its runtime once per segment):
+ * mc_batch_plan_create checks the arguments of mc_batch_run that stay the
+ * same from one launch of a run to the next and resolves the kernel, its LDS
+ * and the co-residency check once; mc_batch_plan_launch then queues
+ * nsweeps sweeps exactly as mc_batch_run with the same arguments would, from
+ * eight arguments.  The plan keeps the pointers it was given (the caller's
+ * arrays, workspace, clock and error_host must outlive it); J and delta are
+ * fixed at creation.  A plan is not thread-safe; destroy it with
+ * mc_batch_plan_destroy. */
+int mc_batch_plan_create(void *stream, int model, int L, int R, int H, int threads, void *state,
+                         uint64_t *rng, float *betas, float J, float delta, const float *sigmas,
+                         unsigned long long *accepted, int exchange_interval, int *parity,
+                         double *energies, int32_t *walker_to_temperature,
+                         int32_t *temperature_to_walker, uint64_t *pt_rng, long long *counts,
+                         long long *trips, uint64_t *halo, uint64_t *energy, int *error,
+                         long long *clock, int *error_host, void **plan);
+int mc_batch_plan_launch(void *plan, int nsweeps, long long sweep0, int sample_interval,
+                         long long sample_first, int max_samples, double *sample_partials,
+                         int32_t *sample_w2t);
+int mc_batch_plan_destroy(void *plan);
 /* Test hook, no counterpart in the reference: later launches make each
  * strip's halo receives wait for tags that no workgroup publishes (when
  * `unreachable` is nonzero and a replica has more than one strip), and every

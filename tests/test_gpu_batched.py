@@ -101,6 +101,47 @@ def test_launches_reuse_the_workspace_without_clearing_it(gpu):
     assert kept.diagnostics()["attempts"].sum() > 0
 
 
+def test_single_call_entry_point_equals_the_launch_plan(gpu):
+    """mc_batch_run (every argument in one call) continues a chain exactly as
+    the launch plan that BatchedRun uses (mc_batch_plan_create, then
+    mc_batch_plan_launch per launch), exchanges included."""
+    import ctypes
+    from core.batched import BatchedRun
+    from mcgpu import _lib
+    from mcgpu.device import as_pointer
+    from models.xy.model import XYModel
+    temps = np.linspace(0.8, 1.1, 6)
+
+    def make():
+        model = XYModel(J=1.0, proposal="gaussian", sigma=0.9, start="random")
+        return BatchedRun(model, 32, temps, seed=21, exchange_interval=3, pt_seed=8, strip_rows=8, threads=256)
+
+    planned, direct = make(), make()
+    for n in (4, 9, 13):
+        planned.run(n)
+        d = direct
+        partials, sw2t = d._no_samples
+        _lib.call("mc_batch_run", d._stream, d.code, d.L, d.R, d.H, d.threads, as_pointer(d.state),
+                  as_pointer(d.rng_states), as_pointer(d.betas), float(d.model.J), 0.0, as_pointer(d.sigmas),
+                  as_pointer(d._accepted), n, d.sweeps, d.exchange_interval, as_pointer(d._parity),
+                  as_pointer(d.energies), as_pointer(d.walker_to_temperature), as_pointer(d.temperature_to_walker),
+                  as_pointer(d._pt_rng), as_pointer(d._swap_counts), as_pointer(d._trips), 0, 0, 0,
+                  as_pointer(partials), as_pointer(sw2t), as_pointer(d._halo), as_pointer(d._energy_words),
+                  as_pointer(d._error), ctypes.cast(d._clock, ctypes.c_void_p), d._error_host.pointer)
+        d._advance(n)
+        d.synchronize()
+    assert getattr(direct, "_plan", None) is None  # the direct run never made a plan
+    np.testing.assert_array_equal(planned.snapshot(), direct.snapshot())
+    np.testing.assert_array_equal(device_rng_as_pairs(planned.rng_states), device_rng_as_pairs(direct.rng_states))
+    np.testing.assert_array_equal(planned.betas.copy_to_host(), direct.betas.copy_to_host())
+    np.testing.assert_array_equal(planned.energies.copy_to_host(), direct.energies.copy_to_host())
+    d_p, d_d = planned.diagnostics(), direct.diagnostics()
+    for key in ("attempts", "accepted", "walker_to_temperature", "temperature_to_walker", "walker_round_trips"):
+        np.testing.assert_array_equal(d_p[key], d_d[key], err_msg=key)
+    assert d_p["next_parity"] == d_d["next_parity"]
+    assert d_p["attempts"].sum() > 0
+
+
 def test_batched_ising_simulation_samples_equal_drop_in_simulation(gpu):
     from core.batched import BatchedRun, BatchedSimulation
     from core.simulation import Simulation
