@@ -29,20 +29,21 @@ namespace mc {
 
 template <class R>
 __device__ __forceinline__ void sincos_r(R x, R &s, R &c);
-// float32: whole turns removed first (2 pi in two parts, so the remainder is
-// in [-pi, pi] up to the rounding of x itself), then the hardware sine and
-// cosine (v_sin_f32, v_cos_f32, arguments in turns).  Measured on MI355X over
-// 2^24 arguments up to |x| = 1000: at most 3.7e-7 from the float64 values,
-// mean error of the cosine -4e-8.  These kernels' float32 results are checked
-// against the float64 ones, not bit for bit (the float64 kernels call the
-// library sincos); the polynomial sine and cosine of mc_fastmath.h took about
-// 30 instructions per site here, the hardware pair about 10.
+// float32: the argument in turns less its nearest whole number of turns,
+// t = x / (2 pi) - n with 1 / (2 pi) in two parts (one fused multiply-add
+// each, so t keeps the bits that x * c_hi - n cancels), then the hardware
+// sine and cosine (v_sin_f32, v_cos_f32, arguments in turns).  These
+// kernels' float32 results are checked against the float64 ones, not bit for
+// bit (the float64 kernels call the library sincos); the polynomial sine and
+// cosine of mc_fastmath.h took about 30 instructions per site here, this pair
+// 6.
 template <>
 __device__ __forceinline__ void sincos_r<float>(float x, float &s, float &c) {
-    const float n = rintf(x * 0x1.45f306p-3f);      // x / (2 pi)
-    float r = fmaf(-n, 6.28318548202514648f, x);   // float32(2 pi)
-    r = fmaf(-n, -1.74845553e-07f, r);             // 2 pi - float32(2 pi)
-    const float t = r * 0x1.45f306p-3f;            // turns, in [-1/2, 1/2]
+    constexpr float kTurnHi = 0x1.45f306p-3f;  // float32(1 / (2 pi))
+    constexpr float kTurnLo = 0x1.b9391p-28f;  // 1 / (2 pi) - kTurnHi
+    const float n = rintf(x * kTurnHi);
+    float t = fmaf(x, kTurnHi, -n);
+    t = fmaf(x, kTurnLo, t);                    // turns, in [-1/2, 1/2] up to rounding
     s = __builtin_amdgcn_sinf(t);
     c = __builtin_amdgcn_cosf(t);
 }
@@ -51,7 +52,13 @@ __device__ __forceinline__ void sincos_r<double>(double x, double &s, double &c)
     ::sincos(x, &s, &c);
 }
 
-__device__ __forceinline__ double twist_of(const double *twist, int r) { return twist ? twist[r] : 0.0; }
+// Per-replica values of an optional array (NULL: 0).  The load is made from a
+// selected address (the array's entry or a zero in device memory) rather than
+// behind a branch: a branch with its own wait before the kernels' first vector
+// loads delayed them by a memory round trip per optional array.
+__device__ const double kNoValue[1] = {0.0};
+__device__ __forceinline__ double replica_value(const double *p, int r) { return *(p ? p + r : kNoValue); }
+__device__ __forceinline__ double twist_of(const double *twist, int r) { return replica_value(twist, r); }
 
 constexpr int kTileRows = 16, kTileCols = 32, kTileThreads = 256;
 constexpr int kHaloRows = kTileRows + 2, kHaloCols = kTileCols + 2;
@@ -60,9 +67,9 @@ constexpr int kHaloRows = kTileRows + 2, kHaloCols = kTileCols + 2;
 struct Coefs {
     const double *alpha, *beta, *gamma;
     __device__ __forceinline__ void at(int r, double &a, double &b, double &g) const {
-        a = alpha ? alpha[r] : 0.0;
-        b = beta ? beta[r] : 0.0;
-        g = gamma ? gamma[r] : 0.0;
+        a = replica_value(alpha, r);
+        b = replica_value(beta, r);
+        g = replica_value(gamma, r);
     }
 };
 
@@ -401,51 +408,94 @@ __device__ __forceinline__ R shfl_dn1(R x) { return wave_shift<0x130>(x); }
 template <class R>
 __device__ __forceinline__ R shfl_up1(R x) { return wave_shift<0x138>(x); }
 
+// acc + x[lane + 1] * x[lane], and acc unchanged in lane 63 (the shift
+// brings 0 there): one v_fmac_f32 whose first operand is read through a DPP
+// wave shift (wave_shl:1).  The compiler does not fold a separate DPP move
+// into v_fmac_f32, so it is written here; s_nop 1 gives the DPP read of x
+// the two wait states it needs after the VALU instruction that wrote x.
+__device__ __forceinline__ float fmac_right(float acc, float x) {
+    asm("s_nop 1\n\tv_fmac_f32_dpp %0, %1, %1 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+        : "+v"(acc)
+        : "v"(x));
+    return acc;
+}
+__device__ __forceinline__ double fmac_right(double acc, double x) { return fma(shfl_dn1(x), x, acc); }
+__device__ __forceinline__ float fma_r(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double fma_r(double a, double b, double c) { return fma(a, b, c); }
+
+// Terms: each thread sums its column segment's bonds with fused
+// multiply-adds; a row bond whose right end is in the next wavefront (lane
+// 63's) is left out of the row loop (the wave shift brings 0), and after one
+// barrier wavefront 0 adds those W * kSegRows bonds from the edge columns
+// that lanes 0 and 63 left in LDS.  Every wavefront writes its own partial
+// (no second barrier), so a replica has (L / kSegRows) * W partials.
 template <class R, int NT>
-__global__ __launch_bounds__(NT) void xy_terms_rows_kernel(const R *__restrict__ th, int nseg, const double *twist,
+__global__ __launch_bounds__(NT) void xy_terms_rows_kernel(const R *__restrict__ th, const double *twist,
                                                           double *__restrict__ partials) {
     constexpr int L = NT, W = NT / 64;
-    __shared__ R edge_c[W][kSegRows], edge_s[W][kSegRows];  // lane 0 of each wavefront
+    __shared__ R edge[2][2][W][kSegRows];  // [lane 0, lane 63][cos, sin][wavefront][row]
     const int r = blockIdx.y, j = threadIdx.x, w = j >> 6, lane = j & 63;
     const int i0 = blockIdx.x * kSegRows;
     const R *b = th + (size_t)r * L * L + j;
     R v[kSegRows + 1], c[kSegRows + 1], sn[kSegRows + 1];
 #pragma unroll
-    for (int k = 0; k <= kSegRows; k++) v[k] = b[(size_t)((i0 + k) % L) * L];
-#pragma unroll
-    for (int k = 0; k <= kSegRows; k++) sincos_r<R>(v[k], sn[k], c[k]);
-    if (lane == 0)
-#pragma unroll
-        for (int k = 0; k < kSegRows; k++) {
-            edge_c[w][k] = c[k];
-            edge_s[w][k] = sn[k];
-        }
+    for (int k = 0; k <= kSegRows; k++) v[k] = b[(size_t)((i0 + k) & (L - 1)) * L];
     R cd, sd;
     twist_cs<R>(twist, r, cd, sd);
-    __syncthreads();
-    const int wn = (w + 1) % W;
+#pragma unroll
+    for (int k = 0; k <= kSegRows; k++) sincos_r<R>(v[k], sn[k], c[k]);
     R ca = 0, sa = 0, cy = 0;
 #pragma unroll
     for (int k = 0; k < kSegRows; k++) {
-        R cr = shfl_dn1(c[k]), sr = shfl_dn1(sn[k]);
-        if (lane == 63) {
-            cr = edge_c[wn][k];
-            sr = edge_s[wn][k];
-        }
-        ca += c[k] * c[k + 1] + sn[k] * sn[k + 1];  // x bond (i, j) - (i+1, j): cos a
-        sa += sn[k] * c[k + 1] - c[k] * sn[k + 1];  //                          sin a
-        cy += bond_c<R>(c[k], sn[k], cr, sr);       // y bond (i, j) - (i, j+1)
+        ca = fma_r(c[k], c[k + 1], fma_r(sn[k], sn[k + 1], ca));   // x bond (i, j) - (i+1, j): cos a
+        sa = fma_r(sn[k], c[k + 1], fma_r(-c[k], sn[k + 1], sa));  //                          sin a
+        cy = fmac_right(fmac_right(cy, c[k]), sn[k]);              // y bond (i, j) - (i, j+1)
     }
+    if (lane == 0 || lane == 63) {
+        const int e = lane == 0 ? 0 : 1;
+#pragma unroll
+        for (int k = 0; k < kSegRows; k++) {
+            edge[e][0][w][k] = c[k];
+            edge[e][1][w][k] = sn[k];
+        }
+    }
+    __syncthreads();
+    if (w == 0)
+        for (int q = lane; q < W * kSegRows; q += 64) {
+            const int ww = q / kSegRows, k = q - ww * kSegRows, wn = (ww + 1) % W;
+            cy = fma_r(edge[1][0][ww][k], edge[0][0][wn][k], cy);
+            cy = fma_r(edge[1][1][ww][k], edge[0][1][wn][k], cy);
+        }
     // this thread's x bonds rotated by -Delta as sums
     const double vv[3] = {(double)(ca * cd + sa * sd), (double)cy, (double)(sa * cd - ca * sd)};
-    block_partial<3, NT>(vv, partials + ((size_t)r * nseg + blockIdx.x) * 3);
+    double *o = partials + (((size_t)r * gridDim.x + blockIdx.x) * W + w) * 3;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const double t = wave_sum(vv[q]);
+        if (lane == 0) o[q] = t;
+    }
 }
 
+// The twisted x bond p - q: (cos(a - D), sin(a - D)) with fused
+// multiply-adds (the whole-row kernels' rounding, not the tile kernels').
+template <class R>
+__device__ __forceinline__ void bond_x_fma(R cp, R sp, R cq, R sq, R cd, R sd, R &c, R &s) {
+    const R ca = fma_r(cp, cq, sp * sq), sa = fma_r(sp, cq, -(cp * sq));
+    c = fma_r(ca, cd, sa * sd);
+    s = fma_r(sa, cd, -(ca * sd));
+}
+
+// Gradient: every bond is evaluated once.  The x bond below row k is the
+// site's down bond (s1, c1) and, carried to row k + 1, that site's up bond
+// (s2, c2); the y bond to the right is the site's s3, and the left
+// neighbour's s3, one lane over, is its s4.  A wavefront's lane 63 takes its
+// right neighbour, and lane 0 its left neighbour's y bond, from the edge
+// columns in LDS.
 template <class R, int NT>
 __global__ __launch_bounds__(NT) void xy_grad_rows_kernel(const R *__restrict__ th, const double *twist, Coefs kc,
                                                          R *__restrict__ grad) {
     constexpr int L = NT, W = NT / 64;
-    __shared__ R edge_c[2][W][kSegRows], edge_s[2][W][kSegRows];  // lanes 0 and 63 of each wavefront
+    __shared__ R edge[2][2][W][kSegRows];  // [lane 0, lane 63][cos, sin][wavefront][row]
     const int r = blockIdx.y, j = threadIdx.x, w = j >> 6, lane = j & 63;
     const int i0 = blockIdx.x * kSegRows;
     const size_t base = (size_t)r * L * L;
@@ -453,54 +503,235 @@ __global__ __launch_bounds__(NT) void xy_grad_rows_kernel(const R *__restrict__ 
     // rows i0 - 1 .. i0 + kSegRows (index k + 1 holds row i0 + k)
     R v[kSegRows + 2], c[kSegRows + 2], sn[kSegRows + 2];
 #pragma unroll
-    for (int k = 0; k < kSegRows + 2; k++) v[k] = b[(size_t)((i0 + k - 1 + L) % L) * L];
+    for (int k = 0; k < kSegRows + 2; k++) v[k] = b[(size_t)((i0 + k - 1) & (L - 1)) * L];
+    R cd, sd;
+    twist_cs<R>(twist, r, cd, sd);
+    double al, be, ga;
+    kc.at(r, al, be, ga);
 #pragma unroll
     for (int k = 0; k < kSegRows + 2; k++) sincos_r<R>(v[k], sn[k], c[k]);
     if (lane == 0 || lane == 63) {
         const int e = lane == 0 ? 0 : 1;
 #pragma unroll
         for (int k = 0; k < kSegRows; k++) {
-            edge_c[e][w][k] = c[k + 1];
-            edge_s[e][w][k] = sn[k + 1];
+            edge[e][0][w][k] = c[k + 1];
+            edge[e][1][w][k] = sn[k + 1];
         }
     }
-    R cd, sd;
-    twist_cs<R>(twist, r, cd, sd);
-    double al, be, ga;
-    kc.at(r, al, be, ga);
     __syncthreads();
     const int wn = (w + 1) % W, wp = (w + W - 1) % W;
+    const R a = (R)al, bb = (R)be, g_ = (R)ga;
+    R c2, s2;  // the up bond (i-1, j) - (i, j)
+    bond_x_fma<R>(c[0], sn[0], c[1], sn[1], cd, sd, c2, s2);
 #pragma unroll
     for (int k = 1; k <= kSegRows; k++) {
-        R cr = shfl_dn1(c[k]), sr = shfl_dn1(sn[k]), cl = shfl_up1(c[k]), sl = shfl_up1(sn[k]);
-        if (lane == 63) {
-            cr = edge_c[0][wn][k - 1];
-            sr = edge_s[0][wn][k - 1];
-        }
-        if (lane == 0) {
-            cl = edge_c[1][wp][k - 1];
-            sl = edge_s[1][wp][k - 1];
-        }
         const R c0 = c[k], s0 = sn[k];
-        R c1, s1, c2, s2, c3, s3, c4, s4;
-        bond_cs<R>(c0, s0, c[k + 1], sn[k + 1], cd, sd, c1, s1);  // (i,j)-(i+1,j), site first
-        bond_cs<R>(c[k - 1], sn[k - 1], c0, s0, cd, sd, c2, s2);  // (i-1,j)-(i,j), site second
-        bond_cs<R>(c0, s0, cr, sr, (R)1, (R)0, c3, s3);           // (i,j)-(i,j+1)
-        bond_cs<R>(cl, sl, c0, s0, (R)1, (R)0, c4, s4);           // (i,j-1)-(i,j)
-        // dCx/dt = -s1 + s2, dCy/dt = -s3 + s4, dSx/dt = c1 - c2 (as xy_grad_kernel)
-        const R g = (R)al * (s2 - s1) + (R)be * (s4 - s3) + (R)ga * (c1 - c2);
+        R c1, s1;  // the down bond (i, j) - (i+1, j)
+        bond_x_fma<R>(c0, s0, c[k + 1], sn[k + 1], cd, sd, c1, s1);
+        R cr = shfl_dn1(c0), sr = shfl_dn1(s0);
+        if (lane == 63) {
+            cr = edge[0][0][wn][k - 1];
+            sr = edge[0][1][wn][k - 1];
+        }
+        const R s3 = fma_r(s0, cr, -(c0 * sr));  // (i, j) - (i, j+1)
+        R s4 = shfl_up1(s3);                     // (i, j-1) - (i, j)
+        if (lane == 0) s4 = fma_r(edge[1][1][wp][k - 1], c0, -(edge[1][0][wp][k - 1] * s0));
+        // dCx/dt = -s1 + s2, dCy/dt = -s3 + s4, dSx/dt = c1 - c2
+        const R g = fma_r(a, s2 - s1, fma_r(bb, s4 - s3, g_ * (c1 - c2)));
         grad[base + (size_t)(i0 + k - 1) * L + j] = g;
+        c2 = c1;
+        s2 = s1;
     }
+}
+
+// ------------------------------------ XY float32, one wavefront per row band
+//
+// For L = 64, 128, 256 a wavefront covers whole rows: lane l holds the M =
+// L / 64 adjacent columns M l .. M l + M - 1 (one float, float2 or float4
+// load per row) of a band of kSegRows rows.  A site's right neighbour is in
+// the same lane, or (last column) lane l + 1's first column by a DPP wave
+// rotate, which also closes the periodic row (lane 63 takes lane 0's): no
+// LDS, no barrier, no edge fix-ups, and one wavefront per workgroup.  Each
+// bond is evaluated once: the x bond below row k is the site's down bond and
+// the next row's up bond; the y bond to the right is the site's s3 and its
+// right neighbour's s4.
+constexpr int kBandRows = 16;
+
+template <int CTRL>
+__device__ __forceinline__ float wave_rotate(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
+}
+// lane i takes lane i + 1's value, lane 63 lane 0's (wave_rol:1)
+__device__ __forceinline__ float rot_next(float x) { return wave_rotate<0x134>(x); }
+// lane i takes lane i - 1's value, lane 0 lane 63's (wave_ror:1)
+__device__ __forceinline__ float rot_prev(float x) { return wave_rotate<0x13C>(x); }
+
+template <int M>
+__device__ __forceinline__ void load_cols(const float *p, float (&x)[M]) {
+    if constexpr (M == 4) {
+        const float4 v = *reinterpret_cast<const float4 *>(p);
+        x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+    } else if constexpr (M == 2) {
+        const float2 v = *reinterpret_cast<const float2 *>(p);
+        x[0] = v.x; x[1] = v.y;
+    } else {
+        x[0] = *p;
+    }
+}
+template <int M>
+__device__ __forceinline__ void store_cols(float *p, const float (&x)[M]) {
+    if constexpr (M == 4)
+        *reinterpret_cast<float4 *>(p) = make_float4(x[0], x[1], x[2], x[3]);
+    else if constexpr (M == 2)
+        *reinterpret_cast<float2 *>(p) = make_float2(x[0], x[1]);
+    else
+        *p = x[0];
+}
+
+// Terms: per-wavefront partials (Cx, Cy, Sx), kBandRows rows each.
+template <int M>
+__global__ __launch_bounds__(64) void xy_terms_band_kernel(const float *__restrict__ th, const double *twist,
+                                                          double *__restrict__ partials) {
+    constexpr int L = 64 * M;
+    const int r = blockIdx.y, lane = threadIdx.x;
+    const int i0 = blockIdx.x * kBandRows;
+    const float *b = th + (size_t)r * L * L + lane * M;
+    float v[kBandRows + 1][M];
+#pragma unroll
+    for (int k = 0; k <= kBandRows; k++) load_cols<M>(b + (size_t)((i0 + k) & (L - 1)) * L, v[k]);
+    float cd, sd;
+    twist_cs<float>(twist, r, cd, sd);
+    float cc[M], sc[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) sincos_r<float>(v[0][m], sc[m], cc[m]);
+    // one set of sums per column: with one wavefront per SIMD nothing hides the
+    // latency of a dependent chain, and M chains of 2 kBandRows terms are
+    // shorter than one of 2 M kBandRows
+    float ca[M] = {}, sa[M] = {}, cy[M] = {};
+#pragma unroll
+    for (int k = 0; k < kBandRows; k++) {
+        float cn[M], sn[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) sincos_r<float>(v[k + 1][m], sn[m], cn[m]);
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            ca[m] = fmaf(cc[m], cn[m], fmaf(sc[m], sn[m], ca[m]));   // x bond (i, j) - (i+1, j): cos a
+            sa[m] = fmaf(sc[m], cn[m], fmaf(-cc[m], sn[m], sa[m]));  //                          sin a
+            const float cr = m < M - 1 ? cc[m + 1] : rot_next(cc[0]);
+            const float sr = m < M - 1 ? sc[m + 1] : rot_next(sc[0]);
+            cy[m] = fmaf(cc[m], cr, fmaf(sc[m], sr, cy[m]));          // y bond (i, j) - (i, j+1)
+        }
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            cc[m] = cn[m];
+            sc[m] = sn[m];
+        }
+    }
+    float ca_t = ca[0], sa_t = sa[0], cy_t = cy[0];
+#pragma unroll
+    for (int m = 1; m < M; m++) {
+        ca_t += ca[m];
+        sa_t += sa[m];
+        cy_t += cy[m];
+    }
+    // this lane's x bonds rotated by -Delta as sums
+    const double f[3] = {(double)(ca_t * cd + sa_t * sd), (double)cy_t, (double)(sa_t * cd - ca_t * sd)};
+    double *o = partials + ((size_t)r * gridDim.x + blockIdx.x) * 3;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const double t = wave_sum(f[q]);
+        if (lane == 0) o[q] = t;
+    }
+}
+
+// Gradient of alpha Cx + beta Cy + gamma Sx, kBandRows rows per wavefront.
+template <int M>
+__global__ __launch_bounds__(64) void xy_grad_band_kernel(const float *__restrict__ th, const double *twist,
+                                                         Coefs kc, float *__restrict__ grad) {
+    constexpr int L = 64 * M;
+    const int r = blockIdx.y, lane = threadIdx.x;
+    const int i0 = blockIdx.x * kBandRows;
+    const size_t base = (size_t)r * L * L;
+    const float *b = th + base + lane * M;
+    // rows i0 - 1 .. i0 + kBandRows (index k + 1 holds row i0 + k)
+    float v[kBandRows + 2][M];
+#pragma unroll
+    for (int k = 0; k < kBandRows + 2; k++) load_cols<M>(b + (size_t)((i0 + k - 1) & (L - 1)) * L, v[k]);
+    float cd, sd;
+    twist_cs<float>(twist, r, cd, sd);
+    double al, be, ga;
+    kc.at(r, al, be, ga);
+    const float a = (float)al, bb = (float)be, gg = (float)ga;
+    float cc[M], sc[M], c2[M], s2[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        float cu, su;  // row i0 - 1
+        sincos_r<float>(v[0][m], su, cu);
+        sincos_r<float>(v[1][m], sc[m], cc[m]);
+        bond_x_fma<float>(cu, su, cc[m], sc[m], cd, sd, c2[m], s2[m]);  // up bond (i-1, j) - (i, j)
+    }
+#pragma unroll
+    for (int k = 1; k <= kBandRows; k++) {
+        float cn[M], sn[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) sincos_r<float>(v[k + 1][m], sn[m], cn[m]);
+        float s3[M];  // y bond (i, j) - (i, j+1)
+#pragma unroll
+        for (int m = 0; m < M - 1; m++) s3[m] = fmaf(sc[m], cc[m + 1], -(cc[m] * sc[m + 1]));
+        s3[M - 1] = fmaf(sc[M - 1], rot_next(cc[0]), -(cc[M - 1] * rot_next(sc[0])));
+        const float s4_first = rot_prev(s3[M - 1]);  // (i, j-1) - (i, j) of the lane's first column
+        float g[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            float c1, s1;  // down bond (i, j) - (i+1, j)
+            bond_x_fma<float>(cc[m], sc[m], cn[m], sn[m], cd, sd, c1, s1);
+            const float s4 = m == 0 ? s4_first : s3[m - 1];
+            // dCx/dt = -s1 + s2, dCy/dt = -s3 + s4, dSx/dt = c1 - c2
+            g[m] = fmaf(a, s2[m] - s1, fmaf(bb, s4 - s3[m], gg * (c1 - c2[m])));
+            c2[m] = c1;
+            s2[m] = s1;
+        }
+        store_cols<M>(grad + base + (size_t)(i0 + k - 1) * L + lane * M, g);
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            cc[m] = cn[m];
+            sc[m] = sn[m];
+        }
+    }
+}
+
+// float32 lattices of L = 64, 128, 256 take the band kernels
+static bool band_path(int L) { return L == 64 || L == 128 || L == 256; }
+
+static void terms_band(void *stream, const float *th, int R_, int L, const double *twist, double *partials) {
+    const dim3 grid((unsigned)(L / kBandRows), (unsigned)R_);
+    if (L == 64)
+        hipLaunchKernelGGL(xy_terms_band_kernel<1>, grid, dim3(64), 0, as_stream(stream), th, twist, partials);
+    else if (L == 128)
+        hipLaunchKernelGGL(xy_terms_band_kernel<2>, grid, dim3(64), 0, as_stream(stream), th, twist, partials);
+    else
+        hipLaunchKernelGGL(xy_terms_band_kernel<4>, grid, dim3(64), 0, as_stream(stream), th, twist, partials);
+}
+static void grad_band(void *stream, const float *th, int R_, int L, const double *twist, Coefs k, float *grad) {
+    const dim3 grid((unsigned)(L / kBandRows), (unsigned)R_);
+    if (L == 64)
+        hipLaunchKernelGGL(xy_grad_band_kernel<1>, grid, dim3(64), 0, as_stream(stream), th, twist, k, grad);
+    else if (L == 128)
+        hipLaunchKernelGGL(xy_grad_band_kernel<2>, grid, dim3(64), 0, as_stream(stream), th, twist, k, grad);
+    else
+        hipLaunchKernelGGL(xy_grad_band_kernel<4>, grid, dim3(64), 0, as_stream(stream), th, twist, k, grad);
 }
 
 // Whole-row kernels take L = 64, 128, ..., 1024 (threads = L)
 static bool rows_path(int L) { return L >= 64 && L <= 1024 && L % 64 == 0 && (L & (L - 1)) == 0; }
 
+// Partials per replica of the whole-row terms kernel: one per wavefront
+static int terms_rows_partials(int L) { return (L / kSegRows) * (L / 64); }
+
 template <class R, int NT>
 static void launch_terms_rows(void *stream, const R *th, int R_, const double *twist, double *partials) {
-    const int nseg = NT / kSegRows;
-    hipLaunchKernelGGL((xy_terms_rows_kernel<R, NT>), dim3((unsigned)nseg, (unsigned)R_), dim3(NT), 0,
-                       as_stream(stream), th, nseg, twist, partials);
+    hipLaunchKernelGGL((xy_terms_rows_kernel<R, NT>), dim3((unsigned)(NT / kSegRows), (unsigned)R_), dim3(NT), 0,
+                       as_stream(stream), th, twist, partials);
 }
 template <class R>
 static void terms_rows(void *stream, const R *th, int R_, int L, const double *twist, double *partials) {
@@ -535,13 +766,17 @@ template <class R, int COMP>
 static int terms_impl(void *stream, const void *state, int R_, int L, const double *twist, double *out) {
     MC_REQUIRE(L >= 2, "invalid lattice size %d", L);
     MC_REQUIRE(R_ >= 1, "invalid replica count %d", R_);
-    const bool rows = COMP == 1 && rows_path(L);
-    const int ntiles = tiles_of(L), nblk = rows ? L / kSegRows : (ntiles + kTermsTiles - 1) / kTermsTiles;
+    const bool band = COMP == 1 && sizeof(R) == 4 && band_path(L);
+    const bool rows = COMP == 1 && !band && rows_path(L);
+    const int ntiles = tiles_of(L);
+    const int nblk = band ? L / kBandRows : rows ? terms_rows_partials(L) : (ntiles + kTermsTiles - 1) / kTermsTiles;
     void *base = nullptr;
     int rc = reduction_scratch(stream, (size_t)R_ * nblk * 3 * sizeof(double), &base);
     if (rc) return rc;
     double *partials = reinterpret_cast<double *>(base);
-    if (rows)
+    if (band)
+        terms_band(stream, (const float *)state, R_, L, twist, partials);
+    else if (rows)
         terms_rows<R>(stream, (const R *)state, R_, L, twist, partials);
     else
         hipLaunchKernelGGL((terms_tile_kernel<R, COMP>), dim3((unsigned)nblk, (unsigned)R_), dim3(kTileThreads), 0,
@@ -579,7 +814,9 @@ int mc_xy_twist_grad(void *stream, int dtype, const void *theta, int R, int L, c
                      const double *alpha, const double *beta, const double *gamma, void *grad) {
     MC_REQUIRE(R >= 1 && L >= 2, "invalid shape R=%d L=%d", R, L);
     const Coefs k{alpha, beta, gamma};
-    if (dtype == MC_F32 && rows_path(L))
+    if (dtype == MC_F32 && band_path(L))
+        grad_band(stream, (const float *)theta, R, L, twist, k, (float *)grad);
+    else if (dtype == MC_F32 && rows_path(L))
         grad_rows<float>(stream, (const float *)theta, R, L, twist, k, (float *)grad);
     else if (dtype == MC_F64 && rows_path(L))
         grad_rows<double>(stream, (const double *)theta, R, L, twist, k, (double *)grad);

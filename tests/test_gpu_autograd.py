@@ -186,13 +186,14 @@ def test_float32_terms_accurate_for_angles_of_many_turns(gpu):
     assert (g32.double() - g64).abs().max() < 3e-4
 
 
-@pytest.mark.parametrize("L", [64, 128])
+@pytest.mark.parametrize("L", [64, 128, 256, 1024])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_whole_row_xy_kernels_against_closed_forms(gpu, L, dtype):
     """The whole-row XY kernels (L a power of two from 64 to 1024: a workgroup per
-    16-row segment, rows by shuffles and LDS across wavefront edges and the
-    periodic wrap) against the bond sums and their gradient evaluated in numpy
-    float64, with a twist per replica."""
+    16-row segment, rows by DPP shifts, the bonds across wavefront edges and the
+    periodic wrap from LDS; 1, 2, 4 and 16 wavefronts per row here) against the
+    bond sums and their gradient evaluated in numpy float64, with a twist per
+    replica."""
     from autograd import lattice_energy
     R = 3
     th64 = _xy(R, L, torch.float64, 91 + L)
