@@ -184,8 +184,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
 
     // ---- LDS carve-up (the RNG streams live in registers)
     const StripView view = strip_view(smem, L, H, false);
-    float *lc = view.c;     // rows 0..H+2
-    float *ls = view.s;     // rows 0..H+2
+    float2 *lcs = view.cs;  // rows 0..H+2: (cos, sin)
     float *lth = view.th;   // rows 1..H+1
     int8_t *lsp = view.sp;  // rows 0..H+2 (Ising)
     unsigned char *pt_base = smem + ((strip_bytes(MODEL, L, H, false) + 15) & ~(size_t)15);
@@ -234,8 +233,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
                 const float th = thv[b];
                 float sv, cv;
                 fast_sincosf(th, sv, cv);
-                lc[lds_index(lr, j, L)] = cv;
-                ls[lds_index(lr, j, L)] = sv;
+                lcs[lds_index(lr, j, L)] = make_float2(cv, sv);
                 if (lr >= 1 && lr <= H + 1) lth[lds_index(lr - 1, j, L)] = th;
             }
         }
@@ -336,8 +334,9 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             for (int m = 0; m < MAXP; m++) {
                 const int k = slot_on[p][m] ? slot_self[p][m] : L;
                 slot_th[p][m] = lth[k - L];
-                slot_c[p][m] = lc[k];
-                slot_s[p][m] = ls[k];
+                const float2 q = lcs[k];
+                slot_c[p][m] = q.x;
+                slot_s[p][m] = q.y;
             }
     }
 
@@ -383,14 +382,15 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             } else {
                 const int self = slot_self[p][m], lf = slot_lf[p][m], rt = slot_rt[p][m];
                 const int up = self - L, dn = self + L;
-                // All eight neighbour reads are issued before the first add:
+                // All four neighbour reads are issued before the first add:
                 // left to itself the scheduler interleaves each read with a
-                // wait and an add, seven LDS round trips in a row per pass
+                // wait and an add, LDS round trips in a row per pass
                 // (measured on MI355X: 124.5 -> 144 site-updates/ns).
-                float cu = lc[up], cd = lc[dn];
-                const float cl = lc[lf], cr = lc[rt];
-                float su = ls[up], sd = ls[dn];
-                const float sl = ls[lf], sr = ls[rt];
+                const float2 nu = lcs[up], nd = lcs[dn], nl = lcs[lf], nr = lcs[rt];
+                float cu = nu.x, cd = nd.x;
+                const float cl = nl.x, cr = nr.x;
+                float su = nu.y, sd = nd.y;
+                const float sl = nl.y, sr = nr.y;
                 if (bnd) {
                     float rs, rc;
                     fast_sincosf(__uint_as_float(received(p, m, n)), rs, rc);
@@ -412,8 +412,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
                 if (slot_draw[p][m].b < w) {
                     const float pr = slot_pr[p][m];
                     lth[up] = pr;
-                    lc[self] = slot_pc[p][m];
-                    ls[self] = slot_ps[p][m];
+                    lcs[self] = make_float2(slot_pc[p][m], slot_ps[p][m]);
                     slot_th[p][m] = pr;
                     slot_c[p][m] = slot_pc[p][m];
                     slot_s[p][m] = slot_ps[p][m];
@@ -451,8 +450,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
             if (MODEL == MODEL_ISING) {
                 lsp[dst] = lsp[src];
             } else {
-                lc[dst] = lc[src];
-                ls[dst] = ls[src];
+                lcs[dst] = lcs[src];
             }
         }
         __syncthreads();
@@ -640,10 +638,11 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
 #pragma unroll
                     for (int m = 0; m < MAXP; m++) {
                         const int dn = slot_self[p][m] + L, rt = slot_rt[p][m];
-                        dc[p][m] = lc[dn];
-                        dsn[p][m] = ls[dn];
-                        rc[p][m] = lc[rt];
-                        rsn[p][m] = ls[rt];
+                        const float2 qd = lcs[dn], qr = lcs[rt];
+                        dc[p][m] = qd.x;
+                        dsn[p][m] = qd.y;
+                        rc[p][m] = qr.x;
+                        rsn[p][m] = qr.y;
                     }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll

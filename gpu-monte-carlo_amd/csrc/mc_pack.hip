@@ -353,8 +353,7 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
                         } else {
                             float sv, cv;
                             fast_sincosf(th[i], sv, cv);
-                            v.c[off + L + ljc] = cv;
-                            v.s[off + L + ljc] = sv;
+                            v.cs[off + L + ljc] = make_float2(cv, sv);
                             v.th[off + ljc] = th[i];
                         }
                     }
@@ -376,8 +375,7 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
                     } else {
                         float sv, cv;
                         fast_sincosf(__uint_as_float(payload), sv, cv);
-                        v.c[lds_index(row, lj, L)] = cv;
-                        v.s[lds_index(row, lj, L)] = sv;
+                        v.cs[lds_index(row, lj, L)] = make_float2(cv, sv);
                     }
                 };
                 int bad = 0;
@@ -402,8 +400,7 @@ __device__ void pack_run(const PackArgs &a, const PackGroup *gp, const PackWork 
                             if (MODEL == MODEL_ISING) {
                                 v.sp[lds_index(dst, lj, L)] = v.sp[lds_index(src, lj, L)];
                             } else {
-                                v.c[lds_index(dst, lj, L)] = v.c[lds_index(src, lj, L)];
-                                v.s[lds_index(dst, lj, L)] = v.s[lds_index(src, lj, L)];
+                                v.cs[lds_index(dst, lj, L)] = v.cs[lds_index(src, lj, L)];
                             }
                         }
                     __syncthreads();

@@ -165,7 +165,7 @@ __host__ __device__ inline size_t strip_bytes(int model, int L, int H, bool rng_
 // LDS arrays of one strip
 struct StripView {
     ulonglong2 *rng;  // rows 1..H+1 (when staged in LDS)
-    float *c, *s;     // rows 0..H+2 (XY)
+    float2 *cs;       // rows 0..H+2 (XY): (cos, sin) of each site, one 8-byte LDS access
     float *th;        // rows 1..H+1 (XY)
     int8_t *sp;       // rows 0..H+2 (Ising)
 };
@@ -174,9 +174,8 @@ __device__ __forceinline__ StripView strip_view(unsigned char *base, int L, int 
     StripView v;
     v.rng = reinterpret_cast<ulonglong2 *>(base);
     unsigned char *site = base + (rng_in_lds ? (size_t)(H + 1) * L * 16 : 0);
-    v.c = reinterpret_cast<float *>(site);
-    v.s = v.c + (H + 3) * L;
-    v.th = v.s + (H + 3) * L;
+    v.cs = reinterpret_cast<float2 *>(site);
+    v.th = reinterpret_cast<float *>(v.cs + (H + 3) * L);
     v.sp = reinterpret_cast<int8_t *>(site);
     return v;
 }
@@ -235,16 +234,16 @@ __device__ __forceinline__ bool site_apply(const StripView &v, const int self, c
         }
         float sp, cp;
         fast_sincosf(pr, sp, cp);
-        const float Cs = ((v.c[up] + v.c[dn]) + v.c[lf]) + v.c[rt];
-        const float Ss = ((v.s[up] + v.s[dn]) + v.s[lf]) + v.s[rt];
-        float t = (cp - v.c[self]) * Cs;
-        t = fmaf(sp - v.s[self], Ss, t);
+        const float2 nu = v.cs[up], nd = v.cs[dn], nl = v.cs[lf], nr = v.cs[rt], no = v.cs[self];
+        const float Cs = ((nu.x + nd.x) + nl.x) + nr.x;
+        const float Ss = ((nu.y + nd.y) + nl.y) + nr.y;
+        float t = (cp - no.x) * Cs;
+        t = fmaf(sp - no.y, Ss, t);
         const float dE = -J * t;
         const bool acc = d.b < accept_weight(-beta * dE);  // dE <= 0 or b < exp(-beta dE)
         if (acc) {
             v.th[own] = pr;
-            v.c[self] = cp;
-            v.s[self] = sp;
+            v.cs[self] = make_float2(cp, sp);
         }
         return acc;
     }
@@ -289,8 +288,9 @@ __device__ __forceinline__ bool site_update(const StripView &v, const int c, con
         const float th = v.th[own];
         // the site's and its neighbours' (cos, sin) are read before the draw,
         // so their LDS latency runs under the proposal's arithmetic
-        const float cu = v.c[up], cd = v.c[dn], cl = v.c[lf], cr = v.c[rt], c0 = v.c[self];
-        const float su = v.s[up], sd = v.s[dn], sl = v.s[lf], sr = v.s[rt], s0 = v.s[self];
+        const float2 u2 = v.cs[up], d2 = v.cs[dn], l2 = v.cs[lf], r2 = v.cs[rt], o2 = v.cs[self];
+        const float cu = u2.x, cd = d2.x, cl = l2.x, cr = r2.x, c0 = o2.x;
+        const float su = u2.y, sd = d2.y, sl = l2.y, sr = r2.y, s0 = o2.y;
         __builtin_amdgcn_sched_barrier(0);
         float pr;
         float w;
@@ -317,8 +317,7 @@ __device__ __forceinline__ bool site_update(const StripView &v, const int c, con
         acc = w < accept_weight(-beta * dE);  // dE <= 0 or w < exp(-beta dE)
         if (acc) {
             v.th[own] = pr;
-            v.c[self] = cp;
-            v.s[self] = sp;
+            v.cs[self] = make_float2(cp, sp);
         }
     }
     ulonglong2 out;
@@ -341,15 +340,16 @@ __device__ __forceinline__ void site_observables(const StripView &v, const int l
         e += sv * ((int)v.sp[dn] + (int)v.sp[rt]);
         if (all) m[0] += sv;
     } else {
-        const float cs = v.c[self], ss = v.s[self];
-        const float bxc = fmaf(ss, v.s[dn], cs * v.c[dn]);
-        const float byc = fmaf(ss, v.s[rt], cs * v.c[rt]);
+        const float2 o = v.cs[self], d = v.cs[dn], r = v.cs[rt];
+        const float cs = o.x, ss = o.y;
+        const float bxc = fmaf(ss, d.y, cs * d.x);
+        const float byc = fmaf(ss, r.y, cs * r.x);
         e += bond_fixed(bxc + byc);
         if (all) {
             m[0] += cs;
             m[1] += ss;
             m[2] += bxc;
-            m[3] += fmaf(ss, v.c[dn], -(cs * v.s[dn]));
+            m[3] += fmaf(ss, d.x, -(cs * d.y));
         }
     }
 }
