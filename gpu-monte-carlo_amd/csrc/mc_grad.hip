@@ -47,9 +47,39 @@ __device__ __forceinline__ void sincos_r<float>(float x, float &s, float &c) {
     s = __builtin_amdgcn_sinf(t);
     c = __builtin_amdgcn_cosf(t);
 }
+// float64: whole quarter turns removed by a three-part Cody-Waite reduction,
+// r = x - n pi/2 with pi/2 = kPio2[0] + kPio2[1] + kPio2[2] in fused
+// multiply-adds (each step rounds once), then the minimax polynomials of
+// fdlibm's __kernel_sin and __kernel_cos (Sun Microsystems, 1993) on
+// [-pi/4, pi/4], each within an ulp there, and the quadrant's swap and
+// signs: about 30 float64 operations, where the library sincos took about
+// 150 instructions per call and, inlined for large arguments only, cost a
+// kernel an occupancy step.  The first step leaves about n kPio2[1] and rounds
+// it, so the result is within about 2e-16 for |x| < 2^30 and about
+// |x| 5e-27 beyond, far below the rounding of x itself (|x| 1.1e-16).  NaN,
+// infinite x and |x| >= 2^50 (where n pi/2 no longer cancels x to within
+// pi/4) give NaN.
 template <>
 __device__ __forceinline__ void sincos_r<double>(double x, double &s, double &c) {
-    ::sincos(x, &s, &c);
+    x = fabs(x) < 0x1p50 ? x : __builtin_nan("");
+    constexpr double kTwoOverPi = 0x1.45f306dc9c883p-1;
+    constexpr double kPio2[3] = {0x1.921fb544p+0, 0x1.0b4611a6p-34, 0x1.3198a2ep-69};
+    constexpr double kS[6] = {-0x1.5555555555549p-3, 0x1.111111110f8a6p-7, -0x1.a01a019c161d5p-13,
+                              0x1.71de357b1fe7dp-19, -0x1.ae5e68a2b9cebp-26, 0x1.5d93a5acfd57cp-33};
+    constexpr double kC[6] = {0x1.555555555554cp-5, -0x1.6c16c16c15177p-10, 0x1.a01a019cb159p-16,
+                              -0x1.27e4f809c52adp-22, 0x1.1ee9ebdb4b1c4p-29, -0x1.8fae9be8838d4p-37};
+    const double n = rint(x * kTwoOverPi);
+    double r = fma(-n, kPio2[0], x);
+    r = fma(-n, kPio2[1], r);
+    r = fma(-n, kPio2[2], r);
+    const double z = r * r;
+    const double ps = fma(r * z, fma(z, fma(z, fma(z, fma(z, fma(z, kS[5], kS[4]), kS[3]), kS[2]), kS[1]), kS[0]), r);
+    const double pc =
+        fma(z * z, fma(z, fma(z, fma(z, fma(z, fma(z, kC[5], kC[4]), kC[3]), kC[2]), kC[1]), kC[0]), fma(-0.5, z, 1.0));
+    const int q = (int)(long long)n & 3;
+    const double sv = (q & 1) ? pc : ps, cv = (q & 1) ? ps : pc;
+    s = (q & 2) ? -sv : sv;
+    c = ((q + 1) & 2) ? -cv : cv;
 }
 
 // Per-replica values of an optional array (NULL: 0).  The load is made from a
@@ -644,6 +674,50 @@ __global__ __launch_bounds__(64) void xy_terms_band_kernel(const float *__restri
     }
 }
 
+// The rows of a band: s1 (and c1) of the down x bond, carried to the next
+// row as s2 (and c2); TWIST false drops the rotation by -Delta and the gamma
+// term (zero twist and gamma), where they change nothing, bit for bit:
+// a rotation by (1, 0) and gamma * (c1 - c2) = 0 are exact.
+template <int M, bool TWIST>
+__device__ __forceinline__ void grad_band_rows(const float (&v)[kBandRows + 2][M], float (&cc)[M], float (&sc)[M],
+                                               float (&c2)[M], float (&s2)[M], float cd, float sd, float a,
+                                               float bb, float gg, float *__restrict__ out, int L) {
+#pragma unroll
+    for (int k = 1; k <= kBandRows; k++) {
+        float cn[M], sn[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) sincos_r<float>(v[k + 1][m], sn[m], cn[m]);
+        float s3[M];  // y bond (i, j) - (i, j+1)
+#pragma unroll
+        for (int m = 0; m < M - 1; m++) s3[m] = fmaf(sc[m], cc[m + 1], -(cc[m] * sc[m + 1]));
+        s3[M - 1] = fmaf(sc[M - 1], rot_next(cc[0]), -(cc[M - 1] * rot_next(sc[0])));
+        const float s4_first = rot_prev(s3[M - 1]);  // (i, j-1) - (i, j) of the lane's first column
+        float g[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const float s4 = m == 0 ? s4_first : s3[m - 1];
+            // dCx/dt = -s1 + s2, dCy/dt = -s3 + s4, dSx/dt = c1 - c2
+            if (TWIST) {
+                float c1, s1;  // down bond (i, j) - (i+1, j)
+                bond_x_fma<float>(cc[m], sc[m], cn[m], sn[m], cd, sd, c1, s1);
+                g[m] = fmaf(a, s2[m] - s1, fmaf(bb, s4 - s3[m], gg * (c1 - c2[m])));
+                c2[m] = c1;
+                s2[m] = s1;
+            } else {
+                const float s1 = fmaf(sc[m], cn[m], -(cc[m] * sn[m]));
+                g[m] = fmaf(a, s2[m] - s1, bb * (s4 - s3[m]));
+                s2[m] = s1;
+            }
+        }
+        store_cols<M>(out + (size_t)(k - 1) * L, g);
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            cc[m] = cn[m];
+            sc[m] = sn[m];
+        }
+    }
+}
+
 // Gradient of alpha Cx + beta Cy + gamma Sx, kBandRows rows per wavefront.
 template <int M>
 __global__ __launch_bounds__(64) void xy_grad_band_kernel(const float *__restrict__ th, const double *twist,
@@ -670,34 +744,11 @@ __global__ __launch_bounds__(64) void xy_grad_band_kernel(const float *__restric
         sincos_r<float>(v[1][m], sc[m], cc[m]);
         bond_x_fma<float>(cu, su, cc[m], sc[m], cd, sd, c2[m], s2[m]);  // up bond (i-1, j) - (i, j)
     }
-#pragma unroll
-    for (int k = 1; k <= kBandRows; k++) {
-        float cn[M], sn[M];
-#pragma unroll
-        for (int m = 0; m < M; m++) sincos_r<float>(v[k + 1][m], sn[m], cn[m]);
-        float s3[M];  // y bond (i, j) - (i, j+1)
-#pragma unroll
-        for (int m = 0; m < M - 1; m++) s3[m] = fmaf(sc[m], cc[m + 1], -(cc[m] * sc[m + 1]));
-        s3[M - 1] = fmaf(sc[M - 1], rot_next(cc[0]), -(cc[M - 1] * rot_next(sc[0])));
-        const float s4_first = rot_prev(s3[M - 1]);  // (i, j-1) - (i, j) of the lane's first column
-        float g[M];
-#pragma unroll
-        for (int m = 0; m < M; m++) {
-            float c1, s1;  // down bond (i, j) - (i+1, j)
-            bond_x_fma<float>(cc[m], sc[m], cn[m], sn[m], cd, sd, c1, s1);
-            const float s4 = m == 0 ? s4_first : s3[m - 1];
-            // dCx/dt = -s1 + s2, dCy/dt = -s3 + s4, dSx/dt = c1 - c2
-            g[m] = fmaf(a, s2[m] - s1, fmaf(bb, s4 - s3[m], gg * (c1 - c2[m])));
-            c2[m] = c1;
-            s2[m] = s1;
-        }
-        store_cols<M>(grad + base + (size_t)(i0 + k - 1) * L + lane * M, g);
-#pragma unroll
-        for (int m = 0; m < M; m++) {
-            cc[m] = cn[m];
-            sc[m] = sn[m];
-        }
-    }
+    float *out = grad + base + (size_t)i0 * L + lane * M;
+    if (cd == 1.0f && sd == 0.0f && gg == 0.0f)  // the replica's values: uniform in the wavefront
+        grad_band_rows<M, false>(v, cc, sc, c2, s2, cd, sd, a, bb, gg, out, L);
+    else
+        grad_band_rows<M, true>(v, cc, sc, c2, s2, cd, sd, a, bb, gg, out, L);
 }
 
 // float32 lattices of L = 64, 128, 256 take the band kernels

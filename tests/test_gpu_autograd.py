@@ -186,6 +186,38 @@ def test_float32_terms_accurate_for_angles_of_many_turns(gpu):
     assert (g32.double() - g64).abs().max() < 3e-4
 
 
+def test_float64_sincos_of_the_kernels_at_large_angles(gpu):
+    """The float64 kernels' own sine and cosine (a three-part reduction by pi/2
+    and fdlibm's polynomials) at angles spread over +-1e6 rad: the energy and
+    gradient equal a numpy float64 evaluation of the same bond identities from
+    each site's np.cos and np.sin to 1e-12, so only the sine and cosine are
+    compared.  An angle of 2^51 rad (past the reduction's range) gives NaN."""
+    from autograd import lattice_energy
+    g = np.random.default_rng(23)
+    R, L = 4, 64
+    host = g.uniform(-1e6, 1e6, (R, L, L))
+    tw = np.array([0.0, 0.3, -1.1, 2.0])
+    th = torch.tensor(host, device=_dev()).requires_grad_(True)
+    H = lattice_energy(th, 1.0, torch.tensor(tw, device=_dev()), "xy")
+    (gr,) = torch.autograd.grad(H.sum(), th)
+    c, s = np.cos(host), np.sin(host)
+    cd, sd = np.cos(tw)[:, None, None], np.sin(tw)[:, None, None]
+
+    def bond(c1, s1, c2, s2):  # cos and sin of theta_1 - theta_2
+        return c1 * c2 + s1 * s2, s1 * c2 - c1 * s2
+
+    cxa, sxa = bond(c, s, np.roll(c, -1, 1), np.roll(s, -1, 1))  # (i, j) - (i+1, j)
+    cx, sx = cxa * cd + sxa * sd, sxa * cd - cxa * sd             # twisted by -Delta
+    cy, sy = bond(c, s, np.roll(c, -1, 2), np.roll(s, -1, 2))     # (i, j) - (i, j+1)
+    H_ref = -(cx.sum((1, 2)) + cy.sum((1, 2)))
+    g_ref = -(-sx + np.roll(sx, 1, 1) - sy + np.roll(sy, 1, 2))
+    np.testing.assert_allclose(H.detach().cpu().numpy(), H_ref, rtol=0, atol=1e-12 * L * L)
+    np.testing.assert_allclose(gr.cpu().numpy(), g_ref, rtol=0, atol=1e-12)
+    host[1, 5, 7] = 2.0 ** 51
+    H2 = lattice_energy(torch.tensor(host, device=_dev()), 1.0, 0.0, "xy").cpu().numpy()
+    assert np.isnan(H2[1]) and np.isfinite(H2[[0, 2, 3]]).all()
+
+
 @pytest.mark.parametrize("L", [64, 128, 256, 1024])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_whole_row_xy_kernels_against_closed_forms(gpu, L, dtype):
