@@ -574,33 +574,55 @@ __global__ __launch_bounds__(NT) void xy_grad_rows_kernel(const R *__restrict__ 
     }
 }
 
-// ------------------------------------ XY float32, one wavefront per row band
+// ------------------------------------------- XY, one wavefront per row band
 //
 // For L = 64, 128, 256 a wavefront covers whole rows: lane l holds the M =
-// L / 64 adjacent columns M l .. M l + M - 1 (one float, float2 or float4
-// load per row) of a band of kSegRows rows.  A site's right neighbour is in
-// the same lane, or (last column) lane l + 1's first column by a DPP wave
-// rotate, which also closes the periodic row (lane 63 takes lane 0's): no
-// LDS, no barrier, no edge fix-ups, and one wavefront per workgroup.  Each
-// bond is evaluated once: the x bond below row k is the site's down bond and
-// the next row's up bond; the y bond to the right is the site's s3 and its
-// right neighbour's s4.
-constexpr int kBandRows = 16;
+// L / 64 adjacent columns M l .. M l + M - 1 (one vector load per row) of a
+// band of BandRows<R> rows (16 in float32, 8 in float64, whose registers are
+// twice as wide).  A site's right neighbour is in the same lane, or (last
+// column) lane l + 1's first column by a DPP wave rotate, which also closes
+// the periodic row (lane 63 takes lane 0's): no LDS, no barrier, no edge
+// fix-ups, and one wavefront per workgroup.  Each bond is evaluated once: the
+// x bond below row k is the site's down bond and the next row's up bond; the
+// y bond to the right is the site's s3 and its right neighbour's s4.
+template <class R>
+struct BandRows {
+    static constexpr int value = sizeof(R) == 4 ? 16 : 8;
+};
 
 template <int CTRL>
 __device__ __forceinline__ float wave_rotate(float x) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
 }
+template <int CTRL>
+__device__ __forceinline__ double wave_rotate(double x) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xf, 0xf, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 // lane i takes lane i + 1's value, lane 63 lane 0's (wave_rol:1)
-__device__ __forceinline__ float rot_next(float x) { return wave_rotate<0x134>(x); }
+template <class R>
+__device__ __forceinline__ R rot_next(R x) { return wave_rotate<0x134>(x); }
 // lane i takes lane i - 1's value, lane 0 lane 63's (wave_ror:1)
-__device__ __forceinline__ float rot_prev(float x) { return wave_rotate<0x13C>(x); }
+template <class R>
+__device__ __forceinline__ R rot_prev(R x) { return wave_rotate<0x13C>(x); }
 
-template <int M>
-__device__ __forceinline__ void load_cols(const float *p, float (&x)[M]) {
-    if constexpr (M == 4) {
-        const float4 v = *reinterpret_cast<const float4 *>(p);
-        x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+// M adjacent values in 16-byte (or smaller) vector accesses
+template <class R, int M>
+__device__ __forceinline__ void load_cols(const R *p, R (&x)[M]) {
+    constexpr int W = 16 / (int)sizeof(R);  // values per 16-byte access
+    if constexpr (M >= W) {
+#pragma unroll
+        for (int q = 0; q < M; q += W) {
+            if constexpr (sizeof(R) == 4) {
+                const float4 v = *reinterpret_cast<const float4 *>(p + q);
+                x[q] = v.x; x[q + 1] = v.y; x[q + 2] = v.z; x[q + 3] = v.w;
+            } else {
+                const double2 v = *reinterpret_cast<const double2 *>(p + q);
+                x[q] = v.x; x[q + 1] = v.y;
+            }
+        }
     } else if constexpr (M == 2) {
         const float2 v = *reinterpret_cast<const float2 *>(p);
         x[0] = v.x; x[1] = v.y;
@@ -608,48 +630,56 @@ __device__ __forceinline__ void load_cols(const float *p, float (&x)[M]) {
         x[0] = *p;
     }
 }
-template <int M>
-__device__ __forceinline__ void store_cols(float *p, const float (&x)[M]) {
-    if constexpr (M == 4)
-        *reinterpret_cast<float4 *>(p) = make_float4(x[0], x[1], x[2], x[3]);
-    else if constexpr (M == 2)
+template <class R, int M>
+__device__ __forceinline__ void store_cols(R *p, const R (&x)[M]) {
+    constexpr int W = 16 / (int)sizeof(R);
+    if constexpr (M >= W) {
+#pragma unroll
+        for (int q = 0; q < M; q += W) {
+            if constexpr (sizeof(R) == 4)
+                *reinterpret_cast<float4 *>(p + q) = make_float4(x[q], x[q + 1], x[q + 2], x[q + 3]);
+            else
+                *reinterpret_cast<double2 *>(p + q) = make_double2(x[q], x[q + 1]);
+        }
+    } else if constexpr (M == 2) {
         *reinterpret_cast<float2 *>(p) = make_float2(x[0], x[1]);
-    else
+    } else {
         *p = x[0];
+    }
 }
 
-// Terms: per-wavefront partials (Cx, Cy, Sx), kBandRows rows each.
-template <int M>
-__global__ __launch_bounds__(64) void xy_terms_band_kernel(const float *__restrict__ th, const double *twist,
+// Terms: per-wavefront partials (Cx, Cy, Sx), BandRows<R> rows each.
+template <class R, int M>
+__global__ __launch_bounds__(64) void xy_terms_band_kernel(const R *__restrict__ th, const double *twist,
                                                           double *__restrict__ partials) {
-    constexpr int L = 64 * M;
+    constexpr int L = 64 * M, BR = BandRows<R>::value;
     const int r = blockIdx.y, lane = threadIdx.x;
-    const int i0 = blockIdx.x * kBandRows;
-    const float *b = th + (size_t)r * L * L + lane * M;
-    float v[kBandRows + 1][M];
+    const int i0 = blockIdx.x * BR;
+    const R *b = th + (size_t)r * L * L + lane * M;
+    R v[BR + 1][M];
 #pragma unroll
-    for (int k = 0; k <= kBandRows; k++) load_cols<M>(b + (size_t)((i0 + k) & (L - 1)) * L, v[k]);
-    float cd, sd;
-    twist_cs<float>(twist, r, cd, sd);
-    float cc[M], sc[M];
+    for (int k = 0; k <= BR; k++) load_cols<R, M>(b + (size_t)((i0 + k) & (L - 1)) * L, v[k]);
+    R cd, sd;
+    twist_cs<R>(twist, r, cd, sd);
+    R cc[M], sc[M];
 #pragma unroll
-    for (int m = 0; m < M; m++) sincos_r<float>(v[0][m], sc[m], cc[m]);
+    for (int m = 0; m < M; m++) sincos_r<R>(v[0][m], sc[m], cc[m]);
     // one set of sums per column: with one wavefront per SIMD nothing hides the
-    // latency of a dependent chain, and M chains of 2 kBandRows terms are
-    // shorter than one of 2 M kBandRows
-    float ca[M] = {}, sa[M] = {}, cy[M] = {};
+    // latency of a dependent chain, and M chains of 2 BR terms are shorter
+    // than one of 2 M BR
+    R ca[M] = {}, sa[M] = {}, cy[M] = {};
 #pragma unroll
-    for (int k = 0; k < kBandRows; k++) {
-        float cn[M], sn[M];
+    for (int k = 0; k < BR; k++) {
+        R cn[M], sn[M];
 #pragma unroll
-        for (int m = 0; m < M; m++) sincos_r<float>(v[k + 1][m], sn[m], cn[m]);
+        for (int m = 0; m < M; m++) sincos_r<R>(v[k + 1][m], sn[m], cn[m]);
 #pragma unroll
         for (int m = 0; m < M; m++) {
-            ca[m] = fmaf(cc[m], cn[m], fmaf(sc[m], sn[m], ca[m]));   // x bond (i, j) - (i+1, j): cos a
-            sa[m] = fmaf(sc[m], cn[m], fmaf(-cc[m], sn[m], sa[m]));  //                          sin a
-            const float cr = m < M - 1 ? cc[m + 1] : rot_next(cc[0]);
-            const float sr = m < M - 1 ? sc[m + 1] : rot_next(sc[0]);
-            cy[m] = fmaf(cc[m], cr, fmaf(sc[m], sr, cy[m]));          // y bond (i, j) - (i, j+1)
+            ca[m] = fma_r(cc[m], cn[m], fma_r(sc[m], sn[m], ca[m]));   // x bond (i, j) - (i+1, j): cos a
+            sa[m] = fma_r(sc[m], cn[m], fma_r(-cc[m], sn[m], sa[m]));  //                          sin a
+            const R cr = m < M - 1 ? cc[m + 1] : rot_next(cc[0]);
+            const R sr = m < M - 1 ? sc[m + 1] : rot_next(sc[0]);
+            cy[m] = fma_r(cc[m], cr, fma_r(sc[m], sr, cy[m]));          // y bond (i, j) - (i, j+1)
         }
 #pragma unroll
         for (int m = 0; m < M; m++) {
@@ -657,7 +687,7 @@ __global__ __launch_bounds__(64) void xy_terms_band_kernel(const float *__restri
             sc[m] = sn[m];
         }
     }
-    float ca_t = ca[0], sa_t = sa[0], cy_t = cy[0];
+    R ca_t = ca[0], sa_t = sa[0], cy_t = cy[0];
 #pragma unroll
     for (int m = 1; m < M; m++) {
         ca_t += ca[m];
@@ -678,38 +708,39 @@ __global__ __launch_bounds__(64) void xy_terms_band_kernel(const float *__restri
 // row as s2 (and c2); TWIST false drops the rotation by -Delta and the gamma
 // term (zero twist and gamma), where they change nothing, bit for bit:
 // a rotation by (1, 0) and gamma * (c1 - c2) = 0 are exact.
-template <int M, bool TWIST>
-__device__ __forceinline__ void grad_band_rows(const float (&v)[kBandRows + 2][M], float (&cc)[M], float (&sc)[M],
-                                               float (&c2)[M], float (&s2)[M], float cd, float sd, float a,
-                                               float bb, float gg, float *__restrict__ out, int L) {
+template <class R, int M, bool TWIST>
+__device__ __forceinline__ void grad_band_rows(const R (&v)[BandRows<R>::value + 2][M], R (&cc)[M], R (&sc)[M],
+                                               R (&c2)[M], R (&s2)[M], R cd, R sd, R a, R bb, R gg,
+                                               R *__restrict__ out, int L) {
+    constexpr int BR = BandRows<R>::value;
 #pragma unroll
-    for (int k = 1; k <= kBandRows; k++) {
-        float cn[M], sn[M];
+    for (int k = 1; k <= BR; k++) {
+        R cn[M], sn[M];
 #pragma unroll
-        for (int m = 0; m < M; m++) sincos_r<float>(v[k + 1][m], sn[m], cn[m]);
-        float s3[M];  // y bond (i, j) - (i, j+1)
+        for (int m = 0; m < M; m++) sincos_r<R>(v[k + 1][m], sn[m], cn[m]);
+        R s3[M];  // y bond (i, j) - (i, j+1)
 #pragma unroll
-        for (int m = 0; m < M - 1; m++) s3[m] = fmaf(sc[m], cc[m + 1], -(cc[m] * sc[m + 1]));
-        s3[M - 1] = fmaf(sc[M - 1], rot_next(cc[0]), -(cc[M - 1] * rot_next(sc[0])));
-        const float s4_first = rot_prev(s3[M - 1]);  // (i, j-1) - (i, j) of the lane's first column
-        float g[M];
+        for (int m = 0; m < M - 1; m++) s3[m] = fma_r(sc[m], cc[m + 1], -(cc[m] * sc[m + 1]));
+        s3[M - 1] = fma_r(sc[M - 1], rot_next(cc[0]), -(cc[M - 1] * rot_next(sc[0])));
+        const R s4_first = rot_prev(s3[M - 1]);  // (i, j-1) - (i, j) of the lane's first column
+        R g[M];
 #pragma unroll
         for (int m = 0; m < M; m++) {
-            const float s4 = m == 0 ? s4_first : s3[m - 1];
+            const R s4 = m == 0 ? s4_first : s3[m - 1];
             // dCx/dt = -s1 + s2, dCy/dt = -s3 + s4, dSx/dt = c1 - c2
             if (TWIST) {
-                float c1, s1;  // down bond (i, j) - (i+1, j)
-                bond_x_fma<float>(cc[m], sc[m], cn[m], sn[m], cd, sd, c1, s1);
-                g[m] = fmaf(a, s2[m] - s1, fmaf(bb, s4 - s3[m], gg * (c1 - c2[m])));
+                R c1, s1;  // down bond (i, j) - (i+1, j)
+                bond_x_fma<R>(cc[m], sc[m], cn[m], sn[m], cd, sd, c1, s1);
+                g[m] = fma_r(a, s2[m] - s1, fma_r(bb, s4 - s3[m], gg * (c1 - c2[m])));
                 c2[m] = c1;
                 s2[m] = s1;
             } else {
-                const float s1 = fmaf(sc[m], cn[m], -(cc[m] * sn[m]));
-                g[m] = fmaf(a, s2[m] - s1, bb * (s4 - s3[m]));
+                const R s1 = fma_r(sc[m], cn[m], -(cc[m] * sn[m]));
+                g[m] = fma_r(a, s2[m] - s1, bb * (s4 - s3[m]));
                 s2[m] = s1;
             }
         }
-        store_cols<M>(out + (size_t)(k - 1) * L, g);
+        store_cols<R, M>(out + (size_t)(k - 1) * L, g);
 #pragma unroll
         for (int m = 0; m < M; m++) {
             cc[m] = cn[m];
@@ -718,59 +749,63 @@ __device__ __forceinline__ void grad_band_rows(const float (&v)[kBandRows + 2][M
     }
 }
 
-// Gradient of alpha Cx + beta Cy + gamma Sx, kBandRows rows per wavefront.
-template <int M>
-__global__ __launch_bounds__(64) void xy_grad_band_kernel(const float *__restrict__ th, const double *twist,
-                                                         Coefs kc, float *__restrict__ grad) {
-    constexpr int L = 64 * M;
+// Gradient of alpha Cx + beta Cy + gamma Sx, BandRows<R> rows per wavefront.
+template <class R, int M>
+__global__ __launch_bounds__(64) void xy_grad_band_kernel(const R *__restrict__ th, const double *twist, Coefs kc,
+                                                         R *__restrict__ grad) {
+    constexpr int L = 64 * M, BR = BandRows<R>::value;
     const int r = blockIdx.y, lane = threadIdx.x;
-    const int i0 = blockIdx.x * kBandRows;
+    const int i0 = blockIdx.x * BR;
     const size_t base = (size_t)r * L * L;
-    const float *b = th + base + lane * M;
-    // rows i0 - 1 .. i0 + kBandRows (index k + 1 holds row i0 + k)
-    float v[kBandRows + 2][M];
+    const R *b = th + base + lane * M;
+    // rows i0 - 1 .. i0 + BR (index k + 1 holds row i0 + k)
+    R v[BR + 2][M];
 #pragma unroll
-    for (int k = 0; k < kBandRows + 2; k++) load_cols<M>(b + (size_t)((i0 + k - 1) & (L - 1)) * L, v[k]);
-    float cd, sd;
-    twist_cs<float>(twist, r, cd, sd);
+    for (int k = 0; k < BR + 2; k++) load_cols<R, M>(b + (size_t)((i0 + k - 1) & (L - 1)) * L, v[k]);
+    R cd, sd;
+    twist_cs<R>(twist, r, cd, sd);
     double al, be, ga;
     kc.at(r, al, be, ga);
-    const float a = (float)al, bb = (float)be, gg = (float)ga;
-    float cc[M], sc[M], c2[M], s2[M];
+    const R a = (R)al, bb = (R)be, gg = (R)ga;
+    R cc[M], sc[M], c2[M], s2[M];
 #pragma unroll
     for (int m = 0; m < M; m++) {
-        float cu, su;  // row i0 - 1
-        sincos_r<float>(v[0][m], su, cu);
-        sincos_r<float>(v[1][m], sc[m], cc[m]);
-        bond_x_fma<float>(cu, su, cc[m], sc[m], cd, sd, c2[m], s2[m]);  // up bond (i-1, j) - (i, j)
+        R cu, su;  // row i0 - 1
+        sincos_r<R>(v[0][m], su, cu);
+        sincos_r<R>(v[1][m], sc[m], cc[m]);
+        bond_x_fma<R>(cu, su, cc[m], sc[m], cd, sd, c2[m], s2[m]);  // up bond (i-1, j) - (i, j)
     }
-    float *out = grad + base + (size_t)i0 * L + lane * M;
-    if (cd == 1.0f && sd == 0.0f && gg == 0.0f)  // the replica's values: uniform in the wavefront
-        grad_band_rows<M, false>(v, cc, sc, c2, s2, cd, sd, a, bb, gg, out, L);
+    R *out = grad + base + (size_t)i0 * L + lane * M;
+    if (cd == (R)1 && sd == (R)0 && gg == (R)0)  // the replica's values: uniform in the wavefront
+        grad_band_rows<R, M, false>(v, cc, sc, c2, s2, cd, sd, a, bb, gg, out, L);
     else
-        grad_band_rows<M, true>(v, cc, sc, c2, s2, cd, sd, a, bb, gg, out, L);
+        grad_band_rows<R, M, true>(v, cc, sc, c2, s2, cd, sd, a, bb, gg, out, L);
 }
 
-// float32 lattices of L = 64, 128, 256 take the band kernels
+// lattices of L = 64, 128, 256 take the band kernels
 static bool band_path(int L) { return L == 64 || L == 128 || L == 256; }
+template <class R>
+static int band_partials(int L) { return L / BandRows<R>::value; }
 
-static void terms_band(void *stream, const float *th, int R_, int L, const double *twist, double *partials) {
-    const dim3 grid((unsigned)(L / kBandRows), (unsigned)R_);
+template <class R>
+static void terms_band(void *stream, const R *th, int R_, int L, const double *twist, double *partials) {
+    const dim3 grid((unsigned)(L / BandRows<R>::value), (unsigned)R_);
     if (L == 64)
-        hipLaunchKernelGGL(xy_terms_band_kernel<1>, grid, dim3(64), 0, as_stream(stream), th, twist, partials);
+        hipLaunchKernelGGL((xy_terms_band_kernel<R, 1>), grid, dim3(64), 0, as_stream(stream), th, twist, partials);
     else if (L == 128)
-        hipLaunchKernelGGL(xy_terms_band_kernel<2>, grid, dim3(64), 0, as_stream(stream), th, twist, partials);
+        hipLaunchKernelGGL((xy_terms_band_kernel<R, 2>), grid, dim3(64), 0, as_stream(stream), th, twist, partials);
     else
-        hipLaunchKernelGGL(xy_terms_band_kernel<4>, grid, dim3(64), 0, as_stream(stream), th, twist, partials);
+        hipLaunchKernelGGL((xy_terms_band_kernel<R, 4>), grid, dim3(64), 0, as_stream(stream), th, twist, partials);
 }
-static void grad_band(void *stream, const float *th, int R_, int L, const double *twist, Coefs k, float *grad) {
-    const dim3 grid((unsigned)(L / kBandRows), (unsigned)R_);
+template <class R>
+static void grad_band(void *stream, const R *th, int R_, int L, const double *twist, Coefs k, R *grad) {
+    const dim3 grid((unsigned)(L / BandRows<R>::value), (unsigned)R_);
     if (L == 64)
-        hipLaunchKernelGGL(xy_grad_band_kernel<1>, grid, dim3(64), 0, as_stream(stream), th, twist, k, grad);
+        hipLaunchKernelGGL((xy_grad_band_kernel<R, 1>), grid, dim3(64), 0, as_stream(stream), th, twist, k, grad);
     else if (L == 128)
-        hipLaunchKernelGGL(xy_grad_band_kernel<2>, grid, dim3(64), 0, as_stream(stream), th, twist, k, grad);
+        hipLaunchKernelGGL((xy_grad_band_kernel<R, 2>), grid, dim3(64), 0, as_stream(stream), th, twist, k, grad);
     else
-        hipLaunchKernelGGL(xy_grad_band_kernel<4>, grid, dim3(64), 0, as_stream(stream), th, twist, k, grad);
+        hipLaunchKernelGGL((xy_grad_band_kernel<R, 4>), grid, dim3(64), 0, as_stream(stream), th, twist, k, grad);
 }
 
 // Whole-row kernels take L = 64, 128, ..., 1024 (threads = L)
@@ -817,16 +852,16 @@ template <class R, int COMP>
 static int terms_impl(void *stream, const void *state, int R_, int L, const double *twist, double *out) {
     MC_REQUIRE(L >= 2, "invalid lattice size %d", L);
     MC_REQUIRE(R_ >= 1, "invalid replica count %d", R_);
-    const bool band = COMP == 1 && sizeof(R) == 4 && band_path(L);
+    const bool band = COMP == 1 && band_path(L);
     const bool rows = COMP == 1 && !band && rows_path(L);
     const int ntiles = tiles_of(L);
-    const int nblk = band ? L / kBandRows : rows ? terms_rows_partials(L) : (ntiles + kTermsTiles - 1) / kTermsTiles;
+    const int nblk = band ? band_partials<R>(L) : rows ? terms_rows_partials(L) : (ntiles + kTermsTiles - 1) / kTermsTiles;
     void *base = nullptr;
     int rc = reduction_scratch(stream, (size_t)R_ * nblk * 3 * sizeof(double), &base);
     if (rc) return rc;
     double *partials = reinterpret_cast<double *>(base);
     if (band)
-        terms_band(stream, (const float *)state, R_, L, twist, partials);
+        terms_band<R>(stream, (const R *)state, R_, L, twist, partials);
     else if (rows)
         terms_rows<R>(stream, (const R *)state, R_, L, twist, partials);
     else
@@ -866,7 +901,9 @@ int mc_xy_twist_grad(void *stream, int dtype, const void *theta, int R, int L, c
     MC_REQUIRE(R >= 1 && L >= 2, "invalid shape R=%d L=%d", R, L);
     const Coefs k{alpha, beta, gamma};
     if (dtype == MC_F32 && band_path(L))
-        grad_band(stream, (const float *)theta, R, L, twist, k, (float *)grad);
+        grad_band<float>(stream, (const float *)theta, R, L, twist, k, (float *)grad);
+    else if (dtype == MC_F64 && band_path(L))
+        grad_band<double>(stream, (const double *)theta, R, L, twist, k, (double *)grad);
     else if (dtype == MC_F32 && rows_path(L))
         grad_rows<float>(stream, (const float *)theta, R, L, twist, k, (float *)grad);
     else if (dtype == MC_F64 && rows_path(L))

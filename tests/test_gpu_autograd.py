@@ -221,11 +221,12 @@ def test_float64_sincos_of_the_kernels_at_large_angles(gpu):
 @pytest.mark.parametrize("L", [64, 128, 256, 1024])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_whole_row_xy_kernels_against_closed_forms(gpu, L, dtype):
-    """The whole-row XY kernels (L a power of two from 64 to 1024: a workgroup per
-    16-row segment, rows by DPP shifts, the bonds across wavefront edges and the
-    periodic wrap from LDS; 1, 2, 4 and 16 wavefronts per row here) against the
-    bond sums and their gradient evaluated in numpy float64, with a twist per
-    replica."""
+    """The XY kernels of whole rows against the bond sums and their gradient
+    evaluated in numpy float64, with a twist per replica (zero for replica 0,
+    which takes the gradient's zero-twist loop): L = 64, 128, 256 run the band
+    kernels (one wavefront per band of rows, 1, 2 or 4 columns per lane, the
+    periodic row closed by DPP rotates), L = 1024 the whole-row workgroups (16
+    wavefronts per row, the bonds across wavefront edges from LDS)."""
     from autograd import lattice_energy
     R = 3
     th64 = _xy(R, L, torch.float64, 91 + L)
