@@ -67,6 +67,32 @@ def test_errors_map_to_reference_exception_types():
         _lib.check(status)
 
 
+def test_batch_launch_plan_checks_its_arguments_before_device_work():
+    """mc_batch_plan_create validates the shape as mc_batch_run does and leaves
+    no plan behind; mc_batch_plan_launch refuses a NULL plan; destroying NULL
+    is a no-op."""
+    from mcgpu import _lib
+    lib = _lib.load()
+    plan = ctypes.c_void_p(1234)
+    nulls = [None] * 14
+    status = lib.mc_batch_plan_create(None, 2, 128, 32, 12, 512, *nulls[:3], 1.0, 0.0, *nulls[:2], 10,
+                                      *nulls[:7], *nulls[:3], None, None, ctypes.byref(plan))
+    with pytest.raises(ValueError, match="divide"):
+        _lib.check(status)
+    assert plan.value is None
+    status = lib.mc_batch_plan_create(None, 2, 128, 32, 16, 384, *nulls[:3], 1.0, 0.0, *nulls[:2], 10,
+                                      *nulls[:7], *nulls[:3], None, None, ctypes.byref(plan))
+    with pytest.raises(ValueError, match="256 or 512"):
+        _lib.check(status)
+    status = lib.mc_batch_plan_create(None, 2, 128, 32, 16, 512, *nulls[:3], 1.0, 0.0, *nulls[:2], 10,
+                                      *nulls[:7], *nulls[:3], None, None, ctypes.byref(plan))
+    with pytest.raises(ValueError, match="sigmas"):
+        _lib.check(status)
+    with pytest.raises(ValueError, match="plan"):
+        _lib.check(lib.mc_batch_plan_launch(None, 10, 0, 0, 0, 0, None, None))
+    assert lib.mc_batch_plan_destroy(None) == 0
+
+
 def test_runtime_validation_precedes_device_allocation():
     from core.runtime import Runtime
     from models.ising.model import IsingModel
