@@ -205,55 +205,26 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
     const int row0 = s * H;  // first global row of the strip
     const bool use_pt = a.exchange_interval > 0;
 
-    // ---- load the strip's state: rows 0..H+2 (global rows row0-1 .. row0+H+1),
-    // the loads of kLoadBatch sites per thread issued before any is used
-    constexpr int kLoadBatch = 8;
-    for (int base = tid; base < (H + 3) * L; base += NT * kLoadBatch) {
-        float thv[kLoadBatch];
-        int spv[kLoadBatch];
-#pragma unroll
-        for (int b = 0; b < kLoadBatch; b++) {
-            const int idx = base + b * NT;
-            if (idx >= (H + 3) * L) continue;
-            const int lr = idx / L, j = idx - lr * L;
-            const size_t gk = rep + (size_t)((row0 + lr - 1 + L) % L) * L + j;
-            if (MODEL == MODEL_ISING)
-                spv[b] = reinterpret_cast<const int8_t *>(a.state)[gk];
-            else
-                thv[b] = reinterpret_cast<const float *>(a.state)[gk];
-        }
-#pragma unroll
-        for (int b = 0; b < kLoadBatch; b++) {
-            const int idx = base + b * NT;
-            if (idx >= (H + 3) * L) continue;
-            const int lr = idx / L, j = idx - lr * L;
-            if (MODEL == MODEL_ISING) {
-                lsp[lds_index(lr, j, L)] = (int8_t)spv[b];
-            } else {
-                const float th = thv[b];
-                float sv, cv;
-                fast_sincosf(th, sv, cv);
-                lcs[lds_index(lr, j, L)] = make_float2(cv, sv);
-                if (lr >= 1 && lr <= H + 1) lth[lds_index(lr - 1, j, L)] = th;
-            }
-        }
-    }
-    if (use_pt) {
-        for (int k = tid; k < R; k += NT) ladder[k] = a.betas[a.t2w[k]];
-        for (int k = tid; k < 2 * (R - 1); k += NT) pt_g[k] = a.pt_rng[k];
-        if (s == 0 && tid < 5) my_trips[tid] = a.trips[tid * R + r];
-    }
-    // this walker's beta, ladder temperature and proposal width (the same in
-    // every thread of the workgroup)
+    // ---- every load that no other load feeds is issued before the strip's:
+    // the walker's PT state, the first chunk of the shared PT streams, ladder
+    // and widths (into registers, stored to LDS after the strip), and the RNG
+    // streams of the thread's sites.  Loads complete in order, so the strip
+    // load's wait covers them: one memory round trip at the start instead of
+    // one per dependent group.
     float my_beta = a.betas[r];
     int my_t = use_pt ? a.w2t[r] : r;
-    if (MODEL == MODEL_XY_GAUSS)
-        for (int k = tid; k < R; k += NT) sig_tab[k] = a.sigmas[k];
-    float sigma = MODEL == MODEL_XY_GAUSS ? a.sigmas[my_t] : 0.0f;
-    if (MODEL == MODEL_ISING && tid < 5) thr[tid] = ising_threshold(4 * tid - 8, my_beta, a.J);
     int parity = use_pt ? *a.parity : 0;
-    bool exchanged = false;  // my_e holds this walker's energy at this launch's last exchange
-    double my_e = 0.0;       // (thread 0)
+    float pt_beta = 0.0f, pt_sigma = 0.0f;
+    int pt_walker_t = 0;
+    uint64_t pt_stream = 0;
+    long long pt_trip = 0;
+    if (use_pt && tid < R) {  // walker tid's beta and ladder temperature (betas are in walker order)
+        pt_beta = a.betas[tid];
+        pt_walker_t = a.w2t[tid];
+    }
+    if (use_pt && tid < 2 * (R - 1)) pt_stream = a.pt_rng[tid];
+    if (use_pt && s == 0 && tid < 5) pt_trip = a.trips[tid * R + r];
+    if (MODEL == MODEL_XY_GAUSS && tid < R) pt_sigma = a.sigmas[tid];
 
     const float J = a.J, delta = a.delta;
     int n_acc = 0;  // accepted moves of this thread since the last flush
@@ -324,9 +295,60 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs a) {
                 slot_pre[p - 2][m] = 0;
             }
         }
+    // ---- load the strip's state: rows 0..H+2 (global rows row0-1 .. row0+H+1),
+    // the loads of kLoadBatch sites per thread issued before any is used
+    constexpr int kLoadBatch = 8;
+    for (int base = tid; base < (H + 3) * L; base += NT * kLoadBatch) {
+        float thv[kLoadBatch];
+        int spv[kLoadBatch];
+#pragma unroll
+        for (int b = 0; b < kLoadBatch; b++) {
+            const int idx = base + b * NT;
+            if (idx >= (H + 3) * L) continue;
+            const int lr = idx / L, j = idx - lr * L;
+            const size_t gk = rep + (size_t)((row0 + lr - 1 + L) % L) * L + j;
+            if (MODEL == MODEL_ISING)
+                spv[b] = reinterpret_cast<const int8_t *>(a.state)[gk];
+            else
+                thv[b] = reinterpret_cast<const float *>(a.state)[gk];
+        }
+#pragma unroll
+        for (int b = 0; b < kLoadBatch; b++) {
+            const int idx = base + b * NT;
+            if (idx >= (H + 3) * L) continue;
+            const int lr = idx / L, j = idx - lr * L;
+            if (MODEL == MODEL_ISING) {
+                lsp[lds_index(lr, j, L)] = (int8_t)spv[b];
+            } else {
+                const float th = thv[b];
+                float sv, cv;
+                fast_sincosf(th, sv, cv);
+                lcs[lds_index(lr, j, L)] = make_float2(cv, sv);
+                if (lr >= 1 && lr <= H + 1) lth[lds_index(lr - 1, j, L)] = th;
+            }
+        }
+    }
+    // the PT state into LDS: ladder[w2t[w]] = betas[w] (w2t is a permutation),
+    // the PT streams, the round-trip record and the widths
+    if (use_pt) {
+        if (tid < R) ladder[pt_walker_t] = pt_beta;
+        for (int k = tid + NT; k < R; k += NT) ladder[a.w2t[k]] = a.betas[k];
+        if (tid < 2 * (R - 1)) pt_g[tid] = pt_stream;
+        for (int k = tid + NT; k < 2 * (R - 1); k += NT) pt_g[k] = a.pt_rng[k];
+        if (s == 0 && tid < 5) my_trips[tid] = pt_trip;
+    }
+    if (MODEL == MODEL_XY_GAUSS) {
+        if (tid < R) sig_tab[tid] = pt_sigma;
+        for (int k = tid + NT; k < R; k += NT) sig_tab[k] = a.sigmas[k];
+    }
+    if (MODEL == MODEL_ISING && tid < 5) thr[tid] = ising_threshold(4 * tid - 8, my_beta, a.J);
+    bool exchanged = false;  // my_e holds this walker's energy at this launch's last exchange
+    double my_e = 0.0;       // (thread 0)
+
     if (MODEL == MODEL_XY_GAUSS) normal_table_k_to_lds(q_tab);
     if (tid == 0) s_err = 0;
     __syncthreads();
+    float sigma = MODEL == MODEL_XY_GAUSS ? sig_tab[my_t] : 0.0f;  // this walker's proposal width
     if (MODEL != MODEL_ISING) {
 #pragma unroll
         for (int p = 0; p < NP; p++)
