@@ -782,8 +782,13 @@ __global__ __launch_bounds__(64) void xy_grad_band_kernel(const R *__restrict__ 
         grad_band_rows<R, M, true>(v, cc, sc, c2, s2, cd, sd, a, bb, gg, out, L);
 }
 
-// lattices of L = 64, 128, 256 take the band kernels
-static bool band_path(int L) { return L == 64 || L == 128 || L == 256; }
+// lattices of L = 64, 128, 256 take the band kernels, whose rows are read and
+// written in 16-byte vectors: arrays that do not start on a 16-byte boundary
+// (a view at an odd offset) take the whole-row kernels' scalar accesses
+static bool band_path(int L, const void *a, const void *b = nullptr) {
+    const bool aligned = ((uintptr_t)a % 16 == 0) && ((uintptr_t)b % 16 == 0);
+    return (L == 64 || L == 128 || L == 256) && aligned;
+}
 template <class R>
 static int band_partials(int L) { return L / BandRows<R>::value; }
 
@@ -852,7 +857,7 @@ template <class R, int COMP>
 static int terms_impl(void *stream, const void *state, int R_, int L, const double *twist, double *out) {
     MC_REQUIRE(L >= 2, "invalid lattice size %d", L);
     MC_REQUIRE(R_ >= 1, "invalid replica count %d", R_);
-    const bool band = COMP == 1 && band_path(L);
+    const bool band = COMP == 1 && band_path(L, state);
     const bool rows = COMP == 1 && !band && rows_path(L);
     const int ntiles = tiles_of(L);
     const int nblk = band ? band_partials<R>(L) : rows ? terms_rows_partials(L) : (ntiles + kTermsTiles - 1) / kTermsTiles;
@@ -900,9 +905,9 @@ int mc_xy_twist_grad(void *stream, int dtype, const void *theta, int R, int L, c
                      const double *alpha, const double *beta, const double *gamma, void *grad) {
     MC_REQUIRE(R >= 1 && L >= 2, "invalid shape R=%d L=%d", R, L);
     const Coefs k{alpha, beta, gamma};
-    if (dtype == MC_F32 && band_path(L))
+    if (dtype == MC_F32 && band_path(L, theta, grad))
         grad_band<float>(stream, (const float *)theta, R, L, twist, k, (float *)grad);
-    else if (dtype == MC_F64 && band_path(L))
+    else if (dtype == MC_F64 && band_path(L, theta, grad))
         grad_band<double>(stream, (const double *)theta, R, L, twist, k, (double *)grad);
     else if (dtype == MC_F32 && rows_path(L))
         grad_rows<float>(stream, (const float *)theta, R, L, twist, k, (float *)grad);

@@ -186,6 +186,30 @@ def test_float32_terms_accurate_for_angles_of_many_turns(gpu):
     assert (g32.double() - g64).abs().max() < 3e-4
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_an_unaligned_state_view_takes_the_scalar_kernels(gpu, dtype):
+    """The band kernels read and write rows in 16-byte vectors; a state view
+    that starts one element past a 16-byte boundary takes the whole-row
+    kernels' scalar accesses instead, with the same energy and gradient as the
+    aligned copy (to each kernel pair's rounding)."""
+    from autograd import lattice_energy
+    R, L = 3, 128
+    base = _xy(R, L, dtype, 5)
+    flat = torch.empty(R * L * L + 1, dtype=dtype, device=_dev())
+    flat[1:] = base.reshape(-1)
+    view = flat[1:].view(R, L, L)
+    assert view.data_ptr() % 16 != 0 and base.data_ptr() % 16 == 0
+    out = []
+    for s in (base, view):
+        x = s.detach().requires_grad_(True)
+        H = lattice_energy(x, 0.8, torch.tensor([0.0, 0.3, -1.1], dtype=torch.float64, device=_dev()), "xy")
+        (g,) = torch.autograd.grad(H.sum(), x)
+        out.append((H.detach().cpu().numpy(), g.double().cpu().numpy()))
+    tol_h, tol_g = (1e-3 * L, 2e-5) if dtype == torch.float32 else (1e-10, 1e-12)
+    np.testing.assert_allclose(out[1][0], out[0][0], rtol=0, atol=tol_h)
+    np.testing.assert_allclose(out[1][1], out[0][1], rtol=0, atol=tol_g)
+
+
 def test_float64_sincos_of_the_kernels_at_large_angles(gpu):
     """The float64 kernels' own sine and cosine (a three-part reduction by pi/2
     and fdlibm's polynomials) at angles spread over +-1e6 rad: the energy and
